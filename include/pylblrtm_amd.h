@@ -1,0 +1,139 @@
+/*
+ * pylblrtm_amd — C ABI of the MI355X line-by-line optical-depth library
+ * (libpylblrtm_amd.so, HIP kernels built for gfx950).
+ *
+ * The library replaces the body of PyLBLRTM's private seam
+ *
+ *     _build_optical_depth(nu, sd, layers, line_cutoff_cm1, tile_width_cm1)
+ *         -> od[n_layers, n_nu]                       (pylblrtm/forward.py:157-317)
+ *
+ * and the column/transmittance assembly its two callers do with it
+ *
+ *     T = exp(-od.sum(axis=0) / mu_view)              (pylblrtm/forward.py:151,
+ *                                                      pylblrtm/config.py:266-273)
+ *
+ * The host keeps the reference's prelude (grouping by (mol_id, iso_id) in
+ * sorted order, warnings, the band mask of forward.py:215-220, molar-mass
+ * fallback) and hands the device the surviving lines.
+ *
+ * Conventions
+ *  - Every array pointer in the structs below is DEVICE memory owned by the
+ *    caller; the library never allocates or frees caller memory.  Scratch
+ *    lives in a caller-provided workspace sized by lbl_workspace_bytes().
+ *  - Calls are host-synchronous (they return after the work on `stream`,
+ *    a hipStream_t, has finished), like the reference function they replace.
+ *    Calls on the same stream from several threads must be serialised.
+ *  - Return codes: LBL_OK, or
+ *      LBL_EINVAL     invalid argument                  -> ValueError
+ *      LBL_ESTRENGTH  non-finite / negative strength     -> FloatingPointError
+ *                     (forward.py:280-283; err_layer/err_group say where)
+ *      LBL_ENONFINITE non-finite optical depth           -> FloatingPointError
+ *                     (forward.py:314-315; err_layer says where)
+ *      LBL_EHIP       HIP runtime error                  -> RuntimeError
+ *    lbl_last_error() returns a thread-local message for the last failure.
+ */
+#ifndef PYLBLRTM_AMD_H
+#define PYLBLRTM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LBL_OK = 0,
+  LBL_EINVAL = 1,
+  LBL_ESTRENGTH = 2,
+  LBL_ENONFINITE = 3,
+  LBL_EHIP = 4
+};
+
+/* Arithmetic of the Voigt line sum. */
+enum {
+  LBL_FP64 = 0,      /* FP64 throughout (tau to ~1e-12 relative of the FP64 oracle) */
+  LBL_MIXED = 1,     /* FP32 line sum, FP64 centre offset + Boltzmann exponent */
+  LBL_SURROGATE = 2  /* LBL_MIXED + MLP surrogate for K(x,y) in its trained domain */
+};
+
+/* Lines that survive the host prelude, SORTED BY nu0 ASCENDING.
+ * Fields are those of SpectralData.lines (pylblrtm/spectral_data.py:21-26,
+ * pylblrtm/hitran_manager.py:314-324); `group` replaces (mol_id, iso_id) by
+ * the group's position in the host's sorted group list. */
+typedef struct lbl_lines {
+  int64_t n_lines;
+  const double *nu0, *sref, *g_air, *g_self, *elow, *n_air, *shift;
+  const int32_t *group;
+} lbl_lines_t;
+
+/* One entry per (mol_id, iso_id) group, in the reference's sorted order
+ * (forward.py:194).  TIPS tables (SpectralData.tips) are concatenated:
+ * group g owns tips_t/tips_q[tips_offset[g] .. tips_offset[g+1]). */
+typedef struct lbl_groups {
+  int32_t n_groups;
+  const int32_t *gas;          /* row of lbl_layers.vmr / .col used by the group */
+  const double *molar_mass;    /* g/mol (28.97 fallback already applied) */
+  const int32_t *tips_offset;  /* n_groups + 1 entries */
+  const double *tips_t, *tips_q;
+} lbl_groups_t;
+
+/* Layer averages of AtmosphericProfile.build_layers (atmosphere.py:110-141). */
+typedef struct lbl_layers {
+  int32_t n_layers, n_gases;
+  const double *p_atm, *t_k;   /* n_layers */
+  const double *vmr, *col;     /* n_gases x n_layers, row-major by gas */
+} lbl_layers_t;
+
+/* The wavenumber grid nu_i = nu_min + i*step (multiply then add, as
+ * forward.py:141-142) and the reference tiling (forward.py:175-176,295-301). */
+typedef struct lbl_grid {
+  double nu_min, step;
+  int64_t n_nu;
+  int64_t tile_samples;        /* max(1, floor(tile_width/(nu[1]-nu[0]) + 1e-8)) */
+  double cutoff;               /* line_cutoff_cm1 */
+  double shift_bound;          /* >= max over lines of |shift| * max_k P_k, the
+                                  bound of forward.py:190-191 (widens the candidate
+                                  line range per tile; inclusion stays exact) */
+} lbl_grid_t;
+
+/* Outputs.  `transmittance` is n_mu x n_nu; optional outputs may be NULL.
+ * Element type: double for LBL_FP64, float otherwise. */
+typedef struct lbl_outputs {
+  int32_t n_mu;
+  const double *mu;            /* HOST array of n_mu viewing cosines in (0, 1] */
+  void *transmittance;         /* exp(-sum_k od_k / mu_m)            (forward.py:151) */
+  void *optical_depth;         /* sum_k od_k (vertical), or NULL     (config.py:270) */
+  void *optical_depth_layers;  /* od[n_layers][n_nu], or NULL        (config.py:273) */
+} lbl_outputs_t;
+
+/* Bytes of scratch lbl_optical_depth() needs for this problem. */
+int lbl_workspace_bytes(const lbl_lines_t *lines, const lbl_layers_t *layers,
+                        const lbl_grid_t *grid, int precision, size_t *bytes);
+
+/* The whole path: per-(layer, line) scaling, tile binning, Voigt line sum,
+ * radiation term, layer sum and exp(-tau/mu).  Replaces
+ * _build_optical_depth (forward.py:157-317) plus forward.py:151.
+ * On LBL_ESTRENGTH / LBL_ENONFINITE, *err_layer and *err_group (host ints,
+ * may be NULL) receive the layer index and group index of the first failure
+ * in the reference's loop order (layer, then group). */
+int lbl_optical_depth(const lbl_lines_t *lines, const lbl_groups_t *groups,
+                      const lbl_layers_t *layers, const lbl_grid_t *grid, int precision,
+                      const lbl_outputs_t *out, void *workspace, size_t workspace_bytes,
+                      int32_t *err_layer, int32_t *err_group, void *stream);
+
+/* Evaluate the device Voigt function K(x, y) = Re w(x + iy) on n device
+ * points with the region dispatch the line sum uses (precision LBL_FP64:
+ * double in/out; LBL_MIXED: float in/out).  Stands in for
+ * scipy.special.wofz(z).real at forward.py:62 in known-answer tests. */
+int lbl_voigt_eval(int64_t n, const void *x, const void *y, void *k_out, int precision,
+                   void *stream);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *lbl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYLBLRTM_AMD_H */

@@ -1,0 +1,26 @@
+"""pylblrtm_amd — MI355X-native line-by-line optical depth.
+
+Drop-in for PyLBLRTM's forward path (pylblrtm/__init__.py:18-24 names):
+``SpectralData``, ``AtmosphericProfile``, ``compute_transmittance``,
+``run_forward``, ``load_config``, ``ForwardResult`` and ``MOLECULE_IDS``.
+The optical-depth work runs in hand-written HIP kernels for gfx950
+(libpylblrtm_amd.so); data ingestion from HITRAN is not part of this package.
+"""
+
+from .spectral_data import SpectralData
+from .atmosphere import AtmosphericProfile
+from .forward import compute_transmittance
+from .config import run_forward, load_config, ForwardResult
+from .molecules import MOLECULE_IDS
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "SpectralData",
+    "AtmosphericProfile",
+    "compute_transmittance",
+    "run_forward",
+    "load_config",
+    "ForwardResult",
+    "MOLECULE_IDS",
+]

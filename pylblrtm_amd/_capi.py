@@ -1,0 +1,97 @@
+"""ctypes binding of libpylblrtm_amd.so (the C ABI declared in
+include/pylblrtm_amd.h) and the mapping of its status codes onto the
+exceptions the reference raises.
+
+There is no CPU fallback: if the library is missing or no ROCm device is
+visible, every compute entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Optional
+
+LIB_PATH = Path(__file__).resolve().parent / "libpylblrtm_amd.so"
+
+LBL_OK, LBL_EINVAL, LBL_ESTRENGTH, LBL_ENONFINITE, LBL_EHIP = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": 0, "mixed": 1, "surrogate": 2}
+
+EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error")
+
+_vp = ctypes.c_void_p
+
+
+class LblLines(ctypes.Structure):
+    _fields_ = [("n_lines", ctypes.c_int64), ("nu0", _vp), ("sref", _vp), ("g_air", _vp),
+                ("g_self", _vp), ("elow", _vp), ("n_air", _vp), ("shift", _vp), ("group", _vp)]
+
+
+class LblGroups(ctypes.Structure):
+    _fields_ = [("n_groups", ctypes.c_int32), ("gas", _vp), ("molar_mass", _vp),
+                ("tips_offset", _vp), ("tips_t", _vp), ("tips_q", _vp)]
+
+
+class LblLayers(ctypes.Structure):
+    _fields_ = [("n_layers", ctypes.c_int32), ("n_gases", ctypes.c_int32), ("p_atm", _vp),
+                ("t_k", _vp), ("vmr", _vp), ("col", _vp)]
+
+
+class LblGrid(ctypes.Structure):
+    _fields_ = [("nu_min", ctypes.c_double), ("step", ctypes.c_double), ("n_nu", ctypes.c_int64),
+                ("tile_samples", ctypes.c_int64), ("cutoff", ctypes.c_double),
+                ("shift_bound", ctypes.c_double)]
+
+
+class LblOutputs(ctypes.Structure):
+    _fields_ = [("n_mu", ctypes.c_int32), ("mu", ctypes.POINTER(ctypes.c_double)),
+                ("transmittance", _vp), ("optical_depth", _vp), ("optical_depth_layers", _vp)]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises LibraryMissing if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise LibraryMissing(
+            f"{LIB_PATH.name} is not built; run `python -m pylblrtm_amd.build` "
+            f"(the HIP path has no CPU fallback)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.POINTER
+    lib.lbl_workspace_bytes.argtypes = [P(LblLines), P(LblLayers), P(LblGrid), ctypes.c_int,
+                                        P(ctypes.c_size_t)]
+    lib.lbl_workspace_bytes.restype = ctypes.c_int
+    lib.lbl_optical_depth.argtypes = [P(LblLines), P(LblGroups), P(LblLayers), P(LblGrid),
+                                      ctypes.c_int, P(LblOutputs), _vp, ctypes.c_size_t,
+                                      P(ctypes.c_int32), P(ctypes.c_int32), _vp]
+    lib.lbl_optical_depth.restype = ctypes.c_int
+    lib.lbl_voigt_eval.argtypes = [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, _vp]
+    lib.lbl_voigt_eval.restype = ctypes.c_int
+    lib.lbl_last_error.argtypes = []
+    lib.lbl_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().lbl_last_error().decode(errors="replace")
+
+
+def raise_for(status: int, what: str) -> None:
+    """Map a non-strength status code to the reference's exception types."""
+    if status == LBL_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if status == LBL_EINVAL:
+        raise ValueError(msg)
+    if status in (LBL_ESTRENGTH, LBL_ENONFINITE):
+        raise FloatingPointError(msg)
+    raise RuntimeError(msg)

@@ -1,0 +1,206 @@
+// extern "C" entry points of libpylblrtm_amd (declared in include/pylblrtm_amd.h).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "internal.h"
+
+namespace {
+
+thread_local char g_last_error[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) return fail(LBL_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct Layout {
+  size_t c, s, y, a, lo, hi, err, total;
+  int64_t n_tiles;
+};
+
+bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl_grid_t* grid,
+                 int precision, Layout* L) {
+  if (!lines || !layers || !grid) return false;
+  if (grid->tile_samples <= 0 || grid->n_nu <= 0) return false;
+  const size_t nrec = (size_t)std::max<int64_t>(lines->n_lines, 0) *
+                      (size_t)std::max<int32_t>(layers->n_layers, 0);
+  const size_t es = precision == LBL_FP64 ? 8 : 4;
+  L->n_tiles = (grid->n_nu + grid->tile_samples - 1) / grid->tile_samples;
+  size_t off = 0;
+  L->c = off; off = align_up(off + nrec * 8);
+  L->s = off; off = align_up(off + nrec * es);
+  L->y = off; off = align_up(off + nrec * es);
+  L->a = off; off = align_up(off + nrec * es);
+  L->lo = off; off = align_up(off + (size_t)L->n_tiles * 4);
+  L->hi = off; off = align_up(off + (size_t)L->n_tiles * 4);
+  L->err = off; off = align_up(off + 4);
+  L->total = off;
+  return true;
+}
+
+int check_inputs(const lbl_lines_t* lines, const lbl_groups_t* groups, const lbl_layers_t* layers,
+                 const lbl_grid_t* grid, int precision) {
+  if (!lines || !groups || !layers || !grid) return fail(LBL_EINVAL, "null argument");
+  if (precision != LBL_FP64 && precision != LBL_MIXED && precision != LBL_SURROGATE)
+    return fail(LBL_EINVAL, "unknown precision %d", precision);
+  if (precision == LBL_SURROGATE)
+    return fail(LBL_EINVAL, "precision LBL_SURROGATE needs lbl_optical_depth_surrogate");
+  if (lines->n_lines < 0 || lines->n_lines > INT32_MAX)
+    return fail(LBL_EINVAL, "n_lines %lld out of range", (long long)lines->n_lines);
+  if (layers->n_layers < 0 || layers->n_layers > (1 << 19))
+    return fail(LBL_EINVAL, "n_layers %d out of range", layers->n_layers);
+  if (groups->n_groups < 0 || groups->n_groups > lbl::kMaxGroups)
+    return fail(LBL_EINVAL, "n_groups %d exceeds %d", groups->n_groups, lbl::kMaxGroups);
+  if (grid->n_nu < 1 || grid->tile_samples < 1)
+    return fail(LBL_EINVAL, "empty grid or tile");
+  if (!(grid->step > 0.0) || !isfinite(grid->nu_min) || !isfinite(grid->cutoff))
+    return fail(LBL_EINVAL, "invalid grid parameters");
+  if (lines->n_lines > 0) {
+    if (!lines->nu0 || !lines->sref || !lines->g_air || !lines->g_self || !lines->elow ||
+        !lines->n_air || !lines->shift || !lines->group)
+      return fail(LBL_EINVAL, "null line array");
+    if (groups->n_groups < 1 || !groups->gas || !groups->molar_mass || !groups->tips_offset ||
+        !groups->tips_t || !groups->tips_q)
+      return fail(LBL_EINVAL, "null group array");
+  }
+  if (layers->n_layers > 0 && (!layers->p_atm || !layers->t_k))
+    return fail(LBL_EINVAL, "null layer array");
+  if (lines->n_lines > 0 && layers->n_layers > 0 && (!layers->vmr || !layers->col))
+    return fail(LBL_EINVAL, "null vmr/col array");
+  return LBL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbl_last_error(void) { return g_last_error; }
+
+int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
+                        const lbl_grid_t* grid, int precision, size_t* bytes) {
+  Layout L;
+  if (!bytes || !make_layout(lines, layers, grid, precision, &L))
+    return fail(LBL_EINVAL, "invalid arguments to lbl_workspace_bytes");
+  *bytes = L.total;
+  return LBL_OK;
+}
+
+int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                      const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
+                      const lbl_outputs_t* out, void* workspace, size_t workspace_bytes,
+                      int32_t* err_layer, int32_t* err_group, void* stream) {
+  g_last_error[0] = '\0';
+  int rc = check_inputs(lines, groups, layers, grid, precision);
+  if (rc != LBL_OK) return rc;
+  if (!out || out->n_mu < 1 || out->n_mu > lbl::kMaxMu || !out->mu || !out->transmittance)
+    return fail(LBL_EINVAL, "outputs: need 1..%d viewing cosines and a transmittance buffer",
+                lbl::kMaxMu);
+  for (int m = 0; m < out->n_mu; ++m)
+    if (!(out->mu[m] > 0.0 && out->mu[m] <= 1.0))
+      return fail(LBL_EINVAL, "mu_view must be in (0, 1]");
+  Layout L;
+  make_layout(lines, layers, grid, precision, &L);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(LBL_EINVAL, "workspace of %zu bytes needed, %zu given", L.total, workspace_bytes);
+  if (L.n_tiles > INT32_MAX) return fail(LBL_EINVAL, "too many tiles");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char* ws = static_cast<char*>(workspace);
+
+  lbl::Records rec{reinterpret_cast<double*>(ws + L.c), ws + L.s, ws + L.y, ws + L.a};
+  uint32_t* err = reinterpret_cast<uint32_t*>(ws + L.err);
+  int32_t* lo = reinterpret_cast<int32_t*>(ws + L.lo);
+  int32_t* hi = reinterpret_cast<int32_t*>(ws + L.hi);
+
+  HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
+
+  lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err};
+  HIP_TRY(lbl::launch_line_params(lp, precision, s));
+
+  // |centre - nu0| <= |shift| * P_air <= shift_bound: the candidate range per
+  // tile is widened by it (and a rounding slack); the per-record test is exact.
+  lbl::TileRangeArgs tr{};
+  tr.nu0 = lines->nu0;
+  tr.n_lines = lines->n_lines;
+  tr.nu_min = grid->nu_min;
+  tr.step = grid->step;
+  tr.cutoff = grid->cutoff;
+  tr.n_nu = grid->n_nu;
+  tr.ts = grid->tile_samples;
+  tr.n_tiles = L.n_tiles;
+  tr.lo = lo;
+  tr.hi = hi;
+  tr.margin = fabs(grid->shift_bound) * (1.0 + 1e-9) + 1e-9;
+  HIP_TRY(lbl::launch_tile_ranges(tr, s));
+
+  lbl::SumArgs sa{};
+  sa.rec = rec;
+  sa.lo = lo;
+  sa.hi = hi;
+  sa.t_k = layers->t_k;
+  sa.n_lines = lines->n_lines;
+  sa.n_layers = layers->n_layers;
+  sa.nu_min = grid->nu_min;
+  sa.step = grid->step;
+  sa.cutoff = grid->cutoff;
+  sa.n_nu = grid->n_nu;
+  sa.ts = grid->tile_samples;
+  sa.n_tiles = L.n_tiles;
+  const int64_t pts = (int64_t)lbl::kBlock * lbl::sum_points_per_thread(precision);
+  sa.slices_per_tile = (int32_t)((grid->tile_samples + pts - 1) / pts);
+  sa.n_wg = L.n_tiles * sa.slices_per_tile;
+  if (sa.n_wg > INT32_MAX) return fail(LBL_EINVAL, "grid too large");
+  sa.n_mu = out->n_mu;
+  for (int m = 0; m < out->n_mu; ++m) sa.inv_mu[m] = 1.0 / out->mu[m];
+  sa.trans = out->transmittance;
+  sa.tau = out->optical_depth;
+  sa.tau_layers = out->optical_depth_layers;
+  sa.err = err;
+  HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));
+
+  uint32_t code = lbl::kErrNone;
+  HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (code != lbl::kErrNone) {
+    const int kind2 = (int)(code >> lbl::kErrGroupBits);
+    const int layer = kind2 >> 1;
+    const int group = (int)(code & ((1u << lbl::kErrGroupBits) - 1));
+    if (err_layer) *err_layer = layer;
+    if (err_group) *err_group = (kind2 & 1) ? -1 : group;
+    if (kind2 & 1) return fail(LBL_ENONFINITE, "Layer %d: non-finite optical depth", layer);
+    return fail(LBL_ESTRENGTH, "Layer %d, group %d: non-finite or negative line strength",
+                layer, group);
+  }
+  return LBL_OK;
+}
+
+int lbl_voigt_eval(int64_t n, const void* x, const void* y, void* k_out, int precision,
+                   void* stream) {
+  g_last_error[0] = '\0';
+  if (n < 0 || (n > 0 && (!x || !y || !k_out)))
+    return fail(LBL_EINVAL, "invalid arguments to lbl_voigt_eval");
+  if (precision != LBL_FP64 && precision != LBL_MIXED)
+    return fail(LBL_EINVAL, "unknown precision %d", precision);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(lbl::launch_voigt_eval(n, x, y, k_out, precision, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LBL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// Device Voigt function K(x, y) = Re w(x + iy), y >= 0, in FP64 and FP32.
+//
+// w(z) = exp(-z^2) erfc(-iz) is the Faddeeva function the reference evaluates
+// with scipy.special.wofz (pylblrtm/forward.py:22,62).  Everything here is
+// written from published mathematics:
+//
+// * Far region: n-point Gauss-Hermite quadrature of
+//     w(z) = (i/pi) * integral exp(-t^2) / (z - t) dt,
+//   i.e. the even truncations of the Laplace continued fraction.  Grouping the
+//   symmetric nodes +-t with weight W gives, with r = x^2 + y^2,
+//     Re[(i/pi) W (1/(z-t) + 1/(z+t))] = (2 y W / pi) (r + t^2) / D,
+//     D = (x^2 - y^2 - t^2)^2 + 4 x^2 y^2,
+//   a strictly positive multiple of y, so K keeps its relative accuracy on
+//   the real axis.  Measured relative error in K (against mpmath):
+//     2 nodes (Humlicek's region-1 form) 2.5/|z|^4, 4 nodes 13.5/|z|^8,
+//     8 nodes < 1e-15 for |z| >= 15.
+// * Core region: the trapezoid rule with step h on the same integral plus the
+//   residue of its pole at t = z (Chiarella & Reichel, Math. Comp. 22 (1968)
+//   137; Matta & Reichel, Math. Comp. 25 (1971) 339), truncation error
+//   ~exp(-pi^2/h^2).  The node lattice (t = n h or t = (n + 1/2) h) is picked
+//   per point so that x stays at least h/4 from every node, which keeps the
+//   pole terms and the residue from cancelling.  With p = exp(2 pi i z / h),
+//   rho = |p| = exp(-2 pi y / h) and sg = +1 for the half-shifted lattice,
+//   -1 for the integer one, the residue adds (for y < pi/h)
+//     Re C = 2 M (sg cos(theta - 2xy) + rho cos 2xy) / (1 + 2 sg rho cos theta + rho^2),
+//     M = exp(y^2 - x^2 - 2 pi y / h), theta = 2 pi x / h,
+//   which reduces to exp(-x^2) on the real axis.
+//   FP64: h = 0.5, |t| <= 6.3 -> ~1e-15 relative.  FP32: h = 0.8,
+//   |t| <= 4.2 -> ~3e-6 relative including FP32 rounding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lbl {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInvPi = 0.31830988618379067154;
+constexpr double kInvSqrtPi = 0.56418958354775628695;
+
+// Region thresholds on r = x^2 + y^2.
+constexpr double kFp64Gh4R2 = 3600.0;   // |z| >= 60: 4 nodes, rel err <= 1e-13
+constexpr double kFp64Gh8R2 = 225.0;    // |z| >= 15: 8 nodes, rel err <= 1e-15
+constexpr float kF32Gh2R2 = 625.0f;     // |z| >= 25: 2 nodes, rel err <= 6.4e-6
+constexpr float kF32Gh4R2 = 64.0f;      // |z| >= 8 : 4 nodes, rel err <= 1e-6
+
+// Gauss-Hermite positive nodes / weights (numpy.polynomial.hermite.hermgauss).
+constexpr double kGh2T = 0.7071067811865475, kGh2W = 0.8862269254527579;
+constexpr double kGh4T[2] = {0.5246476232752904, 1.6506801238857847};
+constexpr double kGh4W[2] = {0.8049140900055127, 0.08131283544724519};
+constexpr double kGh8T[4] = {0.3811869902073221, 1.1571937124467802, 1.981656756695843,
+                             2.930637420257244};
+constexpr double kGh8W[4] = {0.6611470125582415, 0.20780232581489183, 0.017077983007413467,
+                             0.00019960407221136783};
+
+// exp(-t^2) at the trapezoid nodes: t = n h (n >= 1) and t = (n + 1/2) h (n >= 0).
+constexpr double kCr64H = 0.5;
+constexpr double kCr64E0[12] = {
+    0.7788007830714049, 0.36787944117144233, 0.10539922456186433, 0.01831563888873418,
+    0.0019304541362277093, 0.00012340980408667956, 4.785117392129009e-06,
+    1.1253517471925912e-07, 1.6052280551856116e-09, 1.3887943864964021e-11,
+    7.287724095819692e-14, 2.3195228302435696e-16};
+constexpr double kCr64E5[13] = {
+    0.9394130628134758, 0.569782824730923, 0.2096113871510978, 0.04677062238395898,
+    0.006329715427485747, 0.0005195746821548384, 2.586810022265412e-05,
+    7.811489408304491e-07, 1.4307241918567688e-08, 1.5893910094516368e-10,
+    1.0709232382508077e-12, 4.37661850287085e-15, 1.0848552640429378e-17};
+constexpr float kCr32H = 0.8f;
+constexpr float kCr32E0[5] = {0.5272924240430485f, 0.07730474044329971f, 0.0031511115984444358f,
+                              3.5712849641635144e-05f, 1.1253517471925912e-07f};
+constexpr float kCr32E5[5] = {0.8521437889662113f, 0.23692775868212165f, 0.01831563888873418f,
+                              0.0003936690406550776f, 2.352575200009771e-06f};
+
+// ---------------------------------------------------------------- FP64 ----
+
+__device__ __forceinline__ double gh_pair64(double r, double xmy, double fxy, double t, double w) {
+  const double t2 = t * t;
+  const double u = xmy - t2;
+  return w * (r + t2) / fma(u, u, fxy);
+}
+
+// 4-node form, |z| >= 60.
+__device__ __forceinline__ double K_gh4_64(double x2, double y) {
+  const double y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+  const double s = gh_pair64(r, xmy, fxy, kGh4T[0], kGh4W[0]) +
+                   gh_pair64(r, xmy, fxy, kGh4T[1], kGh4W[1]);
+  return 2.0 * y * kInvPi * s;
+}
+
+// 8-node form, |z| >= 15.
+__device__ __forceinline__ double K_gh8_64(double x2, double y) {
+  const double y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+  double s = 0.0;
+#pragma unroll
+  for (int k = 3; k >= 0; --k) s += gh_pair64(r, xmy, fxy, kGh8T[k], kGh8W[k]);
+  return 2.0 * y * kInvPi * s;
+}
+
+// Trapezoid + residue, |z| < 15.
+__device__ inline double K_cr64(double x, double y) {
+  const double h = kCr64H;
+  const double q = fabs(x) * (1.0 / h);
+  const double fr = q - floor(q);
+  const bool half = !(fr >= 0.25 && fr <= 0.75);
+  const double x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+  double s = 0.0;
+  if (half) {
+#pragma unroll
+    for (int n = 12; n >= 0; --n) s += gh_pair64(r, xmy, fxy, (n + 0.5) * h, kCr64E5[n]);
+    s *= 2.0 * y;
+  } else {
+#pragma unroll
+    for (int n = 12; n >= 1; --n) s += gh_pair64(r, xmy, fxy, n * h, kCr64E0[n - 1]);
+    s = 2.0 * y * s + y / r;
+  }
+  double K = s * h * kInvPi;
+  if (y < kPi / h) {
+    const double sg = half ? 1.0 : -1.0;
+    const double rho = exp(-2.0 * kPi * y / h);
+    const double M = exp(y2 - x2 - 2.0 * kPi * y / h);
+    double st, ct, s2, c2;
+    sincospi(2.0 * x / h, &st, &ct);
+    sincos(2.0 * x * y, &s2, &c2);
+    const double cphi = ct * c2 + st * s2;  // cos(theta - 2xy)
+    K += 2.0 * M * (sg * cphi + rho * c2) / (1.0 + 2.0 * sg * rho * ct + rho * rho);
+  }
+  return K;
+}
+
+// Full FP64 region dispatch: relative error ~1e-13 or better everywhere.
+__device__ inline double K_fp64(double x, double y) {
+  const double x2 = x * x;
+  const double r = x2 + y * y;
+  if (r >= kFp64Gh4R2) return K_gh4_64(x2, y);
+  if (r >= kFp64Gh8R2) return K_gh8_64(x2, y);
+  return K_cr64(x, y);
+}
+
+// ---------------------------------------------------------------- FP32 ----
+
+__device__ __forceinline__ float gh_pair32(float r, float xmy, float fxy, float t2, float w) {
+  const float u = xmy - t2;
+  return w * (r + t2) / fmaf(u, u, fxy);
+}
+
+// 2-node form: K = y (r + 1/2) / (sqrt(pi) D), D = x^4 + x^2 (2y^2 - 1) + (y^2 + 1/2)^2.
+__device__ __forceinline__ float K_gh2_32(float x2, float y) {
+  const float y2 = y * y;
+  const float c = y2 + 0.5f;
+  const float D = fmaf(x2, x2 + (2.0f * y2 - 1.0f), c * c);
+  return y * (float)kInvSqrtPi * (x2 + c) / D;
+}
+
+// 4-node form, |z| >= 8.
+__device__ __forceinline__ float K_gh4_32(float x2, float y) {
+  const float y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
+  const float t0 = (float)(kGh4T[0] * kGh4T[0]), t1 = (float)(kGh4T[1] * kGh4T[1]);
+  const float u0 = xmy - t0, u1 = xmy - t1;
+  const float D0 = fmaf(u0, u0, fxy), D1 = fmaf(u1, u1, fxy);
+  const float N = (float)kGh4W[0] * (r + t0) * D1 + (float)kGh4W[1] * (r + t1) * D0;
+  return 2.0f * y * (float)kInvPi * N / (D0 * D1);
+}
+
+// Trapezoid + residue, |z| < 8.
+__device__ inline float K_cr32(float x, float y) {
+  const float h = kCr32H;
+  const float q = fabsf(x) * (1.0f / h);
+  const float fr = q - floorf(q);
+  const bool half = !(fr >= 0.25f && fr <= 0.75f);
+  const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
+  float s = 0.0f;
+  if (half) {
+#pragma unroll
+    for (int n = 4; n >= 0; --n) {
+      const float t = (n + 0.5f) * h;
+      s += gh_pair32(r, xmy, fxy, t * t, kCr32E5[n]);
+    }
+    s *= 2.0f * y;
+  } else {
+#pragma unroll
+    for (int n = 5; n >= 1; --n) {
+      const float t = n * h;
+      s += gh_pair32(r, xmy, fxy, t * t, kCr32E0[n - 1]);
+    }
+    s = 2.0f * y * s + y / r;
+  }
+  float K = s * h * (float)kInvPi;
+  if (y < (float)kPi / h) {
+    const float sg = half ? 1.0f : -1.0f;
+    const float a = 2.0f * (float)kPi / h;
+    const float rho = expf(-a * y);
+    const float M = expf(fmaf(y, y - a, -x2));
+    float st, ct, s2, c2;
+    sincospif(2.0f * x / h, &st, &ct);
+    sincosf(2.0f * x * y, &s2, &c2);
+    const float cphi = ct * c2 + st * s2;
+    K += 2.0f * M * (sg * cphi + rho * c2) / (1.0f + 2.0f * sg * rho * ct + rho * rho);
+  }
+  return K;
+}
+
+// Full FP32 region dispatch: relative error <= ~6.4e-6 everywhere.
+__device__ inline float K_mixed(float x, float y) {
+  const float x2 = x * x;
+  const float r = x2 + y * y;
+  if (r >= kF32Gh2R2) return K_gh2_32(x2, y);
+  if (r >= kF32Gh4R2) return K_gh4_32(x2, y);
+  return K_cr32(x, y);
+}
+
+}  // namespace lbl

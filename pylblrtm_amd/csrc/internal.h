@@ -1,0 +1,88 @@
+// Internal interfaces shared by the HIP translation units of libpylblrtm_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pylblrtm_amd.h"
+
+namespace lbl {
+
+// Physical constants of the reference, used verbatim
+// (pylblrtm/forward.py:32-36).
+constexpr double kTRef = 296.0;
+constexpr double kC2 = 1.4387752;
+constexpr double kDoppler = 3.5810e-7;
+constexpr double kSqrtLn2 = 0x1.aa4499161cd47p-1;     // math.sqrt(math.log(2.0))
+constexpr double kVoigtK = 0x1.e0fdec495104dp-2;      // kSqrtLn2 / math.sqrt(math.pi)
+
+constexpr int kMaxGroups = 1024;   // groups whose q-ratios fit the LDS table
+constexpr int kMaxMu = 8;          // viewing cosines per call
+constexpr int kBlock = 256;        // threads per workgroup of the line sum
+
+// Error word: ((2*layer + kind) << 11) | group, kind 0 = strength, 1 = od.
+// atomicMin over it yields the reference's first failure in loop order.
+constexpr uint32_t kErrNone = 0xFFFFFFFFu;
+constexpr int kErrGroupBits = 11;
+
+// Grid point i exactly as numpy builds it: nu_min + (double(i) * step),
+// rounded after the multiply and after the add (no fused multiply-add).
+__device__ __forceinline__ double grid_nu(double nu_min, double step, int64_t i) {
+  return __dadd_rn(nu_min, __dmul_rn((double)i, step));
+}
+
+struct Records {
+  double* c;     // shifted line centre [n_layers * n_lines], FP64 in every mode
+  void* s;       // sqrt(ln2) / alpha_D
+  void* y;       // sqrt(ln2) * gamma_L / alpha_D
+  void* a;       // strength * sqrt(ln2/pi) / alpha_D (0 when the layer is skipped)
+};
+
+struct LineParamsArgs {
+  lbl_lines_t lines;
+  lbl_groups_t groups;
+  lbl_layers_t layers;
+  Records rec;
+  uint32_t* err;
+};
+
+struct TileRangeArgs {
+  const double* nu0;   // sorted ascending
+  int64_t n_lines;
+  double nu_min, step, cutoff, margin;
+  int64_t n_nu, ts, n_tiles;
+  int32_t* lo;
+  int32_t* hi;
+};
+
+struct SumArgs {
+  Records rec;
+  const int32_t* lo;
+  const int32_t* hi;
+  const double* t_k;
+  int64_t n_lines;
+  int32_t n_layers;
+  double nu_min, step, cutoff;
+  int64_t n_nu, ts, n_tiles;
+  int32_t slices_per_tile;
+  int64_t n_wg;
+  int32_t n_mu;
+  double inv_mu[kMaxMu];
+  void* trans;
+  void* tau;
+  void* tau_layers;
+  uint32_t* err;
+};
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_line_params(const LineParamsArgs& a, int precision, hipStream_t s);
+hipError_t launch_tile_ranges(const TileRangeArgs& a, hipStream_t s);
+hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s);
+hipError_t launch_voigt_eval(int64_t n, const void* x, const void* y, void* k, int precision,
+                             hipStream_t s);
+
+// Points per thread of the line-sum kernels (a workgroup covers kBlock * P
+// consecutive grid points of one reference tile).
+int sum_points_per_thread(int precision);
+
+}  // namespace lbl

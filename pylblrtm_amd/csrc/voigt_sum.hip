@@ -1,0 +1,327 @@
+// Tiled Voigt line sum with the radiation term, layer sum and exp(-tau/mu)
+// fused in (pylblrtm/forward.py:295-315 and :151).
+//
+// Work decomposition.  The reference evaluates every reference tile (a run
+// of tile_samples grid points) against every line whose shifted centre lies
+// in [tile first point - cutoff, tile last point + cutoff] (forward.py:298-301).
+// Here one workgroup owns kBlock * P consecutive points of one tile (P
+// points per thread, contiguous) and walks the layers in order; for each
+// layer it streams that tile's candidate records through LDS in chunks of
+// kBlock, keeps only those inside the tile window (exact FP64 compare), and
+// every thread accumulates its P points.  At the end of a layer each point's
+// sum is multiplied by R(nu, T_k) (forward.py:312), checked for finiteness
+// (forward.py:314-315) and added to the column total; after the last layer
+// the kernel writes exp(-tau/mu) for every requested mu.
+//
+// Region split.  Almost every evaluation is far from its line centre, where
+// the two-node (FP32) / four-node (FP64) Gauss-Hermite form is accurate.  A
+// thread decides once per record whether all of its P points are in that
+// region (a per-lane test on the range of x it covers); lanes that are not
+// take the full region dispatch of faddeeva.h.  Both branches are plain
+// divergent ifs, so a wavefront only pays for the slow branch when one of
+// its lanes needs it.
+//
+// Precision.  FP64 mode: every operation in FP64.  Mixed mode: nu - nu_c is
+// formed in FP64 from the exact grid point and the FP64 centre, rounded to
+// FP32 once per (thread, record); the P points then step by step * s in
+// FP32.  Sums are FP32 over at most 32 records, folded into a compensated
+// (Kahan) per-layer sum, and the per-layer values into a compensated column
+// total.
+#include <math.h>
+
+#include "faddeeva.h"
+#include "internal.h"
+
+namespace lbl {
+namespace {
+
+constexpr int kChunk = kBlock;  // records staged per LDS round
+constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
+
+// Hardware assigns workgroup b to XCD b % 8.  Give every XCD a contiguous
+// run of logical workgroups so neighbouring tiles (which share most of their
+// line records) share an L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
+  const int64_t per = n >> 3, rem = n & 7;
+  const int64_t xcd = b & 7, idx = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
+  const float y = v - comp;
+  const float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+}
+
+__device__ __forceinline__ double fast_rcp64(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ void flag_od(uint32_t* err, int k) {
+  atomicMin(err, (uint32_t)(2 * k + 1) << kErrGroupBits);
+}
+
+// ------------------------------------------------------------------ FP64 --
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_voigt_sum_fp64(const SumArgs a) {
+  __shared__ double s_c[kChunk], s_s[kChunk], s_y[kChunk], s_a[kChunk];
+  __shared__ double s_y2[kChunk], s_w0[kChunk], s_w1[kChunk];
+
+  const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
+  const int64_t tile = wg / a.slices_per_tile;
+  const int slice = (int)(wg - tile * a.slices_per_tile);
+  const int64_t t0 = tile * a.ts;
+  const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)threadIdx.x * P;
+  const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
+  const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
+  const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
+
+  double nu[P], tau[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    nu[j] = grid_nu(a.nu_min, a.step, i0 + j);
+    tau[j] = 0.0;
+  }
+  const double t0sq = kGh4T[0] * kGh4T[0], t1sq = kGh4T[1] * kGh4T[1];
+  const double* rs = static_cast<const double*>(a.rec.s);
+  const double* ry = static_cast<const double*>(a.rec.y);
+  const double* ra = static_cast<const double*>(a.rec.a);
+
+  for (int k = 0; k < a.n_layers; ++k) {
+    double lay[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) lay[j] = 0.0;
+    const int64_t kb = (int64_t)k * a.n_lines;
+    for (int32_t base = rlo; base < rhi; base += kChunk) {
+      const int cnt = min(kChunk, rhi - base);
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) {
+        const int64_t r = kb + base + threadIdx.x;
+        double c = a.rec.c[r], s = rs[r], y = ry[r], am = ra[r];
+        if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped)
+          c = 0.0; s = 0.0; y = 1.0e3; am = 0.0;
+        }
+        const int q = threadIdx.x;
+        const double w = am * 2.0 * y * kInvPi;
+        s_c[q] = c; s_s[q] = s; s_y[q] = y; s_a[q] = am;
+        s_y2[q] = y * y; s_w0[q] = w * kGh4W[0]; s_w1[q] = w * kGh4W[1];
+      }
+      __syncthreads();
+      for (int q = 0; q < cnt; ++q) {
+        const double c = s_c[q], s = s_s[q], y2 = s_y2[q];
+        const double xa = (nu[0] - c) * s, xb = (nu[P - 1] - c) * s;
+        const double mn = xa > 0.0 ? xa : (xb < 0.0 ? -xb : 0.0);
+        if (fma(mn, mn, y2) >= kFp64Gh4R2) {
+          const double w0 = s_w0[q], w1 = s_w1[q], fy2 = 4.0 * y2;
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const double x = (nu[j] - c) * s;
+            const double x2 = x * x;
+            const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+            const double u0 = xmy - t0sq, u1 = xmy - t1sq;
+            const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
+            const double N = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
+            lay[j] = fma(N, fast_rcp64(D0 * D1), lay[j]);
+          }
+        } else {
+          const double y = s_y[q], am = s_a[q];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const double x = (nu[j] - c) * s;
+            lay[j] = fma(am, K_fp64(x, y), lay[j]);
+          }
+        }
+      }
+    }
+    const double T = a.t_k[k];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int64_t i = i0 + j;
+      if (i < t1) {
+        const double v = lay[j] * (nu[j] * tanh(0.5 * kC2 * nu[j] / T));
+        bad |= !isfinite(v);
+        tau[j] += v;
+        if (a.tau_layers) static_cast<double*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
+      }
+    }
+    if (bad) flag_od(a.err, k);
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int64_t i = i0 + j;
+    if (i < t1) {
+      for (int m = 0; m < a.n_mu; ++m)
+        static_cast<double*>(a.trans)[(int64_t)m * a.n_nu + i] = exp(-tau[j] * a.inv_mu[m]);
+      if (a.tau) static_cast<double*>(a.tau)[i] = tau[j];
+    }
+  }
+}
+
+// ----------------------------------------------------------------- mixed --
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
+  __shared__ double s_c[kChunk];
+  __shared__ float4 s_p0[kChunk];  // s, step*s, 2y^2 - 1, (y^2 + 1/2)^2
+  __shared__ float4 s_p1[kChunk];  // A y/sqrt(pi), A y (y^2 + 1/2)/sqrt(pi), y, A
+
+  const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
+  const int64_t tile = wg / a.slices_per_tile;
+  const int slice = (int)(wg - tile * a.slices_per_tile);
+  const int64_t t0 = tile * a.ts;
+  const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)threadIdx.x * P;
+  const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
+  const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
+  const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
+  const double nu_l = grid_nu(a.nu_min, a.step, i0);
+  const float stepf = (float)a.step;
+  const float* rs = static_cast<const float*>(a.rec.s);
+  const float* ry = static_cast<const float*>(a.rec.y);
+  const float* ra = static_cast<const float*>(a.rec.a);
+
+  float tau[P], tauc[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) tau[j] = tauc[j] = 0.0f;
+
+  for (int k = 0; k < a.n_layers; ++k) {
+    float lay[P], layc[P], part[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) lay[j] = layc[j] = part[j] = 0.0f;
+    const int64_t kb = (int64_t)k * a.n_lines;
+    for (int32_t base = rlo; base < rhi; base += kChunk) {
+      const int cnt = min(kChunk, rhi - base);
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) {
+        const int64_t r = kb + base + threadIdx.x;
+        double c = a.rec.c[r];
+        float s = rs[r], y = ry[r], am = ra[r];
+        if (!(c >= lo_b && c <= hi_b)) {
+          c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
+        }
+        const float y2 = y * y, cp = y2 + 0.5f;
+        const float ay = am * y * (float)kInvSqrtPi;
+        s_c[threadIdx.x] = c;
+        s_p0[threadIdx.x] = make_float4(s, stepf * s, 2.0f * y2 - 1.0f, cp * cp);
+        s_p1[threadIdx.x] = make_float4(ay, ay * cp, y, am);
+      }
+      __syncthreads();
+      for (int q = 0; q < cnt; ++q) {
+        const double c = s_c[q];
+        const float4 p0 = s_p0[q];
+        const float4 p1 = s_p1[q];
+        const float x0 = (float)(nu_l - c) * p0.x;
+        const float hs = p0.y;
+        const float xb = fmaf((float)(P - 1), hs, x0);
+        const float mn = x0 > 0.0f ? x0 : (xb < 0.0f ? -xb : 0.0f);
+        const float y = p1.z;
+        if (fmaf(mn, mn, y * y) >= kF32Gh2R2) {
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, hs, x0);
+            const float x2 = x * x;
+            const float D = fmaf(x2, x2 + p0.z, p0.w);
+            const float N = fmaf(p1.x, x2, p1.y);
+            part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, hs, x0);
+            part[j] = fmaf(p1.w, K_mixed(x, y), part[j]);
+          }
+        }
+        if ((q & (kFoldEvery - 1)) == kFoldEvery - 1) {
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            kahan_add(lay[j], layc[j], part[j]);
+            part[j] = 0.0f;
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        kahan_add(lay[j], layc[j], part[j]);
+        part[j] = 0.0f;
+      }
+    }
+    const double T = a.t_k[k];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int64_t i = i0 + j;
+      if (i < t1) {
+        const double nu = grid_nu(a.nu_min, a.step, i);
+        const float R = (float)(nu * tanh(0.5 * kC2 * nu / T));
+        const float v = (lay[j] - layc[j]) * R;
+        bad |= !isfinite(v);
+        kahan_add(tau[j], tauc[j], v);
+        if (a.tau_layers) static_cast<float*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
+      }
+    }
+    if (bad) flag_od(a.err, k);
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int64_t i = i0 + j;
+    if (i < t1) {
+      const float t = tau[j] - tauc[j];
+      for (int m = 0; m < a.n_mu; ++m)
+        static_cast<float*>(a.trans)[(int64_t)m * a.n_nu + i] = expf(-t * (float)a.inv_mu[m]);
+      if (a.tau) static_cast<float*>(a.tau)[i] = t;
+    }
+  }
+}
+
+// --------------------------------------------------------- test entry --
+
+__global__ void k_voigt_eval64(int64_t n, const double* x, const double* y, double* k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) k[i] = K_fp64(x[i], y[i]);
+}
+
+__global__ void k_voigt_eval32(int64_t n, const float* x, const float* y, float* k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) k[i] = K_mixed(x[i], y[i]);
+}
+
+constexpr int kPointsFp64 = 4;
+constexpr int kPointsMixed = 8;
+
+}  // namespace
+
+int sum_points_per_thread(int precision) {
+  return precision == LBL_FP64 ? kPointsFp64 : kPointsMixed;
+}
+
+hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
+  if (a.n_wg <= 0) return hipSuccess;
+  if (precision == LBL_FP64)
+    k_voigt_sum_fp64<kPointsFp64><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+  else
+    k_voigt_sum_mixed<kPointsMixed><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_voigt_eval(int64_t n, const void* x, const void* y, void* k, int precision,
+                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  if (precision == LBL_FP64)
+    k_voigt_eval64<<<g, 256, 0, s>>>(n, static_cast<const double*>(x),
+                                     static_cast<const double*>(y), static_cast<double*>(k));
+  else
+    k_voigt_eval32<<<g, 256, 0, s>>>(n, static_cast<const float*>(x),
+                                     static_cast<const float*>(y), static_cast<float*>(k));
+  return hipGetLastError();
+}
+
+}  // namespace lbl

@@ -1,0 +1,249 @@
+"""Device-resident problem: the host prelude of the reference's
+``_build_optical_depth`` plus the upload of everything the HIP kernels read.
+
+``OpticalDepthProblem`` is built once per (spectral data, layers, grid,
+cutoff, tile width) and can be run any number of times (the benchmark times
+``run``; ``compute_transmittance`` / ``run_forward`` build and run once).
+
+Host prelude = pylblrtm/forward.py:171-245: tile size from the first two grid
+points, the per-line shift bound, groups in sorted (mol_id, iso_id) order
+with the same warnings and skips, the conservative band mask, the 28.97 g/mol
+molar-mass fallback, and the KeyError of a missing partition-sum table.  The
+partition-sum ratios themselves are interpolated on the device.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _capi
+from .molecules import MOL_ID_TO_NAME
+
+T_REF = 296.0
+
+
+def make_grid(nu_min: float, nu_max: float, step: float) -> np.ndarray:
+    """forward.py:141-142."""
+    n = int(np.ceil((nu_max - nu_min) / step)) + 1
+    return nu_min + np.arange(n, dtype=np.float64) * step
+
+
+def tile_samples_for(nu: np.ndarray, tile_width: float) -> int:
+    """forward.py:175-176 (depends on the grid origin through nu[1] - nu[0])."""
+    step = float(nu[1] - nu[0]) if nu.size > 1 else 0.001
+    return max(1, int(np.floor(tile_width / step + 1e-8)))
+
+
+@dataclass
+class Group:
+    mol_id: int
+    iso_id: int
+    gas: str
+    index: np.ndarray        # positions in SpectralData.lines
+    molar_mass: float
+    tips: Tuple[np.ndarray, np.ndarray]
+
+
+def prepare_groups(nu: np.ndarray, sd, layers, cutoff: float, stacklevel: int = 4) -> List[Group]:
+    """forward.py:185-245 without the q-ratio arithmetic (done on the device)."""
+    lines = sd.lines
+    nu0 = np.asarray(lines["nu0"], np.float64)
+    delta = np.asarray(lines["shift"], np.float64)
+    mol = np.asarray(lines["mol_id"])
+    iso = np.asarray(lines["iso_id"])
+    p_max = float(layers.pressure_atm.max()) if layers.n_layers > 0 else 1.0
+    bound = np.abs(delta) * p_max
+    lo = nu[0] - cutoff - bound
+    hi = nu[-1] + cutoff + bound
+    groups: List[Group] = []
+    for m, i in sorted(set(zip(mol.tolist(), iso.tolist()))):
+        gas = MOL_ID_TO_NAME.get(int(m))
+        if gas is None:
+            warnings.warn(f"mol_id={m} is not in the HITRAN catalogue; lines skipped.",
+                          RuntimeWarning, stacklevel=stacklevel)
+            continue
+        if gas not in layers.column_cm2:
+            warnings.warn(
+                f"Gas '{gas}' (mol_id={m}, iso_id={i}) is in the spectral data but not in the "
+                f"atmospheric profile vmr; its lines are skipped.",
+                RuntimeWarning, stacklevel=stacklevel)
+            continue
+        sel = (mol == m) & (iso == i) & (nu0 >= lo) & (nu0 <= hi)
+        if not sel.any():
+            continue
+        key = (int(m), int(i))
+        mm = sd.molar_masses.get(key)
+        if mm is None:
+            warnings.warn(f"No molar mass found for (mol_id={m}, iso_id={i}); "
+                          f"using dry-air mass as fallback.", RuntimeWarning, stacklevel=stacklevel)
+            mm = 28.97
+        if key not in sd.tips:
+            raise KeyError(f"No TIPS data for (mol_id={m}, iso_id={i}). "
+                           f"Available: {sorted(sd.tips.keys())}")
+        t_nodes, q_nodes = sd.tips[key]
+        if len(t_nodes) == 0 or len(t_nodes) != len(q_nodes):
+            raise ValueError(f"TIPS table of (mol_id={m}, iso_id={i}) is empty or ragged")
+        groups.append(Group(int(m), int(i), gas, np.flatnonzero(sel), float(mm),
+                            (np.asarray(t_nodes, np.float64), np.asarray(q_nodes, np.float64))))
+    return groups
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("pylblrtm_amd needs a ROCm GPU: torch.cuda.is_available() is False "
+                           "(the HIP path has no CPU fallback)")
+    return torch
+
+
+class OpticalDepthProblem:
+    """Inputs of the device path, uploaded once."""
+
+    def __init__(self, sd, layers, nu_min: float, nu_max: float, step: float, cutoff: float,
+                 tile_width: float, device=None, stacklevel: int = 4):
+        torch = _torch()
+        self.lib = _capi.load()
+        self.device = torch.device(device if device is not None else f"cuda:{torch.cuda.current_device()}")
+        self.nu = make_grid(nu_min, nu_max, step)
+        self.nu_min, self.step = float(nu_min), float(step)
+        self.n_nu = self.nu.size
+        self.tile_samples = tile_samples_for(self.nu, tile_width)
+        self.cutoff = float(cutoff)
+        self.n_layers = layers.n_layers
+        self.groups = prepare_groups(self.nu, sd, layers, cutoff, stacklevel=stacklevel)
+
+        gases: List[str] = []
+        for g in self.groups:
+            if g.gas not in gases:
+                gases.append(g.gas)
+        self.gases = gases
+        lines = sd.lines
+        if self.groups:
+            idx = np.concatenate([g.index for g in self.groups])
+            gid = np.concatenate([np.full(g.index.size, j, np.int32) for j, g in enumerate(self.groups)])
+            order = np.argsort(np.asarray(lines["nu0"], np.float64)[idx], kind="stable")
+            idx, gid = idx[order], gid[order]
+        else:
+            idx = np.zeros(0, np.int64)
+            gid = np.zeros(0, np.int32)
+        self.n_lines = int(idx.size)
+        self.line_index = idx
+        p_max = float(layers.pressure_atm.max()) if self.n_layers > 0 else 1.0
+        shift_sel = np.asarray(lines["shift"], np.float64)[idx]
+        self.shift_bound = float(np.max(np.abs(shift_sel)) * p_max) if idx.size else 0.0
+
+        def dev(a, dtype):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype=dtype)).to(self.device)
+
+        f8 = np.float64
+        self._t = {}
+        for key in ("nu0", "Sref", "g_air", "g_self", "Elow", "n_air", "shift"):
+            self._t[key] = dev(np.asarray(lines[key], f8)[idx], f8)
+        self._t["group"] = dev(gid, np.int32)
+        offs = np.zeros(len(self.groups) + 1, np.int32)
+        for j, g in enumerate(self.groups):
+            offs[j + 1] = offs[j] + g.tips[0].size
+        cat = lambda k: (np.concatenate([g.tips[k] for g in self.groups]) if self.groups
+                         else np.zeros(1))
+        self._t["gas"] = dev(np.array([gases.index(g.gas) for g in self.groups] or [0], np.int32),
+                             np.int32)
+        self._t["mm"] = dev(np.array([g.molar_mass for g in self.groups] or [1.0]), f8)
+        self._t["toff"] = dev(offs, np.int32)
+        self._t["tips_t"] = dev(cat(0), f8)
+        self._t["tips_q"] = dev(cat(1), f8)
+        nl = max(self.n_layers, 1)
+        vmr = np.zeros((max(len(gases), 1), nl))
+        col = np.zeros((max(len(gases), 1), nl))
+        for j, gname in enumerate(gases):
+            vmr[j, : self.n_layers] = layers.vmr[gname]
+            col[j, : self.n_layers] = layers.column_cm2[gname]
+        self._t["p_atm"] = dev(layers.pressure_atm if self.n_layers else np.zeros(1), f8)
+        self._t["t_k"] = dev(layers.temperature_k if self.n_layers else np.ones(1), f8)
+        self._t["vmr"] = dev(vmr, f8)
+        self._t["col"] = dev(col, f8)
+
+        p = lambda name: self._t[name].data_ptr()
+        self.c_lines = _capi.LblLines(self.n_lines, p("nu0"), p("Sref"), p("g_air"), p("g_self"),
+                                      p("Elow"), p("n_air"), p("shift"), p("group"))
+        self.c_groups = _capi.LblGroups(len(self.groups), p("gas"), p("mm"), p("toff"),
+                                        p("tips_t"), p("tips_q"))
+        self.c_layers = _capi.LblLayers(self.n_layers, len(gases), p("p_atm"), p("t_k"),
+                                        p("vmr"), p("col"))
+        self.c_grid = _capi.LblGrid(self.nu_min, self.step, self.n_nu, self.tile_samples,
+                                    self.cutoff, self.shift_bound)
+        self._ws = {}
+
+    # ------------------------------------------------------------------ run --
+    def workspace(self, precision: int):
+        torch = _torch()
+        if precision not in self._ws:
+            nbytes = ctypes.c_size_t(0)
+            st = self.lib.lbl_workspace_bytes(ctypes.byref(self.c_lines), ctypes.byref(self.c_layers),
+                                              ctypes.byref(self.c_grid), precision, ctypes.byref(nbytes))
+            _capi.raise_for(st, "lbl_workspace_bytes")
+            self._ws[precision] = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8,
+                                              device=self.device)
+        return self._ws[precision]
+
+    def alloc_outputs(self, precision: str, n_mu: int, want_tau: bool, want_layers: bool):
+        torch = _torch()
+        dt = torch.float64 if precision == "fp64" else torch.float32
+        out = {"transmittance": torch.empty((n_mu, self.n_nu), dtype=dt, device=self.device)}
+        if want_tau:
+            out["optical_depth"] = torch.empty(self.n_nu, dtype=dt, device=self.device)
+        if want_layers:
+            out["optical_depth_layers"] = torch.empty((self.n_layers, self.n_nu), dtype=dt,
+                                                      device=self.device)
+        return out
+
+    def run(self, precision: str = "fp64", mus: Sequence[float] = (1.0,), want_tau: bool = False,
+            want_layers: bool = False, outputs: Optional[dict] = None) -> dict:
+        """Run the device path; returns device tensors (transmittance [n_mu, n_nu], ...)."""
+        torch = _torch()
+        if precision not in _capi.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_capi.PRECISIONS)}; got {precision!r}")
+        code = _capi.PRECISIONS[precision]
+        ws = self.workspace(code)
+        out = outputs or self.alloc_outputs(precision, len(mus), want_tau, want_layers)
+        mu_arr = (ctypes.c_double * len(mus))(*[float(m) for m in mus])
+        ptr = lambda k: out[k].data_ptr() if k in out else None
+        c_out = _capi.LblOutputs(len(mus), mu_arr, ptr("transmittance"), ptr("optical_depth"),
+                                 ptr("optical_depth_layers"))
+        err_layer = ctypes.c_int32(-1)
+        err_group = ctypes.c_int32(-1)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        with torch.cuda.device(self.device):
+            st = self.lib.lbl_optical_depth(
+                ctypes.byref(self.c_lines), ctypes.byref(self.c_groups), ctypes.byref(self.c_layers),
+                ctypes.byref(self.c_grid), code, ctypes.byref(c_out), ctypes.c_void_p(ws.data_ptr()),
+                ws.numel(), ctypes.byref(err_layer), ctypes.byref(err_group), ctypes.c_void_p(stream))
+        if st == _capi.LBL_ESTRENGTH:
+            gas = self.groups[err_group.value].gas
+            raise FloatingPointError(
+                f"Layer {err_layer.value}, gas '{gas}': non-finite or negative line strength")
+        if st == _capi.LBL_ENONFINITE:
+            raise FloatingPointError(
+                f"Layer {err_layer.value}: non-finite optical depth after radiation term")
+        _capi.raise_for(st, "lbl_optical_depth")
+        return out
+
+
+def voigt_eval(x: np.ndarray, y: np.ndarray, precision: str = "fp64", device=None) -> np.ndarray:
+    """Device K(x, y) = Re w(x + iy) with the line sum's region dispatch."""
+    torch = _torch()
+    lib = _capi.load()
+    dt = np.float64 if precision == "fp64" else np.float32
+    dev = torch.device(device if device is not None else f"cuda:{torch.cuda.current_device()}")
+    xt = torch.as_tensor(np.ascontiguousarray(x, dt)).to(dev)
+    yt = torch.as_tensor(np.ascontiguousarray(y, dt)).to(dev)
+    kt = torch.empty_like(xt)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    st = lib.lbl_voigt_eval(xt.numel(), xt.data_ptr(), yt.data_ptr(), kt.data_ptr(),
+                            _capi.PRECISIONS[precision], ctypes.c_void_p(stream))
+    _capi.raise_for(st, "lbl_voigt_eval")
+    return kt.cpu().numpy()

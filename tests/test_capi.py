@@ -1,0 +1,66 @@
+"""The C-ABI library loads and exports every function include/*.h declares;
+argument validation happens before any device call (CPU only)."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from pylblrtm_amd import _capi
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = []
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(lbl_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _capi.load()
+    names = declared_functions()
+    assert "lbl_optical_depth" in names and len(names) >= 4
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_capi.EXPORTED) <= set(names)
+
+
+def test_last_error_is_empty_string_initially():
+    assert isinstance(_capi.last_error(), str)
+
+
+def test_invalid_arguments_rejected_without_device():
+    lib = _capi.load()
+    grid = _capi.LblGrid(700.0, 0.001, 100, 10, 25.0, 0.0)
+    lines = _capi.LblLines(0, None, None, None, None, None, None, None, None)
+    groups = _capi.LblGroups(0, None, None, None, None, None)
+    layers = _capi.LblLayers(0, 0, None, None, None, None)
+    mu = (ctypes.c_double * 1)(1.0)
+    out = _capi.LblOutputs(1, mu, None, None, None)
+    st = lib.lbl_optical_depth(ctypes.byref(lines), ctypes.byref(groups), ctypes.byref(layers),
+                               ctypes.byref(grid), 0, ctypes.byref(out), None, 0, None, None, None)
+    assert st == _capi.LBL_EINVAL and "transmittance" in _capi.last_error()
+    st = lib.lbl_optical_depth(ctypes.byref(lines), ctypes.byref(groups), ctypes.byref(layers),
+                               ctypes.byref(grid), 7, ctypes.byref(out), None, 0, None, None, None)
+    assert st == _capi.LBL_EINVAL and "precision" in _capi.last_error()
+    with pytest.raises(ValueError):
+        _capi.raise_for(st, "x")
+
+
+def test_workspace_size_is_host_arithmetic():
+    lib = _capi.load()
+    grid = _capi.LblGrid(500.0, 0.001, 2_000_001, 2000, 25.0, 0.01)
+    lines = _capi.LblLines(500_000, None, None, None, None, None, None, None, None)
+    layers = _capi.LblLayers(50, 5, None, None, None, None)
+    n = ctypes.c_size_t(0)
+    assert lib.lbl_workspace_bytes(ctypes.byref(lines), ctypes.byref(layers), ctypes.byref(grid),
+                                   1, ctypes.byref(n)) == 0
+    assert n.value >= 25_000_000 * 20
+    m = ctypes.c_size_t(0)
+    lib.lbl_workspace_bytes(ctypes.byref(lines), ctypes.byref(layers), ctypes.byref(grid), 0,
+                            ctypes.byref(m))
+    assert m.value >= 25_000_000 * 32 and m.value > n.value

@@ -1,0 +1,177 @@
+"""Parity of the HIP path against the CPU oracle and the reference's golden
+vectors.  Needs a ROCm GPU; everything is called through the C ABI."""
+
+import json
+import os
+import sys
+import warnings
+
+import numpy as np
+import pytest
+from scipy.special import wofz
+
+from oracle import lbl_oracle as O
+from pylblrtm_amd import AtmosphericProfile, SpectralData, compute_transmittance, run_forward, synth
+from pylblrtm_amd.device import OpticalDepthProblem, voigt_eval
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+TAU_RTOL_FP64 = 1e-9     # FP64 mode: tau relative to the FP64 oracle
+DT_ATOL_MIXED = 1e-5     # mixed mode: max |dT| against FP64
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ── device Faddeeva ──────────────────────────────────────────────────────────
+
+def _plane(n=200_000, seed=11):
+    rng = np.random.default_rng(seed)
+    x = np.concatenate([rng.uniform(-10, 10, n // 2), rng.uniform(-120, 120, n // 2)])
+    y = 10.0 ** rng.uniform(-6, 2.5, n)
+    return x, y
+
+
+def test_voigt_fp64_known_answers():
+    say("case: voigt fp64 KAT")
+    kat = json.load(open(os.path.join(HERE, "golden", "wofz_kat.json")))
+    z = np.array([complex(*v) for v in kat["z"]])
+    w = np.array([complex(*v) for v in kat["w"]])
+    K = voigt_eval(z.real, z.imag, "fp64")
+    np.testing.assert_allclose(K, w.real, rtol=1e-12, atol=0)
+
+
+def test_voigt_fp64_against_scipy_plane():
+    say("case: voigt fp64 plane")
+    x, y = _plane()
+    K = voigt_eval(x, y, "fp64")
+    ref = wofz(x + 1j * y).real
+    rel = np.abs(K - ref) / ref
+    assert rel.max() < 1e-12, (rel.max(), x[rel.argmax()], y[rel.argmax()])
+
+
+def test_voigt_fp64_real_axis_and_imag_axis():
+    say("case: voigt fp64 axes")
+    from scipy.special import erfcx
+    x = np.linspace(-5.5, 5.5, 1001)
+    np.testing.assert_allclose(voigt_eval(x, np.zeros_like(x), "fp64"), np.exp(-x * x),
+                               rtol=1e-12, atol=1e-300)
+    y = np.geomspace(1e-6, 1e3, 500)
+    np.testing.assert_allclose(voigt_eval(np.zeros_like(y), y, "fp64"), erfcx(y), rtol=1e-12)
+
+
+def test_voigt_mixed_against_scipy_plane():
+    say("case: voigt fp32 plane")
+    x, y = _plane(seed=12)
+    K = voigt_eval(x, y, "mixed").astype(np.float64)
+    ref = wofz(x.astype(np.float32).astype(np.float64) + 1j * y.astype(np.float32).astype(np.float64)).real
+    rel = np.abs(K - ref) / ref
+    assert rel.max() < 1e-5, (rel.max(), x[rel.argmax()], y[rel.argmax()])
+
+
+# ── golden vectors of the reference ──────────────────────────────────────────
+
+def _golden(name):
+    from golden_io import unpack_case
+    z = np.load(os.path.join(HERE, "golden", f"{name}.npz"))
+    lines, tips, mm, p, t, vmr, meta = unpack_case(z)
+    sd = SpectralData(lines=lines, tips=tips, molar_masses=mm)
+    prof = AtmosphericProfile(p, t, vmr)
+    return z, sd, prof, meta
+
+
+@pytest.mark.parametrize("name", ["co2_mini", "three_gas_edge", "tile1"])
+def test_golden_fp64(name):
+    say(f"case: golden fp64 {name}")
+    z, sd, prof, meta = _golden(name)
+    lo, hi, step = meta["grid"]
+    for j, mu in enumerate(meta["mus"]):
+        cfg = {"spectral": {"nu_min_cm1": lo, "nu_max_cm1": hi, "step_cm1": step},
+               "calculation": {"mu_view": mu, "line_cutoff_cm1": meta["cutoff"],
+                               "tile_width_cm1": meta["tile"]},
+               "output": {"transmittance": True, "optical_depth": True, "optical_depth_layers": True}}
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            res = run_forward(sd, prof, cfg)
+        assert sorted({str(x.message) for x in w}) == meta["warnings"]
+        np.testing.assert_array_equal(res.wavenumber_cm1, z["out_nu"])
+        ref = z["out_od_layers"]
+        np.testing.assert_allclose(res.optical_depth_layers, ref, rtol=TAU_RTOL_FP64, atol=0)
+        np.testing.assert_allclose(res.optical_depth, ref.sum(0), rtol=TAU_RTOL_FP64, atol=0)
+        np.testing.assert_allclose(res.transmittance, z[f"out_T_mu{j}"], rtol=1e-8, atol=1e-300)
+
+
+def test_golden_mixed_transmittance():
+    say("case: golden mixed co2_mini")
+    z, sd, prof, meta = _golden("co2_mini")
+    lo, hi, step = meta["grid"]
+    for j, mu in enumerate(meta["mus"]):
+        nu, T = compute_transmittance(sd, prof, lo, hi, step_cm1=step, line_cutoff_cm1=meta["cutoff"],
+                                      tile_width_cm1=meta["tile"], mu_view=mu, precision="mixed")
+        assert np.max(np.abs(T - z[f"out_T_mu{j}"])) <= DT_ATOL_MIXED
+
+
+def test_golden_error_message():
+    say("case: golden nan strength")
+    z, sd, prof, meta = _golden("nan_strength")
+    lo, hi, step = meta["grid"]
+    for prec in ("fp64", "mixed"):
+        with pytest.raises(FloatingPointError) as e:
+            compute_transmittance(sd, prof, lo, hi, step_cm1=step, line_cutoff_cm1=meta["cutoff"],
+                                  tile_width_cm1=meta["tile"], precision=prec)
+        assert ["FloatingPointError", str(e.value)] == meta["error"]
+
+
+# ── seeded workloads against the oracle ──────────────────────────────────────
+
+def _oracle_tau(sd, layers, nu_min, nu_max, step, cutoff, tile, tiles=None):
+    nu = O.grid(nu_min, nu_max, step)
+    od = O.optical_depth(nu, sd.lines, sd.tips, sd.molar_masses, layers.pressure_atm,
+                         layers.temperature_k, layers.vmr, layers.column_cm2, cutoff, tile,
+                         tiles=tiles)
+    return nu, od
+
+
+def test_co2_window_subset_vs_oracle():
+    say("case: co2 window 500 lines, 20 layers, 715-720 @ 0.001")
+    sd, prof = synth.co2_window(n_lines=500)
+    lay = prof.build_layers()
+    nu, od = _oracle_tau(sd, lay, 715.0, 720.0, 0.001, 25.0, 2.0)
+    tau_ref = od.sum(0)
+    prob = OpticalDepthProblem(sd, lay, 715.0, 720.0, 0.001, 25.0, 2.0)
+    o64 = prob.run("fp64", mus=(1.0, 0.5), want_tau=True, want_layers=True)
+    tau = o64["optical_depth"].cpu().numpy()
+    assert np.max(np.abs(tau - tau_ref) / tau_ref) < TAU_RTOL_FP64
+    np.testing.assert_allclose(o64["optical_depth_layers"].cpu().numpy(), od, rtol=TAU_RTOL_FP64)
+    T1 = o64["transmittance"][0].cpu().numpy()
+    T2 = o64["transmittance"][1].cpu().numpy()
+    np.testing.assert_allclose(T1, np.exp(-tau_ref), rtol=1e-8, atol=1e-300)
+    np.testing.assert_allclose(T2, T1 * T1, rtol=1e-12, atol=1e-300)   # mu = 0.5 doubles the path
+    om = prob.run("mixed", mus=(1.0, 0.5), want_tau=True)
+    for m, mu in enumerate((1.0, 0.5)):
+        Tm = om["transmittance"][m].double().cpu().numpy()
+        assert np.max(np.abs(Tm - np.exp(-tau_ref / mu))) <= DT_ATOL_MIXED
+
+
+def test_five_gas_tiles_vs_oracle():
+    say("case: five gas 2000 lines/gas, 50 layers, 500-2500 @ 0.001, tiles 0/437/1000")
+    sd, prof = synth.five_gas(lines_per_gas=2000)
+    lay = prof.build_layers()
+    tiles = [0, 437, 1000]
+    nu, od = _oracle_tau(sd, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, tiles=tiles)
+    tau_ref = od.sum(0)
+    m = ~np.isnan(tau_ref)
+    prob = OpticalDepthProblem(sd, lay, 500.0, 2500.0, 0.001, 25.0, 2.0)
+    o64 = prob.run("fp64", mus=(1.0,), want_tau=True)
+    tau = o64["optical_depth"].cpu().numpy()
+    rel = np.abs(tau[m] - tau_ref[m]) / tau_ref[m]
+    assert rel.max() < TAU_RTOL_FP64, rel.max()
+    om = prob.run("mixed", mus=(1.0, 0.5), want_tau=True)
+    T64 = o64["transmittance"][0].cpu().numpy()
+    Tm = om["transmittance"][0].double().cpu().numpy()
+    assert np.max(np.abs(Tm - T64)) <= DT_ATOL_MIXED     # whole 2,000,001-point grid
+    Tm5 = om["transmittance"][1].double().cpu().numpy()
+    assert np.max(np.abs(Tm5 - np.exp(-2.0 * tau))) <= DT_ATOL_MIXED
