@@ -1,0 +1,203 @@
+#!/usr/bin/env python
+"""Benchmark of the line-by-line optical-depth path on one MI355X.
+
+Workload (default): five synthetic gases (H2O, CO2, O3, N2O, CH4), 500,000
+lines, 50 layers, grid 500-2500 cm^-1 at 0.001 cm^-1 (2,000,001 points),
+25 cm^-1 cutoff, 2 cm^-1 tiles, mu_view in {1.0, 0.5}.
+
+One step = one pass of the GPU path over the resident inputs: per-(layer,
+line) scaling, tile ranges, the Voigt line sum with radiation term, layer sum
+and exp(-tau/mu) for both viewing cosines.  `value` = exact line-gridpoint
+evaluations of the reference's algorithm per step / seconds per step.
+
+Also reported: the FP64 mode side by side, max |dT| of the timed mode against
+FP64 over the whole grid, the line-sum kernel's FLOP roofline fraction (HIP
+events on the launch stream), and the CPU oracle timed on a bounded sample.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Voigt line-gridpoint evals/s at max|dT|<=1e-5 vs FP64; surrogate vs plain FP32"
+FLOP_PER_EVAL = 16            # far-wing rational form, every mode charged the same
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def time_mode(prob, precision, mus, steps, warmup, capi, torch):
+    for _ in range(warmup):
+        prob.run(precision, mus=mus)
+    torch.cuda.synchronize()
+    capi.enable_timing(True)
+    outs = None
+    k_ms = []
+    lp_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs = prob.run(precision, mus=mus)
+        a, b = capi.last_timings()
+        lp_ms.append(a)
+        k_ms.append(b)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    capi.enable_timing(False)
+    return outs, wall / steps, float(np.mean(k_ms)), float(np.mean(lp_ms))
+
+
+def cpu_baseline(sd, lay, prob, tile, layer_lo, layer_hi, gpu_layers):
+    """Oracle (FP64, scipy wofz, one core) on one reference tile and a band of
+    layers; also the parity of the GPU per-layer optical depth on that sample."""
+    from oracle import lbl_oracle as O
+    t0s, t1s, _, _ = prob.tile_edges()
+    ks = list(range(layer_lo, layer_hi))
+    sub = lambda a: np.asarray(a)[ks]
+    stats = {}
+    t = time.perf_counter()
+    od = O.optical_depth(prob.nu, sd.lines, sd.tips, sd.molar_masses, sub(lay.pressure_atm),
+                         sub(lay.temperature_k), {g: sub(v) for g, v in lay.vmr.items()},
+                         {g: sub(v) for g, v in lay.column_cm2.items()}, prob.cutoff,
+                         2.0, tiles=[tile], stats=stats)
+    dt = time.perf_counter() - t
+    a, b = int(t0s[tile]), int(t1s[tile])
+    res = {"seconds": dt, "evals": int(stats["evals"])}
+    for name, arr in gpu_layers.items():
+        g = arr[ks][:, a:b]
+        r = od[:, a:b]
+        res[name] = float(np.max(np.abs(g - r) / np.maximum(r, 1e-300)))
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64", "surrogate"])
+    ap.add_argument("--lines-per-gas", type=int, default=100_000)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the path shards by tile range, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    from pylblrtm_amd import _capi, synth
+    from pylblrtm_amd.device import OpticalDepthProblem
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    t = time.perf_counter()
+    sd, prof = synth.five_gas(lines_per_gas=args.lines_per_gas)
+    lay = prof.build_layers()
+    prob = OpticalDepthProblem(sd, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, device=dev)
+    E = prob.count_evaluations()
+    log(f"[bench] setup {time.perf_counter() - t:.1f}s, lines {prob.n_lines}, layers "
+        f"{prob.n_layers}, points {prob.n_nu}, evaluations/step {E:.4e}")
+    mus = (1.0, 0.5)
+
+    out, sps, k_ms, lp_ms = time_mode(prob, args.precision, mus, args.steps, args.warmup, _capi, torch)
+    log(f"[bench] {args.precision}: {sps * 1e3:.1f} ms/step, line-sum kernel {k_ms:.1f} ms, "
+        f"line params {lp_ms:.2f} ms, {E / sps:.3e} evals/s")
+    value = E / sps
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "line-gridpoint evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"mixed": "f32", "fp64": "f64", "surrogate": "bf16+f32"}[args.precision],
+        "data": "synthetic (seeded PCG64 line lists and profiles, generated in-process)",
+        "config": {"workload": "five gases, 500,000 lines, 50 layers, 500-2500 cm-1 at 0.001 "
+                               "cm-1 (2,000,001 points), 25 cm-1 cutoff, 2 cm-1 tiles, mu 1.0 and 0.5",
+                   "precision": args.precision, "lines": prob.n_lines, "layers": prob.n_layers,
+                   "grid_points": prob.n_nu, "mu_view": list(mus)},
+        "evals_per_step": E,
+        "line_sum_kernel_ms": k_ms,
+        "line_params_kernel_ms": lp_ms,
+    }
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+    achieved = E * FLOP_PER_EVAL / (k_ms * 1e-3) / 1e12
+    traffic = None
+    tf_path = os.path.join(ROOT, "profiles", "traffic_voigt_sum_five_gas.json")
+    if os.path.exists(tf_path):
+        with open(tf_path) as f:
+            traffic = json.load(f).get(args.precision)
+    result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                          "frac": achieved / peak, "traffic": traffic,
+                          "kernel": "k_voigt_sum_" + ("fp64" if args.precision == "fp64" else "mixed"),
+                          "flop_per_eval": FLOP_PER_EVAL}
+
+    gpu_layers = {}
+    if not args.no_parity:
+        if args.precision != "fp64":
+            o64, sps64, k64, _ = time_mode(prob, "fp64", mus, args.fp64_steps, 0, _capi, torch)
+            log(f"[bench] fp64: {sps64 * 1e3:.1f} ms/step, kernel {k64:.1f} ms, {E / sps64:.3e} evals/s")
+            result["fp64"] = {"evals_per_s": E / sps64, "ms_per_step": sps64 * 1e3,
+                              "line_sum_kernel_ms": k64,
+                              "roofline_frac_fp64_peak": E * FLOP_PER_EVAL / (k64 * 1e-3) / 1e12
+                              / FP64_PEAK_TFLOPS}
+            dT = {}
+            for m, mu in enumerate(mus):
+                d = (out["transmittance"][m].double() - o64["transmittance"][m].double()).abs().max()
+                dT[str(mu)] = float(d)
+            result["max_abs_dT_vs_fp64"] = dT
+            result["dT_within_1e-5"] = bool(max(dT.values()) <= 1e-5)
+            log(f"[bench] max |dT| {args.precision} vs fp64: {dT}")
+        # per-layer optical depth for the oracle sample (untimed run)
+        for prec in sorted({args.precision, "fp64"}):
+            o = prob.run(prec, mus=(1.0,), want_layers=True)
+            gpu_layers[prec] = o["optical_depth_layers"].double().cpu().numpy()
+            del o
+    if not args.no_cpu_baseline:
+        tile, lo, hi = 500, 20, 30
+        log(f"[bench] cpu oracle sample: tile {tile}, layers {lo}-{hi - 1} ...")
+        r = cpu_baseline(sd, lay, prob, tile, lo, hi,
+                         {f"max_rel_od_{k}": v for k, v in gpu_layers.items()})
+        cpu_rate = r["evals"] / r["seconds"]
+        result["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "line-gridpoint evals/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (scipy wofz), reference tile {tile} (2000 points, "
+                      f"1000-1002 cm-1), layers {lo}-{hi - 1}, all 5 gases: {r['evals']:.3e} "
+                      f"evaluations in {r['seconds']:.1f} s on one core of {cpu_model()} "
+                      f"({os.cpu_count()} logical CPUs visible)",
+            "gpu_over_cpu": value / cpu_rate,
+            "full_workload_cpu_seconds_extrapolated": E / cpu_rate,
+        }
+        result["parity_sample_vs_oracle"] = {k: v for k, v in r.items() if k.startswith("max_rel")}
+        log(f"[bench] cpu {cpu_rate:.3e} evals/s; parity {result['parity_sample_vs_oracle']}")
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

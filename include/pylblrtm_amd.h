@@ -129,6 +129,13 @@ int lbl_optical_depth(const lbl_lines_t *lines, const lbl_groups_t *groups,
 int lbl_voigt_eval(int64_t n, const void *x, const void *y, void *k_out, int precision,
                    void *stream);
 
+/* Kernel timing (no counterpart in the reference; for benchmarks).  When
+ * enabled, lbl_optical_depth records HIP events on its stream around the
+ * per-(layer, line) kernel and the line-sum kernel; lbl_last_timings returns
+ * their durations in ms for the calling thread's last call. */
+void lbl_enable_timing(int on);
+int lbl_last_timings(double *ms_line_params, double *ms_voigt_sum);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char *lbl_last_error(void);
 

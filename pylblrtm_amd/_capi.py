@@ -17,7 +17,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libpylblrtm_amd.so"
 LBL_OK, LBL_EINVAL, LBL_ESTRENGTH, LBL_ENONFINITE, LBL_EHIP = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": 0, "mixed": 1, "surrogate": 2}
 
-EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error")
+EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error",
+            "lbl_enable_timing", "lbl_last_timings")
 
 _vp = ctypes.c_void_p
 
@@ -75,6 +76,10 @@ def load() -> ctypes.CDLL:
     lib.lbl_optical_depth.restype = ctypes.c_int
     lib.lbl_voigt_eval.argtypes = [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, _vp]
     lib.lbl_voigt_eval.restype = ctypes.c_int
+    lib.lbl_enable_timing.argtypes = [ctypes.c_int]
+    lib.lbl_enable_timing.restype = None
+    lib.lbl_last_timings.argtypes = [P(ctypes.c_double), P(ctypes.c_double)]
+    lib.lbl_last_timings.restype = ctypes.c_int
     lib.lbl_last_error.argtypes = []
     lib.lbl_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -95,3 +100,15 @@ def raise_for(status: int, what: str) -> None:
     if status in (LBL_ESTRENGTH, LBL_ENONFINITE):
         raise FloatingPointError(msg)
     raise RuntimeError(msg)
+
+
+def enable_timing(on: bool = True) -> None:
+    load().lbl_enable_timing(1 if on else 0)
+
+
+def last_timings():
+    """(ms of the per-(layer, line) kernel, ms of the line-sum kernel) of the
+    calling thread's last lbl_optical_depth call with timing enabled."""
+    a, b = ctypes.c_double(0), ctypes.c_double(0)
+    load().lbl_last_timings(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value

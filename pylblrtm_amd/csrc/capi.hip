@@ -12,6 +12,30 @@ namespace {
 
 thread_local char g_last_error[512] = "";
 
+// Optional per-kernel timing of lbl_optical_depth with HIP events recorded on
+// the caller's stream (one event set per device, created on first use).
+constexpr int kMaxDevices = 64;
+bool g_timing = false;
+hipEvent_t g_events[kMaxDevices][4];
+bool g_events_made[kMaxDevices] = {};
+thread_local double g_ms_line_params = 0.0, g_ms_voigt_sum = 0.0;
+
+hipError_t timing_events(hipEvent_t** ev) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!g_events_made[dev]) {
+    for (int i = 0; i < 4; ++i) {
+      e = hipEventCreate(&g_events[dev][i]);
+      if (e != hipSuccess) return e;
+    }
+    g_events_made[dev] = true;
+  }
+  *ev = g_events[dev];
+  return hipSuccess;
+}
+
 int fail(int code, const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -93,6 +117,14 @@ extern "C" {
 
 const char* lbl_last_error(void) { return g_last_error; }
 
+void lbl_enable_timing(int on) { g_timing = on != 0; }
+
+int lbl_last_timings(double* ms_line_params, double* ms_voigt_sum) {
+  if (ms_line_params) *ms_line_params = g_ms_line_params;
+  if (ms_voigt_sum) *ms_voigt_sum = g_ms_voigt_sum;
+  return LBL_OK;
+}
+
 int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
                         const lbl_grid_t* grid, int precision, size_t* bytes) {
   Layout L;
@@ -128,10 +160,15 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
   int32_t* lo = reinterpret_cast<int32_t*>(ws + L.lo);
   int32_t* hi = reinterpret_cast<int32_t*>(ws + L.hi);
 
+  hipEvent_t* ev = nullptr;
+  if (g_timing) HIP_TRY(timing_events(&ev));
+
   HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
 
   lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err};
+  if (ev) HIP_TRY(hipEventRecord(ev[0], s));
   HIP_TRY(lbl::launch_line_params(lp, precision, s));
+  if (ev) HIP_TRY(hipEventRecord(ev[1], s));
 
   // |centre - nu0| <= |shift| * P_air <= shift_bound: the candidate range per
   // tile is widened by it (and a rounding slack); the per-record test is exact.
@@ -172,11 +209,20 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
   sa.tau = out->optical_depth;
   sa.tau_layers = out->optical_depth_layers;
   sa.err = err;
+  if (ev) HIP_TRY(hipEventRecord(ev[2], s));
   HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));
+  if (ev) HIP_TRY(hipEventRecord(ev[3], s));
 
   uint32_t code = lbl::kErrNone;
   HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (ev) {
+    float a = 0.f, b = 0.f;
+    HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, ev[2], ev[3]));
+    g_ms_line_params = a;
+    g_ms_voigt_sum = b;
+  }
   if (code != lbl::kErrNone) {
     const int kind2 = (int)(code >> lbl::kErrGroupBits);
     const int layer = kind2 >> 1;

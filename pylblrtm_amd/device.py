@@ -133,6 +133,10 @@ class OpticalDepthProblem:
             gid = np.zeros(0, np.int32)
         self.n_lines = int(idx.size)
         self.line_index = idx
+        self.layers = layers
+        self._host_nu0 = np.asarray(lines["nu0"], np.float64)[idx]
+        self._host_shift = np.asarray(lines["shift"], np.float64)[idx]
+        self._host_group = gid
         p_max = float(layers.pressure_atm.max()) if self.n_layers > 0 else 1.0
         shift_sel = np.asarray(lines["shift"], np.float64)[idx]
         self.shift_bound = float(np.max(np.abs(shift_sel)) * p_max) if idx.size else 0.0
@@ -177,6 +181,40 @@ class OpticalDepthProblem:
         self.c_grid = _capi.LblGrid(self.nu_min, self.step, self.n_nu, self.tile_samples,
                                     self.cutoff, self.shift_bound)
         self._ws = {}
+
+    # ------------------------------------------------------------ work count --
+    def tile_edges(self):
+        """(first point, end point, window low, window high) per reference tile."""
+        ts = self.tile_samples
+        t0 = np.arange(0, self.n_nu, ts, dtype=np.int64)
+        t1 = np.minimum(t0 + ts, self.n_nu)
+        return t0, t1, self.nu[t0] - self.cutoff, self.nu[t1 - 1] + self.cutoff
+
+    def count_evaluations(self, layers: Optional[Sequence[int]] = None,
+                          tiles: Optional[Sequence[int]] = None) -> int:
+        """Exact number of (layer, line, grid point) Voigt evaluations the
+        reference performs: sum over layers and tiles of the lines inside the
+        tile window times the tile's points (forward.py:248-309), for groups
+        with a non-zero column."""
+        t0, t1, lo_b, hi_b = self.tile_edges()
+        if tiles is not None:
+            sel = np.asarray(tiles, np.int64)
+            t0, t1, lo_b, hi_b = t0[sel], t1[sel], lo_b[sel], hi_b[sel]
+        npts = t1 - t0
+        total = 0
+        ks = range(self.n_layers) if layers is None else layers
+        lay = self.layers
+        for k in ks:
+            P = float(lay.pressure_atm[k])
+            for j, g in enumerate(self.groups):
+                if float(lay.column_cm2[g.gas][k]) == 0.0:
+                    continue
+                m = self._host_group == j
+                p_self = float(lay.vmr[g.gas][k]) * P
+                c = np.sort(self._host_nu0[m] + self._host_shift[m] * (P - p_self))
+                cnt = np.searchsorted(c, hi_b, side="right") - np.searchsorted(c, lo_b, side="left")
+                total += int(np.sum(np.maximum(cnt, 0) * npts))
+        return total
 
     # ------------------------------------------------------------------ run --
     def workspace(self, precision: int):
