@@ -167,11 +167,24 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_fp64(const SumArgs a) {
 
 // ----------------------------------------------------------------- mixed --
 
+// One staged record of the mixed kernel (48 bytes, 16-byte aligned):
+//   c (FP64 centre), s = sqrt(ln2)/alpha_D, hs = step * s        -> 16 B
+//   cw = y^2 - 1/2, d2 = 2 y^2, ay = A y / sqrt(pi), hw           -> 16 B
+//   y, A (used by the full dispatch only), 2 pad                  -> 16 B
+// hw = sqrt(max(R^2 - y^2, 0)) is the |x| beyond which the two-node form
+// holds (R^2 = kF32Gh2R2), so a lane's P points are all in that region iff
+// x_first >= hw or x_last <= -hw.
+struct alignas(16) MixedRec {
+  double c;
+  float s, hs;
+  float cw, d2, ay, hw;
+  float y, a, pad0, pad1;
+};
+static_assert(sizeof(MixedRec) == 48, "record layout");
+
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
-  __shared__ double s_c[kChunk];
-  __shared__ float4 s_p0[kChunk];  // s, step*s, 2y^2 - 1, (y^2 + 1/2)^2
-  __shared__ float4 s_p1[kChunk];  // A y/sqrt(pi), A y (y^2 + 1/2)/sqrt(pi), y, A
+  __shared__ MixedRec s_rec[kChunk];
 
   const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
   const int64_t tile = wg / a.slices_per_tile;
@@ -204,39 +217,52 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
         const int64_t r = kb + base + threadIdx.x;
         double c = a.rec.c[r];
         float s = rs[r], y = ry[r], am = ra[r];
-        if (!(c >= lo_b && c <= hi_b)) {
+        if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped)
           c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
         }
-        const float y2 = y * y, cp = y2 + 0.5f;
-        const float ay = am * y * (float)kInvSqrtPi;
-        s_c[threadIdx.x] = c;
-        s_p0[threadIdx.x] = make_float4(s, stepf * s, 2.0f * y2 - 1.0f, cp * cp);
-        s_p1[threadIdx.x] = make_float4(ay, ay * cp, y, am);
+        const float y2 = y * y;
+        MixedRec rec;
+        rec.c = c;
+        rec.s = s;
+        rec.hs = stepf * s;
+        rec.cw = y2 - 0.5f;
+        rec.d2 = 2.0f * y2;
+        rec.ay = am * y * (float)kInvSqrtPi;
+        rec.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
+        rec.y = y;
+        rec.a = am;
+        rec.pad0 = rec.pad1 = 0.0f;
+        s_rec[threadIdx.x] = rec;
       }
       __syncthreads();
       for (int q = 0; q < cnt; ++q) {
-        const double c = s_c[q];
-        const float4 p0 = s_p0[q];
-        const float4 p1 = s_p1[q];
-        const float x0 = (float)(nu_l - c) * p0.x;
-        const float hs = p0.y;
+        const MixedRec& R = s_rec[q];
+        const double c = R.c;
+        const float s = R.s, hs = R.hs;
+        const float x0 = (float)(nu_l - c) * s;
         const float xb = fmaf((float)(P - 1), hs, x0);
-        const float mn = x0 > 0.0f ? x0 : (xb < 0.0f ? -xb : 0.0f);
-        const float y = p1.z;
-        if (fmaf(mn, mn, y * y) >= kF32Gh2R2) {
+        const float hw = R.hw;
+        if (x0 >= hw || xb <= -hw) {
+          // two-node form: K = (y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
+          const float cw = R.cw, d2 = R.d2, ay = R.ay;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, hs, x0);
-            const float x2 = x * x;
-            const float D = fmaf(x2, x2 + p0.z, p0.w);
-            const float N = fmaf(p1.x, x2, p1.y);
+            const float w = fmaf(x, x, cw);
+            const float D = fmaf(w, w, d2);
+            const float N = fmaf(ay, w, ay);
             part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
           }
         } else {
-#pragma unroll
+          // One instance of the full dispatch, looped over the P points; the
+          // value lands in part[j] through a static scatter so part[] stays
+          // in registers.
+          const float y = R.y, am = R.a;
+#pragma unroll 1
           for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j, hs, x0);
-            part[j] = fmaf(p1.w, K_mixed(x, y), part[j]);
+            const float v = am * K_mixed(fmaf((float)j, hs, x0), y);
+#pragma unroll
+            for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
           }
         }
         if ((q & (kFoldEvery - 1)) == kFoldEvery - 1) {
@@ -259,8 +285,9 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
     for (int j = 0; j < P; ++j) {
       const int64_t i = i0 + j;
       if (i < t1) {
-        const double nu = grid_nu(a.nu_min, a.step, i);
-        const float R = (float)(nu * tanh(0.5 * kC2 * nu / T));
+        // R(nu, T) in FP32 (relative error ~1e-7, far inside the mixed budget)
+        const float nu = (float)grid_nu(a.nu_min, a.step, i);
+        const float R = nu * tanhf(0.5f * (float)kC2 * nu / (float)T);
         const float v = (lay[j] - layc[j]) * R;
         bad |= !isfinite(v);
         kahan_add(tau[j], tauc[j], v);
