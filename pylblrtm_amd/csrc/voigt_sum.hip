@@ -37,6 +37,7 @@ namespace {
 
 constexpr int kChunk = kBlock;  // records staged per LDS round
 constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
+constexpr int kQueueCap = 8192; // near-centre (record, thread) pairs per chunk (16-bit entries)
 
 // Hardware assigns workgroup b to XCD b % 8.  Give every XCD a contiguous
 // run of logical workgroups so neighbouring tiles (which share most of their
@@ -182,24 +183,41 @@ struct alignas(16) MixedRec {
 };
 static_assert(sizeof(MixedRec) == 48, "record layout");
 
+// Four waves per SIMD (<= 128 VGPRs) lets every tile of a 2 cm^-1 x 0.001
+// grid (1001 workgroups) be resident at once on 256 CUs; at three waves the
+// last quarter of the tiles ran as a second, one-wave-per-SIMD round.
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_voigt_sum_mixed(const SumArgs a) {
   __shared__ MixedRec s_rec[kChunk];
+  // Near-centre work queue: (record, thread) pairs whose P points are not all
+  // in the two-node region are appended here instead of being evaluated by
+  // a wavefront that is otherwise on the fast path, then drained densely by
+  // the whole workgroup at the end of the chunk.  Results come back through
+  // s_acc (LDS float atomics) into the owning thread's partial sums.
+  __shared__ uint16_t s_q[kQueueCap];
+  __shared__ float s_acc[kBlock * P];
+  __shared__ int s_qn;
 
   const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
   const int64_t tile = wg / a.slices_per_tile;
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
-  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)threadIdx.x * P;
+  const int64_t s0 = t0 + (int64_t)slice * (kBlock * P);   // first point of the slice
+  const int64_t i0 = s0 + (int64_t)threadIdx.x * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
   const double nu_l = grid_nu(a.nu_min, a.step, i0);
+  const bool live = i0 < t1;   // lanes wholly past the tile end never write
   const float stepf = (float)a.step;
   const float* rs = static_cast<const float*>(a.rec.s);
   const float* ry = static_cast<const float*>(a.rec.y);
   const float* ra = static_cast<const float*>(a.rec.a);
+
+  for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0f;
+  if (threadIdx.x == 0) s_qn = 0;
 
   float tau[P], tauc[P];
 #pragma unroll
@@ -253,16 +271,25 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
             const float N = fmaf(ay, w, ay);
             part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
           }
-        } else {
-          // One instance of the full dispatch, looped over the P points; the
-          // value lands in part[j] through a static scatter so part[] stays
-          // in registers.
-          const float y = R.y, am = R.a;
+        } else if (live) {
+          // append (q, thread) with one LDS atomic per wavefront
+          const uint64_t m = __ballot(1);
+          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          int qb = 0;
+          if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(m));
+          qb = __builtin_amdgcn_readfirstlane(qb);
+          const int pos = qb + (int)rank;
+          if (pos < kQueueCap) {
+            s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
+          } else {  // queue full: evaluate here
+            const float y = R.y, am = R.a;
 #pragma unroll 1
-          for (int j = 0; j < P; ++j) {
-            const float v = am * K_mixed(fmaf((float)j, hs, x0), y);
+            for (int j = 0; j < P; ++j) {
+              const float v = am * K_mixed(fmaf((float)j, hs, x0), y);
 #pragma unroll
-            for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
+              for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
+            }
           }
         }
         if ((q & (kFoldEvery - 1)) == kFoldEvery - 1) {
@@ -272,6 +299,26 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_mixed(const SumArgs a) {
             part[j] = 0.0f;
           }
         }
+      }
+      __syncthreads();
+      const int nq = min(s_qn, kQueueCap);
+      if (nq > 0) {
+        for (int e = threadIdx.x; e < nq; e += kBlock) {
+          const uint32_t ent = s_q[e];
+          const int q = (int)(ent >> 8), t = (int)(ent & 0xFF);
+          const MixedRec& R = s_rec[q];
+          const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P);
+          const float x0 = (float)(nu_t - R.c) * R.s;
+          for (int j = 0; j < P; ++j)
+            atomicAdd(&s_acc[t * P + j], R.a * K_mixed(fmaf((float)j, R.hs, x0), R.y));
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          part[j] += s_acc[threadIdx.x * P + j];
+          s_acc[threadIdx.x * P + j] = 0.0f;
+        }
+        if (threadIdx.x == 0) s_qn = 0;
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
