@@ -129,6 +129,23 @@ int lbl_optical_depth(const lbl_lines_t *lines, const lbl_groups_t *groups,
 int lbl_voigt_eval(int64_t n, const void *x, const void *y, void *k_out, int precision,
                    void *stream);
 
+/* Adjoint of the transmittance (no counterpart in the reference: the
+ * gradients retrievals need).  Given the forward transmittance T_m(nu)
+ * (float, n_mu x n_nu, from lbl_optical_depth with LBL_MIXED) and an upstream
+ * gradient G_m(nu) = dL/dT_m(nu) (float, same shape), returns
+ *   grad_t[k]          = dL/dT_k            (double, n_layers)
+ *   grad_vmr[g][k]     = dL/dvmr_{k,g}      (double, n_gases x n_layers)
+ * with the column of gas g in layer k taken as vmr_{k,g} * (air column of
+ * layer k), i.e. lbl_layers.col must equal vmr * air.  The Voigt values are
+ * recomputed (FP32), nothing of size lines x grid is stored. */
+int lbl_vjp_workspace_bytes(const lbl_lines_t *lines, const lbl_layers_t *layers,
+                            const lbl_grid_t *grid, size_t *bytes);
+int lbl_transmittance_vjp(const lbl_lines_t *lines, const lbl_groups_t *groups,
+                          const lbl_layers_t *layers, const lbl_grid_t *grid, int32_t n_mu,
+                          const double *mu, const float *transmittance,
+                          const float *grad_transmittance, double *grad_t, double *grad_vmr,
+                          void *workspace, size_t workspace_bytes, void *stream);
+
 /* Kernel timing (no counterpart in the reference; for benchmarks).  When
  * enabled, lbl_optical_depth records HIP events on its stream around the
  * per-(layer, line) kernel and the line-sum kernel; lbl_last_timings returns

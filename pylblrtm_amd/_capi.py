@@ -18,7 +18,8 @@ LBL_OK, LBL_EINVAL, LBL_ESTRENGTH, LBL_ENONFINITE, LBL_EHIP = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": 0, "mixed": 1, "surrogate": 2}
 
 EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error",
-            "lbl_enable_timing", "lbl_last_timings")
+            "lbl_enable_timing", "lbl_last_timings", "lbl_vjp_workspace_bytes",
+            "lbl_transmittance_vjp")
 
 _vp = ctypes.c_void_p
 
@@ -76,6 +77,13 @@ def load() -> ctypes.CDLL:
     lib.lbl_optical_depth.restype = ctypes.c_int
     lib.lbl_voigt_eval.argtypes = [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, _vp]
     lib.lbl_voigt_eval.restype = ctypes.c_int
+    lib.lbl_vjp_workspace_bytes.argtypes = [P(LblLines), P(LblLayers), P(LblGrid),
+                                            P(ctypes.c_size_t)]
+    lib.lbl_vjp_workspace_bytes.restype = ctypes.c_int
+    lib.lbl_transmittance_vjp.argtypes = [P(LblLines), P(LblGroups), P(LblLayers), P(LblGrid),
+                                          ctypes.c_int32, P(ctypes.c_double), _vp, _vp, _vp, _vp,
+                                          _vp, ctypes.c_size_t, _vp]
+    lib.lbl_transmittance_vjp.restype = ctypes.c_int
     lib.lbl_enable_timing.argtypes = [ctypes.c_int]
     lib.lbl_enable_timing.restype = None
     lib.lbl_last_timings.argtypes = [P(ctypes.c_double), P(ctypes.c_double)]

@@ -1,0 +1,353 @@
+// Adjoint (vector-Jacobian product) of the transmittance with respect to the
+// layer temperatures T_k and the layer mixing ratios v_{k,g}, with the layer
+// column of gas g taken as v_{k,g} * air_k (so d col / d v = air_k).
+//
+// Forward model (pylblrtm/forward.py:257-312, :151):
+//   T_m(nu) = exp(-tau(nu)/mu_m),  tau = sum_k R(nu, T_k) sum_l A_kl K(x_kl, y_kl),
+//   A = S sqrt(ln2/pi)/alpha_D,  x = s (nu - c),  y = sqrt(ln2) gamma_L / alpha_D.
+// Given G_m(nu) = dL/dT_m(nu), with W(nu) = -sum_m G_m T_m / mu_m,
+//   dL/dtheta = sum_nu W(nu) d tau(nu)/d theta.
+// Per record and parameter theta the chain rule needs d ln A/d theta,
+// dx/d theta = bx + ax * x and dy/d theta; k_line_grad computes them in FP64:
+//   temperature: d ln A/dT = -Q'(T)/Q(T) + c2 E''/T^2 - (a/T^2)/expm1(a/T)
+//                           + (c2 c/T^2)/sinh(c2 c/T) - 1/(2T),   a = c2 nu0
+//                ax = -1/(2T), bx = 0, dy/dT = -y (n_air + 1/2)/T
+//   mixing ratio (c = nu0 + delta P (1 - v)):
+//                d ln A/dv = 1/v + delta P (1/c + (c2/T)/sinh(c2 c/T))
+//                ax = 0, bx = s delta P, dy/dv = sqrt(ln2) (296/T)^n P (g_self - g_air)/alpha_D
+// and, at every grid point, dR/dT = -R (c2 nu/T^2)/sinh(c2 nu/T).
+// The backward sum recomputes K and its derivatives (no lines x grid storage):
+//   far (|z| >= 25, the forward's two-node form, w = x^2 + y^2 - 1/2, D = w^2 + 2y^2):
+//     K = y (w+1)/(sqrt(pi) D),  dK/dx = 2xy (2y^2 - w^2 - 2w)/(sqrt(pi) D^2),
+//     dK/dy = ((w + 1 + 2y^2) D - 4 y^2 (w+1)^2)/(sqrt(pi) D^2)
+//   near: K, L = Re, Im w from W_mixed, dK/dx = -2(xK - yL), dK/dy = 2(xL + yK) - 2/sqrt(pi).
+#include <math.h>
+
+#include "faddeeva.h"
+#include "internal.h"
+
+namespace lbl {
+namespace {
+
+constexpr int kVjpPoints = 8;
+constexpr int kVjpChunk = kBlock;
+constexpr int kVjpFold = 32;
+
+struct GradArgs {
+  lbl_lines_t lines;
+  lbl_groups_t groups;
+  lbl_layers_t layers;
+  float *at, *dyt, *av, *bxv, *dyv;   // [n_layers * n_lines]
+};
+
+// Slope of the numpy.interp segment used at x (0 outside the table).
+__device__ double interp_slope(double x, const double* xp, const double* fp, int n) {
+  if (n < 2 || !(x >= xp[0]) || !(x <= xp[n - 1])) return 0.0;
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  return (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+}
+
+__device__ double interp_value(double x, const double* xp, const double* fp, int n) {
+  if (n == 1 || x <= xp[0]) return fp[0];
+  if (x >= xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid; else hi = mid;
+  }
+  return fp[lo] + (x - xp[lo]) * (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+}
+
+__global__ __launch_bounds__(256) void k_line_grad(const GradArgs a) {
+  __shared__ double s_dlnq[kMaxGroups];
+  const int k = blockIdx.y;
+  const double T = a.layers.t_k[k];
+  const double P = a.layers.p_atm[k];
+  for (int g = threadIdx.x; g < a.groups.n_groups; g += blockDim.x) {
+    const int o = a.groups.tips_offset[g];
+    const int n = a.groups.tips_offset[g + 1] - o;
+    const double* tt = a.groups.tips_t + o;
+    const double* qq = a.groups.tips_q + o;
+    s_dlnq[g] = -interp_slope(T, tt, qq, n) / interp_value(T, tt, qq, n);
+  }
+  __syncthreads();
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= a.lines.n_lines) return;
+  const int64_t out = (int64_t)k * a.lines.n_lines + l;
+  const int g = a.lines.group[l];
+  const int gas = a.groups.gas[g];
+  const double v = a.layers.vmr[(int64_t)gas * a.layers.n_layers + k];
+  const double col = a.layers.col[(int64_t)gas * a.layers.n_layers + k];
+  if (col == 0.0 || v == 0.0) {
+    a.at[out] = a.dyt[out] = a.av[out] = a.bxv[out] = a.dyv[out] = 0.0f;
+    return;
+  }
+  const double nu0 = a.lines.nu0[l];
+  const double delta = a.lines.shift[l];
+  const double n_air = a.lines.n_air[l];
+  const double p_self = v * P, p_air = P - p_self;
+  const double c = nu0 + delta * p_air;
+  const double doppler = nu0 * kDoppler * sqrt(T / a.groups.molar_mass[g]);
+  const double gscale = pow(kTRef / T, n_air);
+  const double lorentz = gscale * (a.lines.g_air[l] * p_air + a.lines.g_self[l] * p_self);
+  const double y = kSqrtLn2 * lorentz / doppler;
+  const double s = kSqrtLn2 / doppler;
+  const double aa = kC2 * nu0 / T;
+  const double u = kC2 * c / T;
+  const double inv_t = 1.0 / T;
+  a.at[out] = (float)(s_dlnq[g] + kC2 * a.lines.elow[l] * inv_t * inv_t -
+                      (aa * inv_t) / expm1(aa) + (u * inv_t) / sinh(u) - 0.5 * inv_t);
+  a.dyt[out] = (float)(-y * (n_air + 0.5) * inv_t);
+  a.av[out] = (float)(1.0 / v + delta * P * (1.0 / c + (kC2 * inv_t) / sinh(u)));
+  a.bxv[out] = (float)(s * delta * P);
+  a.dyv[out] = (float)(kSqrtLn2 * gscale * P * (a.lines.g_self[l] - a.lines.g_air[l]) / doppler);
+}
+
+struct VjpArgs {
+  Records rec;                 // mixed-precision records of the forward
+  const float *at, *dyt, *av, *bxv, *dyv;
+  const int32_t* group;        // per line
+  const int32_t* gas_of_group;
+  const int32_t* lo;
+  const int32_t* hi;
+  const double* t_k;
+  int64_t n_lines;
+  int32_t n_layers, n_gases;
+  double nu_min, step, cutoff;
+  int64_t n_nu, ts;
+  int32_t slices_per_tile;
+  int64_t n_wg;
+  int32_t n_mu;
+  double inv_mu[kMaxMu];
+  const float* trans;          // [n_mu][n_nu] forward transmittance
+  const float* grad_trans;     // [n_mu][n_nu] upstream gradient
+  double* grad_t;              // [n_layers]
+  double* grad_vmr;            // [n_gases][n_layers]
+};
+
+struct alignas(16) VjpRec {
+  double c;
+  float s, hs;
+  float cw, d2, ay, aq;      // y^2 - 1/2, 2 y^2, A y/sqrt(pi), A/sqrt(pi)
+  float hw, y, a, at;        // |x| threshold, y, A, d ln A/dT
+  float dyt, av, bxv, dyv;
+  int32_t gas, pad0, pad1, pad2;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
+  __shared__ VjpRec s_rec[kVjpChunk];
+  const int64_t wg = blockIdx.x;
+  const int64_t tile = wg / a.slices_per_tile;
+  const int slice = (int)(wg - tile * a.slices_per_tile);
+  const int64_t t0 = tile * a.ts;
+  const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)threadIdx.x * P;
+  const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
+  const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
+  const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
+  const double nu_l = grid_nu(a.nu_min, a.step, i0);
+  const float stepf = (float)a.step;
+  const float* rs = static_cast<const float*>(a.rec.s);
+  const float* ry = static_cast<const float*>(a.rec.y);
+  const float* ra = static_cast<const float*>(a.rec.a);
+
+  float W[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int64_t i = i0 + j;
+    float w = 0.0f;
+    if (i < t1)
+      for (int m = 0; m < a.n_mu; ++m)
+        w -= a.grad_trans[(int64_t)m * a.n_nu + i] * a.trans[(int64_t)m * a.n_nu + i] *
+             (float)a.inv_mu[m];
+    W[j] = w;
+  }
+
+  for (int k = 0; k < a.n_layers; ++k) {
+    const float T = (float)a.t_k[k];
+    const float ax_t = -0.5f / T;
+    float WR[P], lay[P], part[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const float nu = (float)grid_nu(a.nu_min, a.step, i0 + j);
+      WR[j] = W[j] * nu * tanhf(0.5f * (float)kC2 * nu / T);
+      lay[j] = part[j] = 0.0f;
+    }
+    double g_t = 0.0;
+    double g_v[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g) g_v[g] = 0.0;
+    const int64_t kb = (int64_t)k * a.n_lines;
+    for (int32_t base = rlo; base < rhi; base += kVjpChunk) {
+      const int cnt = min(kVjpChunk, rhi - base);
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) {
+        const int64_t r = kb + base + threadIdx.x;
+        double c = a.rec.c[r];
+        float s = rs[r], y = ry[r], am = ra[r];
+        VjpRec q;
+        q.at = a.at[r]; q.dyt = a.dyt[r]; q.av = a.av[r]; q.bxv = a.bxv[r]; q.dyv = a.dyv[r];
+        if (!(c >= lo_b && c <= hi_b)) {
+          c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
+          q.at = q.dyt = q.av = q.bxv = q.dyv = 0.0f;
+        }
+        const float y2 = y * y;
+        q.c = c; q.s = s; q.hs = stepf * s;
+        q.cw = y2 - 0.5f; q.d2 = 2.0f * y2;
+        q.ay = am * y * (float)kInvSqrtPi; q.aq = am * (float)kInvSqrtPi;
+        q.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); q.y = y; q.a = am;
+        q.gas = a.gas_of_group[a.group[base + threadIdx.x]];
+        q.pad0 = q.pad1 = q.pad2 = 0;
+        s_rec[threadIdx.x] = q;
+      }
+      __syncthreads();
+      for (int qi = 0; qi < cnt; ++qi) {
+        const VjpRec& R = s_rec[qi];
+        const float x0 = (float)(nu_l - R.c) * R.s;
+        const float hs = R.hs;
+        const float xb = fmaf((float)(P - 1), hs, x0);
+        float acc_t = 0.0f, acc_v = 0.0f;
+        if (x0 >= R.hw || xb <= -R.hw) {
+          const float cw = R.cw, d2 = R.d2, ay = R.ay, aq = R.aq;
+          const float at = R.at, dyt = R.dyt, av = R.av, bxv = R.bxv, dyv = R.dyv;
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, hs, x0);
+            const float w = fmaf(x, x, cw);
+            const float D = fmaf(w, w, d2);
+            const float rD = __builtin_amdgcn_rcpf(D);
+            const float w1 = w + 1.0f;
+            const float AK = ay * w1 * rD;
+            const float r2 = rD * rD;
+            const float Kx = 2.0f * ay * x * (d2 - w * (w + 2.0f)) * r2;
+            const float Ky = aq * ((w1 + d2) * D - 2.0f * d2 * w1 * w1) * r2;
+            part[j] += AK;
+            acc_t = fmaf(WR[j], at * AK + Kx * (ax_t * x) + Ky * dyt, acc_t);
+            acc_v = fmaf(WR[j], av * AK + Kx * bxv + Ky * dyv, acc_v);
+          }
+        } else {
+#pragma unroll 1
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, hs, x0);
+            float K, L;
+            W_mixed(x, R.y, K, L);
+            const float AK = R.a * K;
+            const float Kx = -2.0f * R.a * (x * K - R.y * L);
+            const float Ky = R.a * (2.0f * (x * L + R.y * K) - 2.0f * (float)kInvSqrtPi);
+            float wr = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < P; ++jj) {
+              wr = (jj == j) ? WR[jj] : wr;
+              part[jj] += (jj == j) ? AK : 0.0f;
+            }
+            acc_t = fmaf(wr, R.at * AK + Kx * (ax_t * x) + Ky * R.dyt, acc_t);
+            acc_v = fmaf(wr, R.av * AK + Kx * R.bxv + Ky * R.dyv, acc_v);
+          }
+        }
+        g_t += (double)acc_t;
+        const int gas = __builtin_amdgcn_readfirstlane(R.gas);
+#pragma unroll
+        for (int g = 0; g < 8; ++g)
+          if (g == gas) g_v[g] += (double)acc_v;
+        if ((qi & (kVjpFold - 1)) == kVjpFold - 1) {
+#pragma unroll
+          for (int j = 0; j < P; ++j) { lay[j] += part[j]; part[j] = 0.0f; }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) { lay[j] += part[j]; part[j] = 0.0f; }
+    }
+    // radiation-term derivative at the grid points
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const float nu = (float)grid_nu(a.nu_min, a.step, i0 + j);
+      const float u = (float)kC2 * nu / T;
+      const float dlnr = -(u / T) / sinhf(u);
+      g_t += (double)(WR[j] * dlnr * lay[j]);
+    }
+    g_t = wave_sum(g_t);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&a.grad_t[k], g_t);
+    for (int g = 0; g < a.n_gases && g < 8; ++g) {
+      const double sv = wave_sum(g_v[g]);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&a.grad_vmr[(int64_t)g * a.n_layers + k], sv);
+    }
+  }
+}
+
+__global__ void k_fill_f64(double* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+}  // namespace
+
+int vjp_points_per_thread() { return kVjpPoints; }
+
+hipError_t launch_line_grad(const lbl_lines_t& lines, const lbl_groups_t& groups,
+                            const lbl_layers_t& layers, float* at, float* dyt, float* av,
+                            float* bxv, float* dyv, hipStream_t s) {
+  if (lines.n_lines <= 0 || layers.n_layers <= 0) return hipSuccess;
+  GradArgs g{lines, groups, layers, at, dyt, av, bxv, dyv};
+  dim3 grid((unsigned)((lines.n_lines + 255) / 256), (unsigned)layers.n_layers);
+  k_line_grad<<<grid, 256, 0, s>>>(g);
+  return hipGetLastError();
+}
+
+hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, const float* av,
+                      const float* bxv, const float* dyv, const int32_t* group,
+                      const int32_t* gas_of_group, const int32_t* lo, const int32_t* hi,
+                      const double* t_k, int64_t n_lines, int n_layers, int n_gases,
+                      const lbl_grid_t& grid, int64_t n_tiles, int n_mu, const double* mu,
+                      const float* trans, const float* grad_trans, double* grad_t,
+                      double* grad_vmr, hipStream_t s) {
+  const int64_t nt = (int64_t)n_layers * (1 + n_gases);
+  if (n_layers > 0) {
+    k_fill_f64<<<(unsigned)((n_layers + 255) / 256), 256, 0, s>>>(grad_t, n_layers);
+    if (n_gases > 0)
+      k_fill_f64<<<(unsigned)(((int64_t)n_gases * n_layers + 255) / 256), 256, 0, s>>>(
+          grad_vmr, (int64_t)n_gases * n_layers);
+  }
+  (void)nt;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || n_tiles <= 0 || n_layers <= 0) return e;
+  VjpArgs a{};
+  a.rec = rec;
+  a.at = at; a.dyt = dyt; a.av = av; a.bxv = bxv; a.dyv = dyv;
+  a.group = group;
+  a.gas_of_group = gas_of_group;
+  a.lo = lo;
+  a.hi = hi;
+  a.t_k = t_k;
+  a.n_lines = n_lines;
+  a.n_layers = n_layers;
+  a.n_gases = n_gases;
+  a.nu_min = grid.nu_min;
+  a.step = grid.step;
+  a.cutoff = grid.cutoff;
+  a.n_nu = grid.n_nu;
+  a.ts = grid.tile_samples;
+  const int64_t pts = (int64_t)kBlock * kVjpPoints;
+  a.slices_per_tile = (int32_t)((grid.tile_samples + pts - 1) / pts);
+  a.n_wg = n_tiles * a.slices_per_tile;
+  a.n_mu = n_mu;
+  for (int m = 0; m < n_mu; ++m) a.inv_mu[m] = 1.0 / mu[m];
+  a.trans = trans;
+  a.grad_trans = grad_trans;
+  a.grad_t = grad_t;
+  a.grad_vmr = grad_vmr;
+  k_vjp_mixed<kVjpPoints><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+}  // namespace lbl

@@ -56,11 +56,12 @@ size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
   size_t c, s, y, a, lo, hi, err, total;
+  size_t g[5];   // adjoint coefficient arrays (only with_grad)
   int64_t n_tiles;
 };
 
 bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl_grid_t* grid,
-                 int precision, Layout* L) {
+                 int precision, Layout* L, bool with_grad = false) {
   if (!lines || !layers || !grid) return false;
   if (grid->tile_samples <= 0 || grid->n_nu <= 0) return false;
   const size_t nrec = (size_t)std::max<int64_t>(lines->n_lines, 0) *
@@ -75,6 +76,10 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->lo = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->hi = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->err = off; off = align_up(off + 4);
+  for (int i = 0; i < 5; ++i) {
+    L->g[i] = off;
+    if (with_grad) off = align_up(off + nrec * 4);
+  }
   L->total = off;
   return true;
 }
@@ -233,6 +238,73 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
     return fail(LBL_ESTRENGTH, "Layer %d, group %d: non-finite or negative line strength",
                 layer, group);
   }
+  return LBL_OK;
+}
+
+int lbl_vjp_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
+                            const lbl_grid_t* grid, size_t* bytes) {
+  Layout L;
+  if (!bytes || !make_layout(lines, layers, grid, LBL_MIXED, &L, true))
+    return fail(LBL_EINVAL, "invalid arguments to lbl_vjp_workspace_bytes");
+  *bytes = L.total;
+  return LBL_OK;
+}
+
+int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                          const lbl_layers_t* layers, const lbl_grid_t* grid, int32_t n_mu,
+                          const double* mu, const float* transmittance,
+                          const float* grad_transmittance, double* grad_t, double* grad_vmr,
+                          void* workspace, size_t workspace_bytes, void* stream) {
+  g_last_error[0] = '\0';
+  int rc = check_inputs(lines, groups, layers, grid, LBL_MIXED);
+  if (rc != LBL_OK) return rc;
+  if (n_mu < 1 || n_mu > lbl::kMaxMu || !mu || !transmittance || !grad_transmittance ||
+      !grad_t || !grad_vmr)
+    return fail(LBL_EINVAL, "vjp: need 1..%d viewing cosines and all gradient buffers",
+                lbl::kMaxMu);
+  for (int m = 0; m < n_mu; ++m)
+    if (!(mu[m] > 0.0 && mu[m] <= 1.0)) return fail(LBL_EINVAL, "mu_view must be in (0, 1]");
+  if (layers->n_gases > 8) return fail(LBL_EINVAL, "vjp supports at most 8 gases");
+  Layout L;
+  make_layout(lines, layers, grid, LBL_MIXED, &L, true);
+  if (!workspace || workspace_bytes < L.total)
+    return fail(LBL_EINVAL, "workspace of %zu bytes needed, %zu given", L.total, workspace_bytes);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  char* ws = static_cast<char*>(workspace);
+  lbl::Records rec{reinterpret_cast<double*>(ws + L.c), ws + L.s, ws + L.y, ws + L.a};
+  uint32_t* err = reinterpret_cast<uint32_t*>(ws + L.err);
+  int32_t* lo = reinterpret_cast<int32_t*>(ws + L.lo);
+  int32_t* hi = reinterpret_cast<int32_t*>(ws + L.hi);
+  float* gc[5];
+  for (int i = 0; i < 5; ++i) gc[i] = reinterpret_cast<float*>(ws + L.g[i]);
+
+  HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
+  lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err};
+  HIP_TRY(lbl::launch_line_params(lp, LBL_MIXED, s));
+  lbl::TileRangeArgs tr{};
+  tr.nu0 = lines->nu0;
+  tr.n_lines = lines->n_lines;
+  tr.nu_min = grid->nu_min;
+  tr.step = grid->step;
+  tr.cutoff = grid->cutoff;
+  tr.n_nu = grid->n_nu;
+  tr.ts = grid->tile_samples;
+  tr.n_tiles = L.n_tiles;
+  tr.lo = lo;
+  tr.hi = hi;
+  tr.margin = fabs(grid->shift_bound) * (1.0 + 1e-9) + 1e-9;
+  HIP_TRY(lbl::launch_tile_ranges(tr, s));
+  HIP_TRY(lbl::launch_line_grad(*lines, *groups, *layers, gc[0], gc[1], gc[2], gc[3], gc[4], s));
+  HIP_TRY(lbl::launch_vjp(rec, gc[0], gc[1], gc[2], gc[3], gc[4], lines->group, groups->gas, lo,
+                          hi, layers->t_k, lines->n_lines, layers->n_layers, layers->n_gases,
+                          *grid, L.n_tiles, n_mu, mu, transmittance, grad_transmittance, grad_t,
+                          grad_vmr, s));
+  uint32_t code = lbl::kErrNone;
+  HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (code != lbl::kErrNone)
+    return fail(LBL_ESTRENGTH, "vjp: non-finite or negative line strength (layer %d)",
+                (int)(code >> lbl::kErrGroupBits) >> 1);
   return LBL_OK;
 }
 

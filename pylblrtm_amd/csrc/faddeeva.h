@@ -207,4 +207,82 @@ __device__ inline float K_mixed(float x, float y) {
   return K_cr32(x, y);
 }
 
+// ------------------------------------------- FP32, both parts of w(z) ----
+// K = Re w and L = Im w with the same region split as K_mixed; used by the
+// adjoint, whose derivatives are dK/dx = -2(xK - yL), dK/dy = 2(xL + yK) - 2/sqrt(pi).
+// Imaginary part of a symmetric node pair: (2 x W / pi) (r - t^2) / D.
+
+__device__ inline void W_gh_pairs32(float x, float y, const double* T, const double* Wt, int n,
+                                    float& K, float& L) {
+  const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
+  float sk = 0.0f, sl = 0.0f;
+  for (int k = n - 1; k >= 0; --k) {
+    const float t2 = (float)(T[k] * T[k]);
+    const float u = xmy - t2;
+    const float inv = 1.0f / fmaf(u, u, fxy);
+    sk = fmaf((float)Wt[k] * (r + t2), inv, sk);
+    sl = fmaf((float)Wt[k] * (r - t2), inv, sl);
+  }
+  K = 2.0f * y * (float)kInvPi * sk;
+  L = 2.0f * x * (float)kInvPi * sl;
+}
+
+__device__ inline void W_cr32(float x, float y, float& K, float& L) {
+  const float h = kCr32H;
+  const float q = fabsf(x) * (1.0f / h);
+  const float fr = q - floorf(q);
+  const bool half = !(fr >= 0.25f && fr <= 0.75f);
+  const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
+  float sk = 0.0f, sl = 0.0f;
+  if (half) {
+#pragma unroll
+    for (int n = 4; n >= 0; --n) {
+      const float t = (n + 0.5f) * h, t2 = t * t, u = xmy - t2;
+      const float e = kCr32E5[n] / fmaf(u, u, fxy);
+      sk = fmaf(e, r + t2, sk);
+      sl = fmaf(e, r - t2, sl);
+    }
+    sk *= 2.0f * y;
+    sl *= 2.0f * x;
+  } else {
+#pragma unroll
+    for (int n = 5; n >= 1; --n) {
+      const float t = n * h, t2 = t * t, u = xmy - t2;
+      const float e = kCr32E0[n - 1] / fmaf(u, u, fxy);
+      sk = fmaf(e, r + t2, sk);
+      sl = fmaf(e, r - t2, sl);
+    }
+    sk = 2.0f * y * sk + y / r;
+    sl = 2.0f * x * sl + x / r;
+  }
+  K = sk * h * (float)kInvPi;
+  L = sl * h * (float)kInvPi;
+  if (y < (float)kPi / h) {
+    const float sg = half ? 1.0f : -1.0f;
+    const float a = 2.0f * (float)kPi / h;
+    const float rho = expf(-a * y);
+    const float M = expf(fmaf(y, y - a, -x2));
+    float st, ct, s2, c2;
+    sincospif(2.0f * x / h, &st, &ct);
+    sincosf(2.0f * x * y, &s2, &c2);
+    const float cphi = ct * c2 + st * s2;   // cos(theta - 2xy)
+    const float sphi = st * c2 - ct * s2;   // sin(theta - 2xy)
+    const float g = 2.0f * M / (1.0f + 2.0f * sg * rho * ct + rho * rho);
+    K += g * (sg * cphi + rho * c2);
+    L += g * (sg * sphi - rho * s2);
+  }
+}
+
+__device__ inline void W_mixed(float x, float y, float& K, float& L) {
+  const float r = x * x + y * y;
+  if (r >= kF32Gh2R2) {
+    const double t[1] = {kGh2T}, w[1] = {kGh2W};
+    W_gh_pairs32(x, y, t, w, 1, K, L);
+  } else if (r >= kF32Gh4R2) {
+    W_gh_pairs32(x, y, kGh4T, kGh4W, 2, K, L);
+  } else {
+    W_cr32(x, y, K, L);
+  }
+}
+
 }  // namespace lbl

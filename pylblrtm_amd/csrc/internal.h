@@ -81,6 +81,18 @@ hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s);
 hipError_t launch_voigt_eval(int64_t n, const void* x, const void* y, void* k, int precision,
                              hipStream_t s);
 
+hipError_t launch_line_grad(const lbl_lines_t& lines, const lbl_groups_t& groups,
+                            const lbl_layers_t& layers, float* at, float* dyt, float* av,
+                            float* bxv, float* dyv, hipStream_t s);
+hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, const float* av,
+                      const float* bxv, const float* dyv, const int32_t* group,
+                      const int32_t* gas_of_group, const int32_t* lo, const int32_t* hi,
+                      const double* t_k, int64_t n_lines, int n_layers, int n_gases,
+                      const lbl_grid_t& grid, int64_t n_tiles, int n_mu, const double* mu,
+                      const float* trans, const float* grad_trans, double* grad_t,
+                      double* grad_vmr, hipStream_t s);
+int vjp_points_per_thread();
+
 // Points per thread of the line-sum kernels (a workgroup covers kBlock * P
 // consecutive grid points of one reference tile).
 int sum_points_per_thread(int precision);

@@ -166,6 +166,14 @@ class OpticalDepthProblem:
         for j, gname in enumerate(gases):
             vmr[j, : self.n_layers] = layers.vmr[gname]
             col[j, : self.n_layers] = layers.column_cm2[gname]
+        # air column per layer (col = vmr * air, atmosphere.py:133-139), recovered
+        # from the gases' columns; used when layer mixing ratios are changed.
+        num = np.zeros(nl)
+        den = np.zeros(nl)
+        for gname in gases:
+            num[: self.n_layers] += layers.column_cm2[gname]
+            den[: self.n_layers] += layers.vmr[gname]
+        self.air_cm2 = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)[: self.n_layers]
         self._t["p_atm"] = dev(layers.pressure_atm if self.n_layers else np.zeros(1), f8)
         self._t["t_k"] = dev(layers.temperature_k if self.n_layers else np.ones(1), f8)
         self._t["vmr"] = dev(vmr, f8)
@@ -181,6 +189,60 @@ class OpticalDepthProblem:
         self.c_grid = _capi.LblGrid(self.nu_min, self.step, self.n_nu, self.tile_samples,
                                     self.cutoff, self.shift_bound)
         self._ws = {}
+
+    # ------------------------------------------------------------ layers --
+    def set_layers(self, temperature_k, vmr) -> None:
+        """Replace the layer temperatures [n_layers] and mixing ratios
+        [n_gases, n_layers] (rows in ``self.gases`` order) on the device; the
+        columns become vmr * air column.  Accepts numpy arrays or device
+        tensors."""
+        torch = _torch()
+        t = torch.as_tensor(temperature_k, dtype=torch.float64, device=self.device).reshape(-1)
+        v = torch.as_tensor(vmr, dtype=torch.float64, device=self.device)
+        if t.numel() != self.n_layers or tuple(v.shape) != (len(self.gases), self.n_layers):
+            raise ValueError(f"expected temperature [{self.n_layers}] and vmr "
+                             f"[{len(self.gases)}, {self.n_layers}]")
+        air = torch.as_tensor(self.air_cm2, dtype=torch.float64, device=self.device)
+        self._t["t_k"][: self.n_layers].copy_(t)
+        self._t["vmr"][: len(self.gases), : self.n_layers].copy_(v)
+        self._t["col"][: len(self.gases), : self.n_layers].copy_(v * air[None, :])
+
+    def layer_state(self):
+        """(temperature [n_layers], vmr [n_gases, n_layers]) currently on the device."""
+        return (self._t["t_k"][: self.n_layers].clone(),
+                self._t["vmr"][: len(self.gases), : self.n_layers].clone())
+
+    def vjp(self, transmittance, grad_transmittance, mus: Sequence[float]):
+        """dL/dT_k [n_layers] and dL/dvmr [n_gases, n_layers] (float64 device
+        tensors) for the mixed-precision forward ``transmittance`` [n_mu, n_nu]
+        and upstream gradient ``grad_transmittance`` of the same shape."""
+        torch = _torch()
+        T = transmittance.to(torch.float32).contiguous()
+        G = grad_transmittance.to(device=self.device, dtype=torch.float32).contiguous()
+        if tuple(T.shape) != (len(mus), self.n_nu) or tuple(G.shape) != tuple(T.shape):
+            raise ValueError("transmittance / gradient must be [n_mu, n_nu]")
+        if "vjp" not in self._ws:
+            nbytes = ctypes.c_size_t(0)
+            st = self.lib.lbl_vjp_workspace_bytes(ctypes.byref(self.c_lines),
+                                                  ctypes.byref(self.c_layers),
+                                                  ctypes.byref(self.c_grid), ctypes.byref(nbytes))
+            _capi.raise_for(st, "lbl_vjp_workspace_bytes")
+            self._ws["vjp"] = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8,
+                                          device=self.device)
+        ws = self._ws["vjp"]
+        gt = torch.zeros(max(self.n_layers, 1), dtype=torch.float64, device=self.device)
+        gv = torch.zeros((max(len(self.gases), 1), max(self.n_layers, 1)), dtype=torch.float64,
+                         device=self.device)
+        mu_arr = (ctypes.c_double * len(mus))(*[float(m) for m in mus])
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        with torch.cuda.device(self.device):
+            st = self.lib.lbl_transmittance_vjp(
+                ctypes.byref(self.c_lines), ctypes.byref(self.c_groups), ctypes.byref(self.c_layers),
+                ctypes.byref(self.c_grid), len(mus), mu_arr, T.data_ptr(), G.data_ptr(),
+                gt.data_ptr(), gv.data_ptr(), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                ctypes.c_void_p(stream))
+        _capi.raise_for(st, "lbl_transmittance_vjp")
+        return gt[: self.n_layers], gv[: len(self.gases), : self.n_layers]
 
     # ------------------------------------------------------------ work count --
     def tile_edges(self):
