@@ -93,6 +93,50 @@ def cpu_baseline(sd, lay, prob, tile, layer_lo, layer_hi, gpu_layers):
     return res
 
 
+def accuracy_report(sd, lay, prob, gpu_layers, dev, torch):
+    """Mixed vs FP64 (GPU; FP64 agrees with the CPU oracle to ~1e-10) over the
+    whole grid: relative tau error per layer and per gas, and max |dT| on a
+    translucent variant of the workload (line strengths x 1e-6) where T is
+    not saturated at 0."""
+    from pylblrtm_amd import synth
+    from pylblrtm_amd.device import OpticalDepthProblem
+    from pylblrtm_amd.spectral_data import SpectralData
+    rep = {}
+    od64, odm = gpu_layers["fp64"], gpu_layers["mixed"]
+    pos = od64 > 0
+    rel = np.where(pos, np.abs(odm - od64) / np.where(pos, od64, 1.0), 0.0)
+    rep["max_rel_tau_per_layer"] = [float(v) for v in rel.max(axis=1)]
+    t64, tm = od64.sum(0), odm.sum(0)
+    rep["max_rel_tau_column"] = float(np.max(np.abs(tm - t64) / t64))
+    per_gas = {}
+    mol = np.asarray(sd.lines["mol_id"])
+    for name, mid, _mass in synth.FIVE_GASES:
+        m = mol == mid
+        sdg = SpectralData(lines={k: np.asarray(v)[m] for k, v in sd.lines.items()},
+                           tips=sd.tips, molar_masses=sd.molar_masses)
+        pg = OpticalDepthProblem(sdg, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, device=dev)
+        a = pg.run("fp64", mus=(1.0,), want_tau=True)["optical_depth"]
+        b = pg.run("mixed", mus=(1.0,), want_tau=True)["optical_depth"].double()
+        per_gas[name] = float(((b - a).abs() / a).max())
+        del pg, a, b
+    rep["max_rel_tau_per_gas"] = per_gas
+    lines = dict(sd.lines)
+    lines["Sref"] = np.asarray(sd.lines["Sref"]) * 1e-6
+    sdt = SpectralData(lines=lines, tips=sd.tips, molar_masses=sd.molar_masses)
+    pt = OpticalDepthProblem(sdt, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, device=dev)
+    Ta = pt.run("fp64", mus=(1.0, 0.5))["transmittance"]
+    Tb = pt.run("mixed", mus=(1.0, 0.5))["transmittance"].double()
+    rep["translucent_sref_x1e-6"] = {
+        "max_abs_dT": {"1.0": float((Tb[0] - Ta[0]).abs().max()),
+                       "0.5": float((Tb[1] - Ta[1]).abs().max())},
+        "fraction_of_points_0.01<T<0.99": float(((Ta[0] > 0.01) & (Ta[0] < 0.99)).double().mean()),
+        "median_T": float(Ta[0].median()),
+    }
+    log(f"[bench] accuracy: column rel tau {rep['max_rel_tau_column']:.2e}, per gas {per_gas}, "
+        f"translucent {rep['translucent_sref_x1e-6']}")
+    return rep
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +233,8 @@ def main() -> int:
             o = prob.run(prec, mus=(1.0,), want_layers=True)
             gpu_layers[prec] = o["optical_depth_layers"].double().cpu().numpy()
             del o
+    if not args.no_parity and "mixed" in gpu_layers:
+        result["accuracy"] = accuracy_report(sd, lay, prob, gpu_layers, dev, torch)
     if not args.no_cpu_baseline:
         tile, lo, hi = 500, 20, 30
         log(f"[bench] cpu oracle sample: tile {tile}, layers {lo}-{hi - 1} ...")
