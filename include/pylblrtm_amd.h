@@ -146,6 +146,34 @@ int lbl_transmittance_vjp(const lbl_lines_t *lines, const lbl_groups_t *groups,
                           const float *grad_transmittance, double *grad_t, double *grad_vmr,
                           void *workspace, size_t workspace_bytes, void *stream);
 
+/* Learned Voigt surrogate (no counterpart in the reference; its Voigt is
+ * scipy.special.wofz at forward.py:62).  MLP 2-64-64-64-64-1 (SiLU) on
+ * u = x/50, v = (log10 y + 1)/3 predicting t = log(K / K_ref),
+ * K_ref = exp(-x^2)/(1 + sqrt(pi) y) + y/(sqrt(pi)(x^2 + y^2 + 1/2)).
+ * Parameters: FP32 vector of lbl_mlp_num_params() entries (layout in
+ * pylblrtm_amd/csrc/surrogate.h); wbf16 = BF16 copies of the three hidden
+ * 64x64 weight matrices (3*64*64 entries) used by the MFMA kernels. */
+int lbl_mlp_num_params(void);
+/* n seeded samples x ~ U[-50,50], log10 y ~ U[-4,2], target t from the FP64
+ * device Faddeeva function. */
+int lbl_mlp_make_data(int64_t n, uint64_t seed, float *x, float *log10y, float *target,
+                      void *stream);
+int lbl_mlp_pack(const float *params, unsigned short *wbf16, void *stream);
+int lbl_mlp_train_workspace_bytes(int batch, size_t *bytes);
+/* One step on samples idx(j) = (perm_a*j + perm_b) mod n_data, j = base..base+batch-1
+ * (n_data a power of two, perm_a odd): gradient of mean (out - t)^2 into
+ * grad (device, num_params floats), loss sum into loss (device float), then,
+ * if lr > 0, Adam (beta 0.9/0.999, eps 1e-8, step adam_t >= 1) on params and
+ * a fresh wbf16.  Asynchronous on `stream`. */
+int lbl_mlp_train_step(float *params, float *adam_m, float *adam_v, unsigned short *wbf16,
+                       const float *x, const float *log10y, const float *target, int64_t n_data,
+                       uint64_t perm_a, uint64_t perm_b, int64_t base, int batch, float lr,
+                       int adam_t, float *grad, float *loss, void *workspace,
+                       size_t workspace_bytes, void *stream);
+/* Network output t for n points (BF16 MFMA inference). */
+int lbl_mlp_eval(int64_t n, const float *x, const float *log10y, const float *params,
+                 const unsigned short *wbf16, float *t_out, void *stream);
+
 /* Kernel timing (no counterpart in the reference; for benchmarks).  When
  * enabled, lbl_optical_depth records HIP events on its stream around the
  * per-(layer, line) kernel and the line-sum kernel; lbl_last_timings returns

@@ -19,7 +19,8 @@ PRECISIONS = {"fp64": 0, "mixed": 1, "surrogate": 2}
 
 EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error",
             "lbl_enable_timing", "lbl_last_timings", "lbl_vjp_workspace_bytes",
-            "lbl_transmittance_vjp")
+            "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
+            "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval")
 
 _vp = ctypes.c_void_p
 
@@ -84,6 +85,21 @@ def load() -> ctypes.CDLL:
                                           ctypes.c_int32, P(ctypes.c_double), _vp, _vp, _vp, _vp,
                                           _vp, ctypes.c_size_t, _vp]
     lib.lbl_transmittance_vjp.restype = ctypes.c_int
+    lib.lbl_mlp_num_params.argtypes = []
+    lib.lbl_mlp_num_params.restype = ctypes.c_int
+    lib.lbl_mlp_make_data.argtypes = [ctypes.c_int64, ctypes.c_uint64, _vp, _vp, _vp, _vp]
+    lib.lbl_mlp_make_data.restype = ctypes.c_int
+    lib.lbl_mlp_pack.argtypes = [_vp, _vp, _vp]
+    lib.lbl_mlp_pack.restype = ctypes.c_int
+    lib.lbl_mlp_train_workspace_bytes.argtypes = [ctypes.c_int, P(ctypes.c_size_t)]
+    lib.lbl_mlp_train_workspace_bytes.restype = ctypes.c_int
+    lib.lbl_mlp_train_step.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
+                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int64,
+                                       ctypes.c_int, ctypes.c_float, ctypes.c_int, _vp, _vp, _vp,
+                                       ctypes.c_size_t, _vp]
+    lib.lbl_mlp_train_step.restype = ctypes.c_int
+    lib.lbl_mlp_eval.argtypes = [ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]
+    lib.lbl_mlp_eval.restype = ctypes.c_int
     lib.lbl_enable_timing.argtypes = [ctypes.c_int]
     lib.lbl_enable_timing.restype = None
     lib.lbl_last_timings.argtypes = [P(ctypes.c_double), P(ctypes.c_double)]

@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpylblrtm_amd.so"
-SOURCES = ("line_params.hip", "voigt_sum.hip", "adjoint.hip", "capi.hip")
+SOURCES = ("line_params.hip", "voigt_sum.hip", "adjoint.hip", "surrogate.hip", "capi.hip")
 ARCH = os.environ.get("PYLBLRTM_AMD_ARCH", "gfx950")
 
 

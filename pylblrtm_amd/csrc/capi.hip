@@ -308,6 +308,74 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   return LBL_OK;
 }
 
+int lbl_mlp_num_params(void) { return lbl::mlp_num_params(); }
+
+int lbl_mlp_make_data(int64_t n, uint64_t seed, float* x, float* log10y, float* target,
+                      void* stream) {
+  g_last_error[0] = '\0';
+  if (n < 0 || (n > 0 && (!x || !log10y || !target)))
+    return fail(LBL_EINVAL, "invalid arguments to lbl_mlp_make_data");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(lbl::launch_mlp_make_data(n, seed, x, log10y, target, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LBL_OK;
+}
+
+int lbl_mlp_pack(const float* params, unsigned short* wbf16, void* stream) {
+  g_last_error[0] = '\0';
+  if (!params || !wbf16) return fail(LBL_EINVAL, "invalid arguments to lbl_mlp_pack");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(lbl::launch_mlp_pack(params, wbf16, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LBL_OK;
+}
+
+int lbl_mlp_train_workspace_bytes(int batch, size_t* bytes) {
+  if (!bytes || batch <= 0 || batch % 64) return fail(LBL_EINVAL, "batch must be a positive multiple of 64");
+  const size_t rows = (size_t)batch / 64;
+  *bytes = align_up(rows * lbl::mlp_num_params() * sizeof(float)) + align_up(rows * sizeof(float));
+  return LBL_OK;
+}
+
+int lbl_mlp_train_step(float* params, float* adam_m, float* adam_v, unsigned short* wbf16,
+                       const float* x, const float* log10y, const float* target, int64_t n_data,
+                       uint64_t perm_a, uint64_t perm_b, int64_t base, int batch, float lr,
+                       int adam_t, float* grad, float* loss, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  g_last_error[0] = '\0';
+  size_t need = 0;
+  int rc = lbl_mlp_train_workspace_bytes(batch, &need);
+  if (rc != LBL_OK) return rc;
+  if (!params || !wbf16 || !x || !log10y || !target || !grad || !loss || !workspace ||
+      (lr > 0.0f && (!adam_m || !adam_v || adam_t < 1)))
+    return fail(LBL_EINVAL, "invalid arguments to lbl_mlp_train_step");
+  if (n_data <= 0 || (n_data & (n_data - 1)) || (perm_a & 1u) == 0)
+    return fail(LBL_EINVAL, "n_data must be a power of two and perm_a odd");
+  if (workspace_bytes < need)
+    return fail(LBL_EINVAL, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+  char* ws = static_cast<char*>(workspace);
+  const size_t rows = (size_t)batch / 64;
+  float* partial = reinterpret_cast<float*>(ws);
+  float* loss_partial =
+      reinterpret_cast<float*>(ws + align_up(rows * lbl::mlp_num_params() * sizeof(float)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(lbl::launch_mlp_step(params, adam_m, adam_v, wbf16, x, log10y, target, n_data, perm_a,
+                               perm_b, base, batch, lr, adam_t, partial, loss_partial, grad, loss,
+                               s));
+  return LBL_OK;
+}
+
+int lbl_mlp_eval(int64_t n, const float* x, const float* log10y, const float* params,
+                 const unsigned short* wbf16, float* t_out, void* stream) {
+  g_last_error[0] = '\0';
+  if (n < 0 || (n > 0 && (!x || !log10y || !params || !wbf16 || !t_out)))
+    return fail(LBL_EINVAL, "invalid arguments to lbl_mlp_eval");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(lbl::launch_mlp_eval(n, x, log10y, params, wbf16, t_out, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return LBL_OK;
+}
+
 int lbl_voigt_eval(int64_t n, const void* x, const void* y, void* k_out, int precision,
                    void* stream) {
   g_last_error[0] = '\0';

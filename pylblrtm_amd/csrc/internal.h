@@ -93,6 +93,18 @@ hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, con
                       double* grad_vmr, hipStream_t s);
 int vjp_points_per_thread();
 
+int mlp_num_params();
+hipError_t launch_mlp_make_data(int64_t n, uint64_t seed, float* x, float* ly, float* t,
+                                hipStream_t s);
+hipError_t launch_mlp_pack(const float* p, unsigned short* wbf, hipStream_t s);
+hipError_t launch_mlp_step(float* p, float* m, float* v, unsigned short* wbf, const float* x,
+                           const float* ly, const float* t, int64_t n_data, uint64_t perm_a,
+                           uint64_t perm_b, int64_t base, int batch, float lr, int adam_t,
+                           float* partial, float* loss_partial, float* grad, float* loss,
+                           hipStream_t s);
+hipError_t launch_mlp_eval(int64_t n, const float* x, const float* ly, const float* p,
+                           const unsigned short* wbf, float* t, hipStream_t s);
+
 // Points per thread of the line-sum kernels (a workgroup covers kBlock * P
 // consecutive grid points of one reference tile).
 int sum_points_per_thread(int precision);
