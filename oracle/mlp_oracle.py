@@ -47,7 +47,7 @@ def dsilu(z):
 
 
 def inputs(x, ly):
-    u = (np.asarray(x, np.float32) * np.float32(1.0 / 50.0)).astype(np.float64)
+    u = 2.0 * np.arcsinh(np.abs(np.asarray(x, np.float32).astype(np.float64))) * float(np.float32(0.21714253)) - 1.0
     v = ((np.asarray(ly, np.float32) + np.float32(1.0)) * np.float32(1.0 / 3.0)).astype(np.float64)
     return u, v
 

@@ -1,8 +1,10 @@
 // Learned surrogate of the Voigt function K(x, y) (no counterpart in the
 // reference, whose Voigt is scipy.special.wofz at pylblrtm/forward.py:62).
 //
-// Network: MLP 2 -> 64 -> 64 -> 64 -> 64 -> 1 with SiLU, inputs
-//   u = x / 50, v = (log10 y + 1) / 3   (domain |x| <= 50, -4 <= log10 y <= 2),
+// Network: MLP 2 -> 64 -> 64 -> 64 -> 64 -> 1 with SiLU on the inputs
+//   u = 2 asinh(|x|) / asinh(50) - 1,  v = (log10 y + 1) / 3
+// (domain |x| <= 50, -4 <= log10 y <= 2).  K is even in x, and asinh spreads
+// the Doppler core (|x| < ~3, 3% of a linear scale) over half of u.
 // output t ~ log(K / K_ref) with the cheap positive reference
 //   K_ref(x, y) = exp(-x^2) / (1 + sqrt(pi) y) + y / (sqrt(pi) (x^2 + y^2 + 1/2)),
 // so K = K_ref * exp(t) and |t| stays below ~0.7 over the whole domain.
@@ -33,7 +35,7 @@ constexpr int kNumParams = kB5 + 1;    // 12,737
 constexpr int kHiddenW[3] = {kW2, kW3, kW4};
 constexpr int kHiddenB[3] = {kB2, kB3, kB4};
 
-constexpr float kXScale = 1.0f / 50.0f;
+constexpr float kInvAsinh50 = 0.21714253f;   // 1 / asinh(50)
 constexpr float kXMax = 50.0f;
 constexpr float kLyMin = -4.0f, kLyMax = 2.0f;
 
@@ -47,6 +49,11 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
   return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float input_u(float x) {
+  return 2.0f * asinhf(fabsf(x)) * kInvAsinh50 - 1.0f;
+}
+__device__ __forceinline__ float input_v(float log10y) { return (log10y + 1.0f) * (1.0f / 3.0f); }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
 

@@ -79,8 +79,8 @@ __global__ __launch_bounds__(64) void k_fwd_bwd(const float* __restrict__ p,
   // sample index: an affine bijection of the data set (perm_a odd, n_data a power of two)
   const uint64_t j = (uint64_t)(base + (int64_t)blockIdx.x * 64 + l);
   const int64_t idx = (int64_t)((perm_a * j + perm_b) & (uint64_t)(n_data - 1));
-  const float u = dx[idx] * kXScale;
-  const float v = (dly[idx] + 1.0f) * (1.0f / 3.0f);
+  const float u = input_u(dx[idx]);
+  const float v = input_v(dly[idx]);
   const float tgt = dt[idx];
   S.u[l] = u;
   S.v[l] = v;
@@ -250,8 +250,8 @@ __global__ __launch_bounds__(64) void k_eval(int64_t n, const float* __restrict_
                                              float* __restrict__ t) {
   const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
   const bool ok = i < n;
-  const float u = ok ? x[i] * kXScale : 0.0f;
-  const float v = ok ? (ly[i] + 1.0f) * (1.0f / 3.0f) : 0.0f;
+  const float u = ok ? input_u(x[i]) : 0.0f;
+  const float v = ok ? input_v(ly[i]) : 0.0f;
   const float r = wave_eval(u, v, p, wbf);   // all 64 lanes active
   if (ok) t[i] = r;
 }

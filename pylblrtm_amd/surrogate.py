@@ -1,6 +1,7 @@
 """Learned surrogate of the Voigt function K(x, y) = Re w(x + iy).
 
-MLP 2 -> 64 -> 64 -> 64 -> 64 -> 1 (SiLU) on u = x/50, v = (log10 y + 1)/3,
+MLP 2 -> 64 -> 64 -> 64 -> 64 -> 1 (SiLU) on u = 2 asinh(|x|)/asinh(50) - 1,
+v = (log10 y + 1)/3,
 trained on the GPU by the library's own BF16-MFMA forward/backward kernels
 and Adam (``lbl_mlp_train_step``) on seeded samples labelled by the FP64
 device Faddeeva function.  The network predicts t = log(K / K_ref) with
