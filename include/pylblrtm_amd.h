@@ -122,6 +122,26 @@ int lbl_optical_depth(const lbl_lines_t *lines, const lbl_groups_t *groups,
                       const lbl_outputs_t *out, void *workspace, size_t workspace_bytes,
                       int32_t *err_layer, int32_t *err_group, void *stream);
 
+/* Network of the learned Voigt surrogate (see lbl_mlp_* below) and its gate:
+ * a near-centre point uses the network iff |x| <= 50, -4 <= log10 y <= 2 and
+ * A * nu_c * K_ref(x, y) <= gate (a bound on its optical-depth contribution
+ * divided by the network's relative error budget); all other points keep the
+ * exact FP32 Faddeeva dispatch. */
+typedef struct lbl_surrogate {
+  const float *params;           /* device, lbl_mlp_num_params() floats */
+  const unsigned short *wbf16;   /* device, 3*64*64 BF16 hidden weights */
+  double gate;
+} lbl_surrogate_t;
+
+/* lbl_optical_depth with precision LBL_SURROGATE (no counterpart in the
+ * reference): the mixed line sum with the network fused into the near-centre
+ * evaluations.  Outputs are float. */
+int lbl_optical_depth_surrogate(const lbl_lines_t *lines, const lbl_groups_t *groups,
+                                const lbl_layers_t *layers, const lbl_grid_t *grid,
+                                const lbl_outputs_t *out, const lbl_surrogate_t *sur,
+                                void *workspace, size_t workspace_bytes, int32_t *err_layer,
+                                int32_t *err_group, void *stream);
+
 /* Evaluate the device Voigt function K(x, y) = Re w(x + iy) on n device
  * points with the region dispatch the line sum uses (precision LBL_FP64:
  * double in/out; LBL_MIXED: float in/out).  Stands in for

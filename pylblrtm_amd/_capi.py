@@ -20,7 +20,8 @@ PRECISIONS = {"fp64": 0, "mixed": 1, "surrogate": 2}
 EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error",
             "lbl_enable_timing", "lbl_last_timings", "lbl_vjp_workspace_bytes",
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
-            "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval")
+            "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
+            "lbl_optical_depth_surrogate")
 
 _vp = ctypes.c_void_p
 
@@ -44,6 +45,10 @@ class LblGrid(ctypes.Structure):
     _fields_ = [("nu_min", ctypes.c_double), ("step", ctypes.c_double), ("n_nu", ctypes.c_int64),
                 ("tile_samples", ctypes.c_int64), ("cutoff", ctypes.c_double),
                 ("shift_bound", ctypes.c_double)]
+
+
+class LblSurrogate(ctypes.Structure):
+    _fields_ = [("params", _vp), ("wbf16", _vp), ("gate", ctypes.c_double)]
 
 
 class LblOutputs(ctypes.Structure):
@@ -76,6 +81,11 @@ def load() -> ctypes.CDLL:
                                       ctypes.c_int, P(LblOutputs), _vp, ctypes.c_size_t,
                                       P(ctypes.c_int32), P(ctypes.c_int32), _vp]
     lib.lbl_optical_depth.restype = ctypes.c_int
+    lib.lbl_optical_depth_surrogate.argtypes = [P(LblLines), P(LblGroups), P(LblLayers),
+                                                P(LblGrid), P(LblOutputs), P(LblSurrogate), _vp,
+                                                ctypes.c_size_t, P(ctypes.c_int32),
+                                                P(ctypes.c_int32), _vp]
+    lib.lbl_optical_depth_surrogate.restype = ctypes.c_int
     lib.lbl_voigt_eval.argtypes = [ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int, _vp]
     lib.lbl_voigt_eval.restype = ctypes.c_int
     lib.lbl_vjp_workspace_bytes.argtypes = [P(LblLines), P(LblLayers), P(LblGrid),

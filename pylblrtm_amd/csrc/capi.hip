@@ -89,8 +89,6 @@ int check_inputs(const lbl_lines_t* lines, const lbl_groups_t* groups, const lbl
   if (!lines || !groups || !layers || !grid) return fail(LBL_EINVAL, "null argument");
   if (precision != LBL_FP64 && precision != LBL_MIXED && precision != LBL_SURROGATE)
     return fail(LBL_EINVAL, "unknown precision %d", precision);
-  if (precision == LBL_SURROGATE)
-    return fail(LBL_EINVAL, "precision LBL_SURROGATE needs lbl_optical_depth_surrogate");
   if (lines->n_lines < 0 || lines->n_lines > INT32_MAX)
     return fail(LBL_EINVAL, "n_lines %lld out of range", (long long)lines->n_lines);
   if (layers->n_layers < 0 || layers->n_layers > (1 << 19))
@@ -139,13 +137,16 @@ int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
   return LBL_OK;
 }
 
-int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
-                      const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
-                      const lbl_outputs_t* out, void* workspace, size_t workspace_bytes,
-                      int32_t* err_layer, int32_t* err_group, void* stream) {
+static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                              const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
+                              const lbl_outputs_t* out, const lbl_surrogate_t* sur,
+                              void* workspace, size_t workspace_bytes, int32_t* err_layer,
+                              int32_t* err_group, void* stream) {
   g_last_error[0] = '\0';
   int rc = check_inputs(lines, groups, layers, grid, precision);
   if (rc != LBL_OK) return rc;
+  if (precision == LBL_SURROGATE && (!sur || !sur->params || !sur->wbf16 || !(sur->gate >= 0.0)))
+    return fail(LBL_EINVAL, "precision LBL_SURROGATE needs network parameters and a gate >= 0");
   if (!out || out->n_mu < 1 || out->n_mu > lbl::kMaxMu || !out->mu || !out->transmittance)
     return fail(LBL_EINVAL, "outputs: need 1..%d viewing cosines and a transmittance buffer",
                 lbl::kMaxMu);
@@ -214,6 +215,11 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
   sa.tau = out->optical_depth;
   sa.tau_layers = out->optical_depth_layers;
   sa.err = err;
+  if (precision == LBL_SURROGATE) {
+    sa.mlp_params = sur->params;
+    sa.mlp_wbf = sur->wbf16;
+    sa.mlp_gate = (float)sur->gate;
+  }
   if (ev) HIP_TRY(hipEventRecord(ev[2], s));
   HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[3], s));
@@ -239,6 +245,25 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
                 layer, group);
   }
   return LBL_OK;
+}
+
+int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                      const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
+                      const lbl_outputs_t* out, void* workspace, size_t workspace_bytes,
+                      int32_t* err_layer, int32_t* err_group, void* stream) {
+  if (precision == LBL_SURROGATE)
+    return fail(LBL_EINVAL, "precision LBL_SURROGATE needs lbl_optical_depth_surrogate");
+  return optical_depth_impl(lines, groups, layers, grid, precision, out, nullptr, workspace,
+                            workspace_bytes, err_layer, err_group, stream);
+}
+
+int lbl_optical_depth_surrogate(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                                const lbl_layers_t* layers, const lbl_grid_t* grid,
+                                const lbl_outputs_t* out, const lbl_surrogate_t* sur,
+                                void* workspace, size_t workspace_bytes, int32_t* err_layer,
+                                int32_t* err_group, void* stream) {
+  return optical_depth_impl(lines, groups, layers, grid, LBL_SURROGATE, out, sur, workspace,
+                            workspace_bytes, err_layer, err_group, stream);
 }
 
 int lbl_vjp_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,

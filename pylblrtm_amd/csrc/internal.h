@@ -72,6 +72,10 @@ struct SumArgs {
   void* tau;
   void* tau_layers;
   uint32_t* err;
+  // surrogate mode only
+  const float* mlp_params;
+  const unsigned short* mlp_wbf;
+  float mlp_gate;   // a point uses the network iff A * c * K_ref(x, y) <= mlp_gate
 };
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);

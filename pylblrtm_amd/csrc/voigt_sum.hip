@@ -31,6 +31,7 @@
 
 #include "faddeeva.h"
 #include "internal.h"
+#include "surrogate.h"
 
 namespace lbl {
 namespace {
@@ -186,9 +187,19 @@ static_assert(sizeof(MixedRec) == 48, "record layout");
 // Four waves per SIMD (<= 128 VGPRs) lets every tile of a 2 cm^-1 x 0.001
 // grid (1001 workgroups) be resident at once on 256 CUs; at three waves the
 // last quarter of the tiles ran as a second, one-wave-per-SIMD round.
-template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
-void k_voigt_sum_mixed(const SumArgs a) {
+// Surrogate mode: a queued point goes to the network iff it lies in the
+// trained domain and its optical-depth contribution bound A c K_ref(x, y)
+// (R(nu, T) <= nu ~ c near the centre, K <= 1.6 K_ref over the domain) is
+// below the gate; everything else keeps the exact FP32 dispatch.
+__device__ __forceinline__ bool surrogate_point(float x, float y, float ac, float gate) {
+  if (!(fabsf(x) <= mlp::kXMax)) return false;
+  const float ly = __log10f(y);
+  if (!(ly >= mlp::kLyMin && ly <= mlp::kLyMax)) return false;
+  return ac * mlp::kref(x, y) <= gate;
+}
+
+template <int P, bool kSurrogate>
+__device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ MixedRec s_rec[kChunk];
   // Near-centre work queue: (record, thread) pairs whose P points are not all
   // in the two-node region are appended here instead of being evaluated by
@@ -309,8 +320,39 @@ void k_voigt_sum_mixed(const SumArgs a) {
           const MixedRec& R = s_rec[q];
           const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P);
           const float x0 = (float)(nu_t - R.c) * R.s;
-          for (int j = 0; j < P; ++j)
-            atomicAdd(&s_acc[t * P + j], R.a * K_mixed(fmaf((float)j, R.hs, x0), R.y));
+          const float ac = R.a * (float)R.c;
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, R.hs, x0);
+            if (kSurrogate && surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
+            atomicAdd(&s_acc[t * P + j], R.a * K_mixed(x, R.y));
+          }
+        }
+        if constexpr (kSurrogate) {
+          // network pass: each wavefront takes 8 queue entries = 64 points
+          // per round (lane = entry (l / 8), point (l % 8)); all lanes stay
+          // active through the MFMA evaluation.
+          const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+          for (int g = wave; g * 8 < nq; g += kBlock / 64) {
+            const int e = g * 8 + (lane >> 3), j = lane & 7;
+            bool use = false;
+            float x = 0.0f, y = 1.0f, am = 0.0f;
+            int t = 0;
+            if (e < nq) {
+              const uint32_t ent = s_q[e];
+              const int q = (int)(ent >> 8);
+              t = (int)(ent & 0xFF);
+              const MixedRec& R = s_rec[q];
+              const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P);
+              x = fmaf((float)j, R.hs, (float)(nu_t - R.c) * R.s);
+              y = R.y;
+              am = R.a;
+              use = surrogate_point(x, y, am * (float)R.c, a.mlp_gate);
+            }
+            const float u = use ? mlp::input_u(x) : 0.0f;
+            const float v = use ? mlp::input_v(__log10f(y)) : 0.0f;
+            const float tt = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
+            if (use) atomicAdd(&s_acc[t * P + j], am * mlp::kref(x, y) * __expf(tt));
+          }
         }
         __syncthreads();
 #pragma unroll
@@ -355,6 +397,19 @@ void k_voigt_sum_mixed(const SumArgs a) {
   }
 }
 
+template <int P>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_voigt_sum_mixed(const SumArgs a) {
+  voigt_sum_mixed_body<P, false>(a);
+}
+
+// The network's MFMA tiles need ~96 more registers; two waves per SIMD.
+template <int P>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+void k_voigt_sum_surrogate(const SumArgs a) {
+  voigt_sum_mixed_body<P, true>(a);
+}
+
 // --------------------------------------------------------- test entry --
 
 __global__ void k_voigt_eval64(int64_t n, const double* x, const double* y, double* k) {
@@ -380,6 +435,8 @@ hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0) return hipSuccess;
   if (precision == LBL_FP64)
     k_voigt_sum_fp64<kPointsFp64><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+  else if (precision == LBL_SURROGATE)
+    k_voigt_sum_surrogate<kPointsMixed><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
   else
     k_voigt_sum_mixed<kPointsMixed><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
   return hipGetLastError();

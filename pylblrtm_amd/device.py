@@ -302,13 +302,18 @@ class OpticalDepthProblem:
         return out
 
     def run(self, precision: str = "fp64", mus: Sequence[float] = (1.0,), want_tau: bool = False,
-            want_layers: bool = False, outputs: Optional[dict] = None) -> dict:
-        """Run the device path; returns device tensors (transmittance [n_mu, n_nu], ...)."""
+            want_layers: bool = False, outputs: Optional[dict] = None, surrogate=None,
+            gate: Optional[float] = None) -> dict:
+        """Run the device path; returns device tensors (transmittance [n_mu, n_nu], ...).
+
+        precision "surrogate" uses ``surrogate`` (a VoigtSurrogate; default:
+        the committed network) for near-centre points that pass ``gate``
+        (default SURROGATE_GATE)."""
         torch = _torch()
         if precision not in _capi.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_capi.PRECISIONS)}; got {precision!r}")
         code = _capi.PRECISIONS[precision]
-        ws = self.workspace(code)
+        ws = self.workspace(_capi.PRECISIONS["mixed"] if precision == "surrogate" else code)
         out = outputs or self.alloc_outputs(precision, len(mus), want_tau, want_layers)
         mu_arr = (ctypes.c_double * len(mus))(*[float(m) for m in mus])
         ptr = lambda k: out[k].data_ptr() if k in out else None
@@ -318,10 +323,22 @@ class OpticalDepthProblem:
         err_group = ctypes.c_int32(-1)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         with torch.cuda.device(self.device):
-            st = self.lib.lbl_optical_depth(
-                ctypes.byref(self.c_lines), ctypes.byref(self.c_groups), ctypes.byref(self.c_layers),
-                ctypes.byref(self.c_grid), code, ctypes.byref(c_out), ctypes.c_void_p(ws.data_ptr()),
-                ws.numel(), ctypes.byref(err_layer), ctypes.byref(err_group), ctypes.c_void_p(stream))
+            if precision == "surrogate":
+                from .surrogate import SURROGATE_GATE, default_surrogate
+                net = surrogate if surrogate is not None else default_surrogate(self.device)
+                c_sur = _capi.LblSurrogate(net.params.data_ptr(), net.wbf.data_ptr(),
+                                           float(SURROGATE_GATE if gate is None else gate))
+                st = self.lib.lbl_optical_depth_surrogate(
+                    ctypes.byref(self.c_lines), ctypes.byref(self.c_groups),
+                    ctypes.byref(self.c_layers), ctypes.byref(self.c_grid), ctypes.byref(c_out),
+                    ctypes.byref(c_sur), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                    ctypes.byref(err_layer), ctypes.byref(err_group), ctypes.c_void_p(stream))
+            else:
+                st = self.lib.lbl_optical_depth(
+                    ctypes.byref(self.c_lines), ctypes.byref(self.c_groups),
+                    ctypes.byref(self.c_layers), ctypes.byref(self.c_grid), code,
+                    ctypes.byref(c_out), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                    ctypes.byref(err_layer), ctypes.byref(err_group), ctypes.c_void_p(stream))
         if st == _capi.LBL_ESTRENGTH:
             gas = self.groups[err_group.value].gas
             raise FloatingPointError(

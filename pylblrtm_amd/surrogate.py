@@ -26,6 +26,24 @@ import numpy as np
 from . import _capi
 
 DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "data" / "voigt_surrogate.npz"
+
+# Gate of the fused surrogate: a near-centre point uses the network iff
+# A * nu_c * K_ref(x, y) <= SURROGATE_GATE, i.e. its optical-depth error is at
+# most TAU_BUDGET when the network's relative error is <= REL_ERR_BOUND and
+# K <= 1.6 K_ref (exp(0.45), the largest log-ratio over the domain).
+TAU_BUDGET = 1e-9
+REL_ERR_BOUND = 0.05
+SURROGATE_GATE = TAU_BUDGET / (1.6 * REL_ERR_BOUND)
+
+_default = {}
+
+
+def default_surrogate(device=None):
+    """The committed network, loaded once per device."""
+    key = str(device)
+    if key not in _default:
+        _default[key] = VoigtSurrogate.load(DEFAULT_WEIGHTS, device=device)
+    return _default[key]
 X_MAX = 50.0
 LY_MIN, LY_MAX = -4.0, 2.0
 

@@ -89,3 +89,92 @@ def test_committed_network_accuracy():
     say(f"surrogate rel err: median {np.median(rel):.2e}, p99 {np.quantile(rel, 0.99):.2e}, "
         f"max {rel.max():.2e}")
     assert np.quantile(rel, 0.99) < 2e-2 and rel.max() < 1e-1
+
+
+def _five_gas_small():
+    from pylblrtm_amd import synth
+    from pylblrtm_amd.device import OpticalDepthProblem
+    sd, prof = synth.five_gas(lines_per_gas=2000)
+    lines = dict(sd.lines)
+    lines["Sref"] = np.asarray(sd.lines["Sref"]) * 1e-6      # translucent: T not saturated
+    from pylblrtm_amd import SpectralData
+    sdt = SpectralData(lines=lines, tips=sd.tips, molar_masses=sd.molar_masses)
+    return OpticalDepthProblem(sdt, prof.build_layers(), 900.0, 1100.0, 0.001, 25.0, 2.0)
+
+
+def test_fused_surrogate_meets_dT_tolerance_when_gated():
+    say("case: fused surrogate, gated, five gas 900-1100")
+    if not S.DEFAULT_WEIGHTS.exists():
+        pytest.skip("no trained weights committed")
+    prob = _five_gas_small()
+    ref = prob.run("fp64", mus=(1.0, 0.5), want_tau=True)
+    sur = prob.run("surrogate", mus=(1.0, 0.5), want_tau=True)
+    mix = prob.run("mixed", mus=(1.0, 0.5), want_tau=True)
+    for m in range(2):
+        dT = (sur["transmittance"][m].double() - ref["transmittance"][m]).abs().max().item()
+        assert dT <= 1e-5, dT
+    dtau = (sur["optical_depth"].double() - ref["optical_depth"]).abs()
+    rel = (dtau / ref["optical_depth"]).max().item()
+    relm = ((mix["optical_depth"].double() - ref["optical_depth"]).abs() / ref["optical_depth"]).max().item()
+    say(f"gated surrogate: max |dtau| {dtau.max().item():.2e}, max rel tau {rel:.2e} "
+        f"(mixed {relm:.2e})")
+    # the gate bounds the absolute optical-depth error (|dT| <= |dtau|)
+    assert dtau.max().item() <= 1e-5
+
+
+def test_fused_surrogate_ungated_error_is_bounded_by_network_error():
+    say("case: fused surrogate, every in-domain point on the network")
+    if not S.DEFAULT_WEIGHTS.exists():
+        pytest.skip("no trained weights committed")
+    prob = _five_gas_small()
+    ref = prob.run("fp64", mus=(1.0,), want_tau=True)
+    sur = prob.run("surrogate", mus=(1.0,), want_tau=True, gate=float("inf"))
+    rel = ((sur["optical_depth"].double() - ref["optical_depth"]).abs() / ref["optical_depth"]).max().item()
+    dT = (sur["transmittance"][0].double() - ref["transmittance"][0]).abs().max().item()
+    say(f"ungated surrogate: max rel tau {rel:.2e}, max |dT| {dT:.2e}")
+    assert np.isfinite(rel) and rel < S.REL_ERR_BOUND
+
+
+def test_in_domain_network_vs_exact_dispatch():
+    """What the network gains over the exact FP32 Faddeeva dispatch on the
+    same in-domain points: accuracy and throughput (printed)."""
+    say("case: in-domain network vs exact dispatch")
+    import time
+    import torch
+    from pylblrtm_amd.device import voigt_eval
+    if not S.DEFAULT_WEIGHTS.exists():
+        pytest.skip("no trained weights committed")
+    s = S.VoigtSurrogate.load()
+    n = 1 << 22
+    rng = np.random.default_rng(31)
+    x = rng.uniform(-50, 50, n).astype(np.float32)
+    ly = rng.uniform(-4, 2, n).astype(np.float32)
+    xt = torch.as_tensor(x, device=s.device)
+    lyt = torch.as_tensor(ly, device=s.device)
+    yt = torch.pow(10.0, lyt)
+    from pylblrtm_amd import _capi
+    lib = _capi.load()
+    kt = torch.empty_like(xt)
+    stream = torch.cuda.current_stream(s.device).cuda_stream
+    def exact():
+        lib.lbl_voigt_eval(n, xt.data_ptr(), yt.data_ptr(), kt.data_ptr(), 1, stream)
+    def net():
+        s.eval_t(xt, lyt)
+    for f in (exact, net):
+        f()
+    torch.cuda.synchronize()
+    times = {}
+    for name, f in (("exact", exact), ("net", net)):
+        t0 = time.perf_counter()
+        for _ in range(5):
+            f()
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / 5
+    tnet = s.eval_t(xt, lyt).double().cpu().numpy()
+    K = S.k_ref(x.astype(np.float64), 10.0 ** ly.astype(np.float64)) * np.exp(tnet)
+    ref = voigt_eval(x.astype(np.float64), 10.0 ** ly.astype(np.float64), "fp64")
+    rel = np.abs(K - ref) / ref
+    say(f"in-domain: exact FP32 dispatch {n / times['exact']:.3e} evals/s, network "
+        f"{n / times['net']:.3e} evals/s (ratio {times['exact'] / times['net']:.3f}); "
+        f"network rel err p50 {np.median(rel):.2e} p99 {np.quantile(rel, 0.99):.2e} max {rel.max():.2e}")
+    assert rel.max() < S.REL_ERR_BOUND
