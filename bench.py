@@ -114,7 +114,7 @@ def accuracy_report(sd, lay, prob, gpu_layers, dev, torch):
         m = mol == mid
         sdg = SpectralData(lines={k: np.asarray(v)[m] for k, v in sd.lines.items()},
                            tips=sd.tips, molar_masses=sd.molar_masses)
-        pg = OpticalDepthProblem(sdg, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, device=dev)
+        pg = OpticalDepthProblem(sdg, lay, 500.0, 2500.0, prob.step, 25.0, 2.0, device=dev)
         a = pg.run("fp64", mus=(1.0,), want_tau=True)["optical_depth"]
         b = pg.run("mixed", mus=(1.0,), want_tau=True)["optical_depth"].double()
         per_gas[name] = float(((b - a).abs() / a).max())
@@ -123,18 +123,128 @@ def accuracy_report(sd, lay, prob, gpu_layers, dev, torch):
     lines = dict(sd.lines)
     lines["Sref"] = np.asarray(sd.lines["Sref"]) * 1e-6
     sdt = SpectralData(lines=lines, tips=sd.tips, molar_masses=sd.molar_masses)
-    pt = OpticalDepthProblem(sdt, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, device=dev)
+    pt = OpticalDepthProblem(sdt, lay, 500.0, 2500.0, prob.step, 25.0, 2.0, device=dev)
     Ta = pt.run("fp64", mus=(1.0, 0.5))["transmittance"]
     Tb = pt.run("mixed", mus=(1.0, 0.5))["transmittance"].double()
+    Ts = pt.run("surrogate", mus=(1.0, 0.5))["transmittance"].double()
     rep["translucent_sref_x1e-6"] = {
         "max_abs_dT": {"1.0": float((Tb[0] - Ta[0]).abs().max()),
                        "0.5": float((Tb[1] - Ta[1]).abs().max())},
+        "surrogate_max_abs_dT": {"1.0": float((Ts[0] - Ta[0]).abs().max()),
+                                 "0.5": float((Ts[1] - Ta[1]).abs().max())},
         "fraction_of_points_0.01<T<0.99": float(((Ta[0] > 0.01) & (Ta[0] < 0.99)).double().mean()),
         "median_T": float(Ta[0].median()),
     }
     log(f"[bench] accuracy: column rel tau {rep['max_rel_tau_column']:.2e}, per gas {per_gas}, "
         f"translucent {rep['translucent_sref_x1e-6']}")
     return rep
+
+
+def surrogate_report(prob, E, value_plain, mus, steps, capi, torch):
+    """Surrogate mode on the same workload, and the network vs the exact FP32
+    dispatch on identical in-domain (x, y) points."""
+    from pylblrtm_amd import surrogate as S
+    out, sps, k_ms, _ = time_mode(prob, "surrogate", mus, steps, 1, capi, torch)
+    rep = {"evals_per_s": E / sps, "ms_per_step": sps * 1e3, "line_sum_kernel_ms": k_ms,
+           "ratio_vs_plain_fp32": (E / sps) / value_plain, "gate": S.SURROGATE_GATE,
+           "weights": S.DEFAULT_WEIGHTS.name}
+    net = S.default_surrogate(prob.device)
+    rep["training"] = {k: v for k, v in net.meta.items() if k != "history"}
+    n = 1 << 22
+    g = torch.Generator(device=prob.device).manual_seed(31)
+    x = (torch.rand(n, device=prob.device, generator=g) * 100.0 - 50.0).float()
+    ly = (torch.rand(n, device=prob.device, generator=g) * 6.0 - 4.0).float()
+    y = torch.pow(10.0, ly)
+    kt = torch.empty_like(x)
+    lib = capi.load()
+    stream = torch.cuda.current_stream(prob.device).cuda_stream
+
+    def exact():
+        lib.lbl_voigt_eval(n, x.data_ptr(), y.data_ptr(), kt.data_ptr(), 1, stream)
+
+    def network():
+        net.eval_t(x, ly)
+
+    res = {}
+    for name, f in (("exact_fp32_dispatch", exact), ("network_bf16_mfma", network)):
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            f()
+        torch.cuda.synchronize()
+        res[name] = n * 5 / (time.perf_counter() - t0)
+    rep["in_domain_evals_per_s"] = res
+    rep["in_domain_ratio_network_vs_exact"] = res["network_bf16_mfma"] / res["exact_fp32_dispatch"]
+    log(f"[bench] surrogate: {sps * 1e3:.1f} ms/step ({rep['ratio_vs_plain_fp32']:.3f}x plain FP32); "
+        f"in-domain network/exact = {rep['in_domain_ratio_network_vs_exact']:.3f}")
+    return rep
+
+
+def adjoint_bench(args, dev, torch):
+    """Batched gradients: seeded perturbations of the five-gas layers (T_k +
+    N(0, 2 K), vmr x lognormal(0, 0.1)) over 1000-1100 cm^-1 at 1e-4 cm^-1
+    (1,000,001 points); per profile one mixed forward and one adjoint
+    (dT/dT_k, dT/dvmr_{k,g}).  Checked on the first profile against central
+    differences (h = 1e-3 relative) of the FP64 forward for a few parameters."""
+    from pylblrtm_amd import synth
+    from pylblrtm_amd.device import OpticalDepthProblem
+    sd, prof = synth.five_gas()
+    lay = prof.build_layers()
+    prob = OpticalDepthProblem(sd, lay, 1000.0, 1100.0, 1e-4, 25.0, 2.0, device=dev)
+    E = prob.count_evaluations()
+    T_p, vmr_p, _ = synth.perturbations(lay, n_profiles=args.adjoint)
+    V = lambda i: np.stack([vmr_p[g][i] for g in prob.gases])
+    mus = (1.0,)
+    rng = np.random.default_rng(5)
+    G = torch.as_tensor(rng.normal(size=(1, prob.n_nu)), dtype=torch.float32, device=dev)
+    grads = []
+    prob.set_layers(T_p[0], V(0))
+    prob.vjp(prob.run("mixed", mus=mus)["transmittance"], G, mus)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.adjoint):
+        prob.set_layers(T_p[i], V(i))
+        T = prob.run("mixed", mus=mus)["transmittance"]
+        gt, gv = prob.vjp(T, G, mus)
+        grads.append((gt, gv))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.adjoint
+    # finite-difference spot check on profile 0 (layers with most signal)
+    prob.set_layers(T_p[0], V(0))
+    gt0, gv0 = (x.cpu().numpy() for x in grads[0])
+    def loss(tt, vv):
+        prob.set_layers(tt, vv)
+        return float((prob.run("fp64", mus=mus)["transmittance"] * G.double()).sum())
+    checks = []
+    for k in np.argsort(-np.abs(gt0))[:3]:
+        h = 1e-3 * T_p[0][k]
+        tp, tm = T_p[0].copy(), T_p[0].copy()
+        tp[k] += h
+        tm[k] -= h
+        fd = (loss(tp, V(0)) - loss(tm, V(0))) / (2 * h)
+        checks.append({"param": f"T[{int(k)}]", "adjoint": float(gt0[k]), "fd": fd,
+                       "rel": abs(gt0[k] - fd) / max(abs(fd), 1e-300)})
+    g_idx = np.unravel_index(np.argsort(-np.abs(gv0), axis=None)[:3], gv0.shape)
+    for g, k in zip(*g_idx):
+        h = 1e-3 * V(0)[g, k]
+        vp, vm = V(0), V(0)
+        vp[g, k] += h
+        vm[g, k] -= h
+        fd = (loss(T_p[0], vp) - loss(T_p[0], vm)) / (2 * h)
+        checks.append({"param": f"vmr[{prob.gases[g]}][{int(k)}]", "adjoint": float(gv0[g, k]),
+                       "fd": fd, "rel": abs(gv0[g, k] - fd) / max(abs(fd), 1e-300)})
+    res = {"metric": "profiles/s (mixed forward + adjoint, 50 layers x 5 gases)",
+           "value": 1.0 / dt, "unit": "profiles/s", "n_gpus": 1, "steps": args.adjoint,
+           "ms_per_profile": dt * 1e3, "forward_evals_per_profile": E,
+           "evals_per_s_forward_plus_adjoint": 2 * E / dt, "higher_is_better": True,
+           "dtype": "f32", "data": "synthetic (seeded PCG64)",
+           "config": {"workload": f"five-gas lines, {prob.n_lines} in window, 50 layers, 1000-1100 "
+                                  f"cm-1 at 1e-4 ({prob.n_nu:,} points), {args.adjoint} perturbed profiles"},
+           "fd_checks_profile0": checks}
+    log(f"[bench] adjoint: {dt * 1e3:.1f} ms/profile; fd checks {checks}")
+    print(json.dumps(res), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -147,6 +257,10 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--step", type=float, default=0.001, help="grid step [cm-1]")
+    ap.add_argument("--adjoint", type=int, default=0, metavar="N_PROFILES",
+                    help="run the batched-gradient workload instead (1000-1100 cm-1 at 1e-4)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -172,10 +286,12 @@ def main() -> int:
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    if args.adjoint:
+        return adjoint_bench(args, dev, torch)
     t = time.perf_counter()
     sd, prof = synth.five_gas(lines_per_gas=args.lines_per_gas)
     lay = prof.build_layers()
-    prob = OpticalDepthProblem(sd, lay, 500.0, 2500.0, 0.001, 25.0, 2.0, device=dev)
+    prob = OpticalDepthProblem(sd, lay, 500.0, 2500.0, args.step, 25.0, 2.0, device=dev)
     E = prob.count_evaluations()
     log(f"[bench] setup {time.perf_counter() - t:.1f}s, lines {prob.n_lines}, layers "
         f"{prob.n_layers}, points {prob.n_nu}, evaluations/step {E:.4e}")
@@ -192,8 +308,8 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"mixed": "f32", "fp64": "f64", "surrogate": "bf16+f32"}[args.precision],
         "data": "synthetic (seeded PCG64 line lists and profiles, generated in-process)",
-        "config": {"workload": "five gases, 500,000 lines, 50 layers, 500-2500 cm-1 at 0.001 "
-                               "cm-1 (2,000,001 points), 25 cm-1 cutoff, 2 cm-1 tiles, mu 1.0 and 0.5",
+        "config": {"workload": f"five gases, 500,000 lines, 50 layers, 500-2500 cm-1 at {args.step:g} "
+                               f"cm-1 ({prob.n_nu:,} points), 25 cm-1 cutoff, 2 cm-1 tiles, mu 1.0 and 0.5",
                    "precision": args.precision, "lines": prob.n_lines, "layers": prob.n_layers,
                    "grid_points": prob.n_nu, "mu_view": list(mus)},
         "evals_per_step": E,
@@ -211,6 +327,10 @@ def main() -> int:
                           "frac": achieved / peak, "traffic": traffic,
                           "kernel": "k_voigt_sum_" + ("fp64" if args.precision == "fp64" else "mixed"),
                           "flop_per_eval": FLOP_PER_EVAL}
+
+    if args.precision == "mixed" and not args.no_surrogate:
+        result["surrogate"] = surrogate_report(prob, E, value, mus, max(2, args.steps // 2), _capi,
+                                               torch)
 
     gpu_layers = {}
     if not args.no_parity:
