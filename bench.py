@@ -146,8 +146,14 @@ def surrogate_report(prob, E, value_plain, mus, steps, capi, torch):
     from pylblrtm_amd import surrogate as S
     out, sps, k_ms, _ = time_mode(prob, "surrogate", mus, steps, 1, capi, torch)
     rep = {"evals_per_s": E / sps, "ms_per_step": sps * 1e3, "line_sum_kernel_ms": k_ms,
+           "dtype": "bf16 MFMA network + f32",
            "ratio_vs_plain_fp32": (E / sps) / value_plain, "gate": S.SURROGATE_GATE,
            "weights": S.DEFAULT_WEIGHTS.name}
+    census = {}
+    prob.run("surrogate", mus=mus, census=census)
+    census_ug = {}
+    prob.run("surrogate", mus=mus, census=census_ug, gate=float("inf"))
+    rep["census"] = {"gated": census, "ungated": census_ug, "evaluations_per_step": E}
     net = S.default_surrogate(prob.device)
     rep["training"] = {k: v for k, v in net.meta.items() if k != "history"}
     n = 1 << 22

@@ -131,6 +131,8 @@ typedef struct lbl_surrogate {
   const float *params;           /* device, lbl_mlp_num_params() floats */
   const unsigned short *wbf16;   /* device, 3*64*64 BF16 hidden weights */
   double gate;
+  uint64_t *n_network;           /* device counter, incremented by network evaluations (nullable) */
+  uint64_t *n_exact_near;        /* device counter, exact near-centre evaluations (nullable) */
 } lbl_surrogate_t;
 
 /* lbl_optical_depth with precision LBL_SURROGATE (no counterpart in the

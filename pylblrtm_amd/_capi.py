@@ -48,7 +48,8 @@ class LblGrid(ctypes.Structure):
 
 
 class LblSurrogate(ctypes.Structure):
-    _fields_ = [("params", _vp), ("wbf16", _vp), ("gate", ctypes.c_double)]
+    _fields_ = [("params", _vp), ("wbf16", _vp), ("gate", ctypes.c_double),
+                ("n_network", _vp), ("n_exact_near", _vp)]
 
 
 class LblOutputs(ctypes.Structure):

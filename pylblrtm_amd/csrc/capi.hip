@@ -219,6 +219,8 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     sa.mlp_params = sur->params;
     sa.mlp_wbf = sur->wbf16;
     sa.mlp_gate = (float)sur->gate;
+    sa.mlp_count = reinterpret_cast<unsigned long long*>(sur->n_network);
+    sa.exact_count = reinterpret_cast<unsigned long long*>(sur->n_exact_near);
   }
   if (ev) HIP_TRY(hipEventRecord(ev[2], s));
   HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));

@@ -76,6 +76,8 @@ struct SumArgs {
   const float* mlp_params;
   const unsigned short* mlp_wbf;
   float mlp_gate;   // a point uses the network iff A * c * K_ref(x, y) <= mlp_gate
+  unsigned long long* mlp_count;     // network evaluations (nullable)
+  unsigned long long* exact_count;   // exact near-centre evaluations (nullable)
 };
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);

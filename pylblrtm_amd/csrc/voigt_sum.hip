@@ -321,11 +321,14 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P);
           const float x0 = (float)(nu_t - R.c) * R.s;
           const float ac = R.a * (float)R.c;
+          int exact = 0;
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, R.hs, x0);
             if (kSurrogate && surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
             atomicAdd(&s_acc[t * P + j], R.a * K_mixed(x, R.y));
+            ++exact;
           }
+          if (kSurrogate && a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
         }
         if constexpr (kSurrogate) {
           // network pass: each wavefront takes 8 queue entries = 64 points
@@ -348,6 +351,9 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
               am = R.a;
               use = surrogate_point(x, y, am * (float)R.c, a.mlp_gate);
             }
+            const uint64_t users = __ballot(use);
+            if (users == 0) continue;   // wavefront-uniform: nothing for the network here
+            if (a.mlp_count && lane == 0) atomicAdd(a.mlp_count, (unsigned long long)__popcll(users));
             const float u = use ? mlp::input_u(x) : 0.0f;
             const float v = use ? mlp::input_v(__log10f(y)) : 0.0f;
             const float tt = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
@@ -403,9 +409,11 @@ void k_voigt_sum_mixed(const SumArgs a) {
   voigt_sum_mixed_body<P, false>(a);
 }
 
-// The network's MFMA tiles need ~96 more registers; two waves per SIMD.
+// Same register budget as the plain kernel: the network's MFMA tiles spill
+// around the (rarely taken) network pass instead of halving the occupancy
+// of the main loop.
 template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_surrogate(const SumArgs a) {
   voigt_sum_mixed_body<P, true>(a);
 }

@@ -303,12 +303,13 @@ class OpticalDepthProblem:
 
     def run(self, precision: str = "fp64", mus: Sequence[float] = (1.0,), want_tau: bool = False,
             want_layers: bool = False, outputs: Optional[dict] = None, surrogate=None,
-            gate: Optional[float] = None) -> dict:
+            gate: Optional[float] = None, census: Optional[dict] = None) -> dict:
         """Run the device path; returns device tensors (transmittance [n_mu, n_nu], ...).
 
         precision "surrogate" uses ``surrogate`` (a VoigtSurrogate; default:
         the committed network) for near-centre points that pass ``gate``
-        (default SURROGATE_GATE)."""
+        (default SURROGATE_GATE).  If ``census`` is a dict, it receives the
+        number of network and exact near-centre evaluations."""
         torch = _torch()
         if precision not in _capi.PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_capi.PRECISIONS)}; got {precision!r}")
@@ -326,8 +327,11 @@ class OpticalDepthProblem:
             if precision == "surrogate":
                 from .surrogate import SURROGATE_GATE, default_surrogate
                 net = surrogate if surrogate is not None else default_surrogate(self.device)
+                counts = torch.zeros(2, dtype=torch.int64, device=self.device)
                 c_sur = _capi.LblSurrogate(net.params.data_ptr(), net.wbf.data_ptr(),
-                                           float(SURROGATE_GATE if gate is None else gate))
+                                           float(SURROGATE_GATE if gate is None else gate),
+                                           counts.data_ptr() if census is not None else None,
+                                           counts.data_ptr() + 8 if census is not None else None)
                 st = self.lib.lbl_optical_depth_surrogate(
                     ctypes.byref(self.c_lines), ctypes.byref(self.c_groups),
                     ctypes.byref(self.c_layers), ctypes.byref(self.c_grid), ctypes.byref(c_out),
@@ -347,6 +351,9 @@ class OpticalDepthProblem:
             raise FloatingPointError(
                 f"Layer {err_layer.value}: non-finite optical depth after radiation term")
         _capi.raise_for(st, "lbl_optical_depth")
+        if precision == "surrogate" and census is not None:
+            c = counts.cpu().tolist()
+            census.update(network=int(c[0]), exact_near=int(c[1]))
         return out
 
 
