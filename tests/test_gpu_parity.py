@@ -175,3 +175,21 @@ def test_five_gas_tiles_vs_oracle():
     assert np.max(np.abs(Tm - T64)) <= DT_ATOL_MIXED     # whole 2,000,001-point grid
     Tm5 = om["transmittance"][1].double().cpu().numpy()
     assert np.max(np.abs(Tm5 - np.exp(-2.0 * tau))) <= DT_ATOL_MIXED
+
+
+def test_co2_window_full_size_tiles_vs_oracle():
+    say("case: co2 window full size (5000 lines, 20 layers, 700-750 @ 0.001), tiles 0/12/25")
+    sd, prof = synth.co2_window()
+    lay = prof.build_layers()
+    tiles = [0, 12, 25]
+    nu, od = _oracle_tau(sd, lay, 700.0, 750.0, 0.001, 25.0, 2.0, tiles=tiles)
+    m = ~np.isnan(od[0])
+    prob = OpticalDepthProblem(sd, lay, 700.0, 750.0, 0.001, 25.0, 2.0)
+    assert prob.n_nu == 50_001 and prob.tile_samples == 2000
+    o64 = prob.run("fp64", mus=(1.0,), want_tau=True, want_layers=True)
+    odl = o64["optical_depth_layers"].cpu().numpy()
+    rel = np.abs(odl[:, m] - od[:, m]) / od[:, m]
+    assert rel.max() < TAU_RTOL_FP64, rel.max()
+    om = prob.run("mixed", mus=(1.0,), want_tau=True, want_layers=True)
+    relm = np.abs(om["optical_depth_layers"].double().cpu().numpy()[:, m] - od[:, m]) / od[:, m]
+    assert relm.max() < 2e-5, relm.max()
