@@ -158,9 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
   const double nu_l = grid_nu(a.nu_min, a.step, i0);
   const float stepf = (float)a.step;
-  const float* rs = static_cast<const float*>(a.rec.s);
-  const float* ry = static_cast<const float*>(a.rec.y);
-  const float* ra = static_cast<const float*>(a.rec.a);
+  const MixedRec* __restrict__ recs = a.rec.mix;
 
   float W[P];
 #pragma unroll
@@ -194,8 +192,9 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       __syncthreads();
       if ((int)threadIdx.x < cnt) {
         const int64_t r = kb + base + threadIdx.x;
-        double c = a.rec.c[r];
-        float s = rs[r], y = ry[r], am = ra[r];
+        const MixedRec& mr = recs[r];
+        double c = mr.c;
+        float s = mr.s, y = mr.y, am = mr.a;
         VjpRec q;
         q.at = a.at[r]; q.dyt = a.dyt[r]; q.av = a.av[r]; q.bxv = a.bxv[r]; q.dyv = a.dyv[r];
         if (!(c >= lo_b && c <= hi_b)) {

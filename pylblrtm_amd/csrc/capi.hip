@@ -66,13 +66,14 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   if (grid->tile_samples <= 0 || grid->n_nu <= 0) return false;
   const size_t nrec = (size_t)std::max<int64_t>(lines->n_lines, 0) *
                       (size_t)std::max<int32_t>(layers->n_layers, 0);
-  const size_t es = precision == LBL_FP64 ? 8 : 4;
+  const bool fp64 = precision == LBL_FP64;
   L->n_tiles = (grid->n_nu + grid->tile_samples - 1) / grid->tile_samples;
   size_t off = 0;
-  L->c = off; off = align_up(off + nrec * 8);
-  L->s = off; off = align_up(off + nrec * es);
-  L->y = off; off = align_up(off + nrec * es);
-  L->a = off; off = align_up(off + nrec * es);
+  // FP64: SoA c, s, y, a (8 B each); mixed: one 48-byte MixedRec per record at L->c
+  L->c = off; off = align_up(off + nrec * (fp64 ? 8 : sizeof(lbl::MixedRec)));
+  L->s = off; off = align_up(off + (fp64 ? nrec * 8 : 0));
+  L->y = off; off = align_up(off + (fp64 ? nrec * 8 : 0));
+  L->a = off; off = align_up(off + (fp64 ? nrec * 8 : 0));
   L->lo = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->hi = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->err = off; off = align_up(off + 4);
@@ -161,7 +162,8 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   hipStream_t s = static_cast<hipStream_t>(stream);
   char* ws = static_cast<char*>(workspace);
 
-  lbl::Records rec{reinterpret_cast<double*>(ws + L.c), ws + L.s, ws + L.y, ws + L.a};
+  lbl::Records rec{reinterpret_cast<double*>(ws + L.c), ws + L.s, ws + L.y, ws + L.a,
+                   reinterpret_cast<lbl::MixedRec*>(ws + L.c)};
   uint32_t* err = reinterpret_cast<uint32_t*>(ws + L.err);
   int32_t* lo = reinterpret_cast<int32_t*>(ws + L.lo);
   int32_t* hi = reinterpret_cast<int32_t*>(ws + L.hi);
@@ -171,7 +173,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
 
   HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
 
-  lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err};
+  lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err, grid->step};
   if (ev) HIP_TRY(hipEventRecord(ev[0], s));
   HIP_TRY(lbl::launch_line_params(lp, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[1], s));
@@ -205,7 +207,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   sa.n_nu = grid->n_nu;
   sa.ts = grid->tile_samples;
   sa.n_tiles = L.n_tiles;
-  const int64_t pts = (int64_t)lbl::kBlock * lbl::sum_points_per_thread(precision);
+  const int64_t pts = (int64_t)lbl::sum_block(precision) * lbl::sum_points_per_thread(precision);
   sa.slices_per_tile = (int32_t)((grid->tile_samples + pts - 1) / pts);
   sa.n_wg = L.n_tiles * sa.slices_per_tile;
   if (sa.n_wg > INT32_MAX) return fail(LBL_EINVAL, "grid too large");
@@ -298,7 +300,7 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
     return fail(LBL_EINVAL, "workspace of %zu bytes needed, %zu given", L.total, workspace_bytes);
   hipStream_t s = static_cast<hipStream_t>(stream);
   char* ws = static_cast<char*>(workspace);
-  lbl::Records rec{reinterpret_cast<double*>(ws + L.c), ws + L.s, ws + L.y, ws + L.a};
+  lbl::Records rec{nullptr, nullptr, nullptr, nullptr, reinterpret_cast<lbl::MixedRec*>(ws + L.c)};
   uint32_t* err = reinterpret_cast<uint32_t*>(ws + L.err);
   int32_t* lo = reinterpret_cast<int32_t*>(ws + L.lo);
   int32_t* hi = reinterpret_cast<int32_t*>(ws + L.hi);
@@ -306,7 +308,7 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   for (int i = 0; i < 5; ++i) gc[i] = reinterpret_cast<float*>(ws + L.g[i]);
 
   HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
-  lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err};
+  lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err, grid->step};
   HIP_TRY(lbl::launch_line_params(lp, LBL_MIXED, s));
   lbl::TileRangeArgs tr{};
   tr.nu0 = lines->nu0;

@@ -31,11 +31,29 @@ __device__ __forceinline__ double grid_nu(double nu_min, double step, int64_t i)
   return __dadd_rn(nu_min, __dmul_rn((double)i, step));
 }
 
+// Per-(layer, line) record of the mixed-precision paths (48 bytes, one
+// 16-byte-aligned struct so the line sum stages it with three 16-byte loads):
+//   c  shifted centre (FP64), s = sqrt(ln2)/alpha_D, hs = step * s,
+//   cw = y^2 - 1/2, d2 = 2 y^2, ay = A y / sqrt(pi)   (two-node far form),
+//   hw = sqrt(max(25^2 - y^2, 0))  (|x| beyond which the far form holds),
+//   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D.
+// A skipped (zero-column) record has c = NaN, ay = a = 0, d2 = 1.
+struct alignas(16) MixedRec {
+  double c;
+  float s, hs;
+  float cw, d2, ay, hw;
+  float y, a, pad0, pad1;
+};
+static_assert(sizeof(MixedRec) == 48, "record layout");
+
 struct Records {
-  double* c;     // shifted line centre [n_layers * n_lines], FP64 in every mode
+  // FP64 mode: SoA arrays [n_layers * n_lines]
+  double* c;     // shifted line centre
   void* s;       // sqrt(ln2) / alpha_D
   void* y;       // sqrt(ln2) * gamma_L / alpha_D
   void* a;       // strength * sqrt(ln2/pi) / alpha_D (0 when the layer is skipped)
+  // mixed / surrogate modes: AoS [n_layers * n_lines]
+  MixedRec* mix;
 };
 
 struct LineParamsArgs {
@@ -44,6 +62,7 @@ struct LineParamsArgs {
   lbl_layers_t layers;
   Records rec;
   uint32_t* err;
+  double step;   // grid step (for MixedRec::hs)
 };
 
 struct TileRangeArgs {
@@ -111,8 +130,9 @@ hipError_t launch_mlp_step(float* p, float* m, float* v, unsigned short* wbf, co
 hipError_t launch_mlp_eval(int64_t n, const float* x, const float* ly, const float* p,
                            const unsigned short* wbf, float* t, hipStream_t s);
 
-// Points per thread of the line-sum kernels (a workgroup covers kBlock * P
-// consecutive grid points of one reference tile).
+// Points per thread and threads per workgroup of the line-sum kernels (a
+// workgroup covers block * P consecutive grid points of one reference tile).
 int sum_points_per_thread(int precision);
+int sum_block(int precision);
 
 }  // namespace lbl

@@ -11,6 +11,7 @@
 // up to the last-bit differences of exp/expm1/tanh/pow.
 #include <math.h>
 
+#include "faddeeva.h"
 #include "internal.h"
 
 #pragma clang fp contract(off)
@@ -47,28 +48,30 @@ __device__ double interp_like_numpy(double x, const double* xp, const double* fp
   return res;
 }
 
-template <typename R>
-struct Store;
+__device__ __forceinline__ void store64(const Records& r, int64_t i, double c, double s, double y,
+                                        double a) {
+  r.c[i] = c;
+  static_cast<double*>(r.s)[i] = s;
+  static_cast<double*>(r.y)[i] = y;
+  static_cast<double*>(r.a)[i] = a;
+}
 
-template <>
-struct Store<double> {
-  __device__ static void put(const Records& r, int64_t i, double c, double s, double y, double a) {
-    r.c[i] = c;
-    static_cast<double*>(r.s)[i] = s;
-    static_cast<double*>(r.y)[i] = y;
-    static_cast<double*>(r.a)[i] = a;
-  }
-};
-
-template <>
-struct Store<float> {
-  __device__ static void put(const Records& r, int64_t i, double c, float s, float y, float a) {
-    r.c[i] = c;
-    static_cast<float*>(r.s)[i] = s;
-    static_cast<float*>(r.y)[i] = y;
-    static_cast<float*>(r.a)[i] = a;
-  }
-};
+__device__ __forceinline__ void store_mixed(const Records& r, int64_t i, double c, float s, float y,
+                                            float a, float step) {
+  MixedRec m;
+  const float y2 = y * y;
+  m.c = c;
+  m.s = s;
+  m.hs = step * s;
+  m.cw = y2 - 0.5f;
+  m.d2 = 2.0f * y2;
+  m.ay = a * y * (float)kInvSqrtPi;
+  m.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
+  m.y = y;
+  m.a = a;
+  m.pad0 = m.pad1 = 0.0f;
+  r.mix[i] = m;
+}
 
 // R(nu, T) = nu * tanh(c2 nu / 2T)  (forward.py:41-48)
 __device__ __forceinline__ double radiation64(double nu, double T) {
@@ -107,7 +110,10 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
   if (col == 0.0) {
     // forward.py:253-255: the group is skipped for this layer.  A NaN centre
     // fails every tile's inclusion test, so the record is never summed.
-    Store<R>::put(a.rec, out, __builtin_nan(""), R(0), R(1), R(0));
+    if constexpr (sizeof(R) == 8)
+      store64(a.rec, out, __builtin_nan(""), 0.0, 1.0, 0.0);
+    else
+      store_mixed(a.rec, out, __builtin_nan(""), 0.0f, 1.0f, 0.0f, 0.0f);
     return;
   }
   const double nu0 = a.lines.nu0[l];
@@ -127,8 +133,8 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
     const double lorentz =
         pow(kTRef / T, a.lines.n_air[l]) * (a.lines.g_air[l] * p_air + a.lines.g_self[l] * p_self);
     const double doppler = nu0 * kDoppler * sqrt(T / molar_mass);
-    Store<double>::put(a.rec, out, c, kSqrtLn2 / doppler, kSqrtLn2 * lorentz / doppler,
-                       strength * kVoigtK / doppler);
+    store64(a.rec, out, c, kSqrtLn2 / doppler, kSqrtLn2 * lorentz / doppler,
+            strength * kVoigtK / doppler);
   } else {
     const float Tf = (float)T;
     const float nu0f = (float)nu0;
@@ -143,8 +149,8 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
                           ((float)a.lines.g_air[l] * (float)p_air +
                            (float)a.lines.g_self[l] * (float)p_self);
     const float doppler = nu0f * (float)kDoppler * sqrtf(Tf / (float)molar_mass);
-    Store<float>::put(a.rec, out, c, (float)kSqrtLn2 / doppler,
-                      (float)kSqrtLn2 * lorentz / doppler, strength * (float)kVoigtK / doppler);
+    store_mixed(a.rec, out, c, (float)kSqrtLn2 / doppler, (float)kSqrtLn2 * lorentz / doppler,
+                strength * (float)kVoigtK / doppler, (float)a.step);
   }
 }
 

@@ -169,24 +169,6 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_fp64(const SumArgs a) {
 
 // ----------------------------------------------------------------- mixed --
 
-// One staged record of the mixed kernel (48 bytes, 16-byte aligned):
-//   c (FP64 centre), s = sqrt(ln2)/alpha_D, hs = step * s        -> 16 B
-//   cw = y^2 - 1/2, d2 = 2 y^2, ay = A y / sqrt(pi), hw           -> 16 B
-//   y, A (used by the full dispatch only), 2 pad                  -> 16 B
-// hw = sqrt(max(R^2 - y^2, 0)) is the |x| beyond which the two-node form
-// holds (R^2 = kF32Gh2R2), so a lane's P points are all in that region iff
-// x_first >= hw or x_last <= -hw.
-struct alignas(16) MixedRec {
-  double c;
-  float s, hs;
-  float cw, d2, ay, hw;
-  float y, a, pad0, pad1;
-};
-static_assert(sizeof(MixedRec) == 48, "record layout");
-
-// Four waves per SIMD (<= 128 VGPRs) lets every tile of a 2 cm^-1 x 0.001
-// grid (1001 workgroups) be resident at once on 256 CUs; at three waves the
-// last quarter of the tiles ran as a second, one-wave-per-SIMD round.
 // Surrogate mode: a queued point goes to the network iff it lies in the
 // trained domain and its optical-depth contribution bound A c K_ref(x, y)
 // (R(nu, T) <= nu ~ c near the centre, K <= 1.6 K_ref over the domain) is
@@ -198,6 +180,11 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
   return ac * mlp::kref(x, y) <= gate;
 }
 
+// Mixed-precision line sum: kMixedBlock threads x P points per workgroup.
+// Records arrive precomputed (MixedRec); staging is a copy plus the tile's
+// exact inclusion test, two records per thread per chunk of kChunk.
+constexpr int kMixedBlock = 256;
+
 template <int P, bool kSurrogate>
 __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ MixedRec s_rec[kChunk];
@@ -207,7 +194,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // the whole workgroup at the end of the chunk.  Results come back through
   // s_acc (LDS float atomics) into the owning thread's partial sums.
   __shared__ uint16_t s_q[kQueueCap];
-  __shared__ float s_acc[kBlock * P];
+  __shared__ float s_acc[kMixedBlock * P];
   __shared__ int s_qn;
 
   const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
@@ -215,17 +202,14 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
-  const int64_t s0 = t0 + (int64_t)slice * (kBlock * P);   // first point of the slice
+  const int64_t s0 = t0 + (int64_t)slice * (kMixedBlock * P);   // first point of the slice
   const int64_t i0 = s0 + (int64_t)threadIdx.x * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
   const double nu_l = grid_nu(a.nu_min, a.step, i0);
   const bool live = i0 < t1;   // lanes wholly past the tile end never write
-  const float stepf = (float)a.step;
-  const float* rs = static_cast<const float*>(a.rec.s);
-  const float* ry = static_cast<const float*>(a.rec.y);
-  const float* ra = static_cast<const float*>(a.rec.a);
+  const MixedRec* __restrict__ recs = a.rec.mix;
 
   for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0f;
   if (threadIdx.x == 0) s_qn = 0;
@@ -242,26 +226,17 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     for (int32_t base = rlo; base < rhi; base += kChunk) {
       const int cnt = min(kChunk, rhi - base);
       __syncthreads();
-      if ((int)threadIdx.x < cnt) {
-        const int64_t r = kb + base + threadIdx.x;
-        double c = a.rec.c[r];
-        float s = rs[r], y = ry[r], am = ra[r];
-        if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped)
-          c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
+#pragma unroll
+      for (int h = 0; h < kChunk / kMixedBlock; ++h) {
+        const int q = threadIdx.x + h * kMixedBlock;
+        if (q < cnt) {
+          MixedRec m = recs[kb + base + q];
+          if (!(m.c >= lo_b && m.c <= hi_b)) {  // outside the tile window (or skipped)
+            m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.cw = 0.0f; m.d2 = 1.0f;
+            m.ay = 0.0f; m.hw = 0.0f; m.a = 0.0f;
+          }
+          s_rec[q] = m;
         }
-        const float y2 = y * y;
-        MixedRec rec;
-        rec.c = c;
-        rec.s = s;
-        rec.hs = stepf * s;
-        rec.cw = y2 - 0.5f;
-        rec.d2 = 2.0f * y2;
-        rec.ay = am * y * (float)kInvSqrtPi;
-        rec.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
-        rec.y = y;
-        rec.a = am;
-        rec.pad0 = rec.pad1 = 0.0f;
-        s_rec[threadIdx.x] = rec;
       }
       __syncthreads();
       for (int q = 0; q < cnt; ++q) {
@@ -284,11 +259,11 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           }
         } else if (live) {
           // append (q, thread) with one LDS atomic per wavefront
-          const uint64_t m = __ballot(1);
+          const uint64_t msk = __ballot(1);
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
           int qb = 0;
-          if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(m));
+          if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(msk));
           qb = __builtin_amdgcn_readfirstlane(qb);
           const int pos = qb + (int)rank;
           if (pos < kQueueCap) {
@@ -314,7 +289,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       __syncthreads();
       const int nq = min(s_qn, kQueueCap);
       if (nq > 0) {
-        for (int e = threadIdx.x; e < nq; e += kBlock) {
+        for (int e = threadIdx.x; e < nq; e += kMixedBlock) {
           const uint32_t ent = s_q[e];
           const int q = (int)(ent >> 8), t = (int)(ent & 0xFF);
           const MixedRec& R = s_rec[q];
@@ -331,12 +306,13 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           if (kSurrogate && a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
         }
         if constexpr (kSurrogate) {
-          // network pass: each wavefront takes 8 queue entries = 64 points
-          // per round (lane = entry (l / 8), point (l % 8)); all lanes stay
-          // active through the MFMA evaluation.
+          // network pass: each wavefront takes 64 / P queue entries (64
+          // points) per round (lane = entry (l / P), point (l % P)); all lanes
+          // stay active through the MFMA evaluation.
+          constexpr int kPer = 64 / P;
           const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-          for (int g = wave; g * 8 < nq; g += kBlock / 64) {
-            const int e = g * 8 + (lane >> 3), j = lane & 7;
+          for (int g = wave; g * kPer < nq; g += kMixedBlock / 64) {
+            const int e = g * kPer + lane / P, j = lane % P;
             bool use = false;
             float x = 0.0f, y = 1.0f, am = 0.0f;
             int t = 0;
@@ -403,17 +379,20 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   }
 }
 
+// Four waves per SIMD (<= 128 VGPRs): every tile of a 2 cm^-1 x 0.001 grid
+// (1001 workgroups) is resident at once on 256 CUs.  (Measured: 3 waves/SIMD
+// left a second partial round, +12%; 16 points x 128 threads at 2 waves/SIMD
+// hid latency worse, +20%.)
 template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_mixed(const SumArgs a) {
   voigt_sum_mixed_body<P, false>(a);
 }
 
-// Same register budget as the plain kernel: the network's MFMA tiles spill
-// around the (rarely taken) network pass instead of halving the occupancy
-// of the main loop.
+// Same register budget: the network's MFMA tiles spill around the (rarely
+// taken) network pass instead of halving the occupancy of the main loop.
 template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_surrogate(const SumArgs a) {
   voigt_sum_mixed_body<P, true>(a);
 }
@@ -439,14 +418,16 @@ int sum_points_per_thread(int precision) {
   return precision == LBL_FP64 ? kPointsFp64 : kPointsMixed;
 }
 
+int sum_block(int precision) { return precision == LBL_FP64 ? kBlock : kMixedBlock; }
+
 hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0) return hipSuccess;
   if (precision == LBL_FP64)
     k_voigt_sum_fp64<kPointsFp64><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
   else if (precision == LBL_SURROGATE)
-    k_voigt_sum_surrogate<kPointsMixed><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+    k_voigt_sum_surrogate<kPointsMixed><<<(unsigned)a.n_wg, kMixedBlock, 0, s>>>(a);
   else
-    k_voigt_sum_mixed<kPointsMixed><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+    k_voigt_sum_mixed<kPointsMixed><<<(unsigned)a.n_wg, kMixedBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 

@@ -59,8 +59,8 @@ def test_workspace_size_is_host_arithmetic():
     n = ctypes.c_size_t(0)
     assert lib.lbl_workspace_bytes(ctypes.byref(lines), ctypes.byref(layers), ctypes.byref(grid),
                                    1, ctypes.byref(n)) == 0
-    assert n.value >= 25_000_000 * 20
+    assert n.value >= 25_000_000 * 48          # one 48-byte record per (layer, line)
     m = ctypes.c_size_t(0)
     lib.lbl_workspace_bytes(ctypes.byref(lines), ctypes.byref(layers), ctypes.byref(grid), 0,
                             ctypes.byref(m))
-    assert m.value >= 25_000_000 * 32 and m.value > n.value
+    assert m.value >= 25_000_000 * 32          # FP64: four 8-byte arrays
