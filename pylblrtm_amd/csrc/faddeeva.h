@@ -37,9 +37,11 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kInvPi = 0.31830988618379067154;
 constexpr double kInvSqrtPi = 0.56418958354775628695;
 
-// Region thresholds on r = x^2 + y^2.
-constexpr double kFp64Gh4R2 = 3600.0;   // |z| >= 60: 4 nodes, rel err <= 1e-13
-constexpr double kFp64Gh8R2 = 225.0;    // |z| >= 15: 8 nodes, rel err <= 1e-15
+// Region thresholds on r = x^2 + y^2.  FP64 keeps every region below 1.1e-11
+// relative error in K (the FP64 mode's target is 1e-9 in tau).
+constexpr double kFp64Gh2R2 = 490000.0; // |z| >= 700: 2 nodes, rel err <= 1.04e-11
+constexpr double kFp64Gh4R2 = 1225.0;   // |z| >= 35 : 4 nodes, rel err <= 5.1e-12
+constexpr double kFp64Gh8R2 = 225.0;    // |z| >= 15 : 8 nodes, rel err <= 1e-15
 constexpr float kF32Gh2R2 = 625.0f;     // |z| >= 25: 2 nodes, rel err <= 6.4e-6
 constexpr float kF32Gh4R2 = 64.0f;      // |z| >= 8 : 4 nodes, rel err <= 1e-6
 
@@ -78,7 +80,14 @@ __device__ __forceinline__ double gh_pair64(double r, double xmy, double fxy, do
   return w * (r + t2) / fma(u, u, fxy);
 }
 
-// 4-node form, |z| >= 60.
+// 2-node form, |z| >= 700: K = y (w + 1) / (sqrt(pi) (w^2 + 2 y^2)), w = x^2 + y^2 - 1/2.
+__device__ __forceinline__ double K_gh2_64(double x2, double y) {
+  const double y2 = y * y;
+  const double w = x2 + (y2 - 0.5);
+  return y * kInvSqrtPi * (w + 1.0) / fma(w, w, 2.0 * y2);
+}
+
+// 4-node form, |z| >= 35.
 __device__ __forceinline__ double K_gh4_64(double x2, double y) {
   const double y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
   const double s = gh_pair64(r, xmy, fxy, kGh4T[0], kGh4W[0]) +
@@ -126,10 +135,11 @@ __device__ inline double K_cr64(double x, double y) {
   return K;
 }
 
-// Full FP64 region dispatch: relative error ~1e-13 or better everywhere.
+// Full FP64 region dispatch: relative error <= 1.1e-11 everywhere.
 __device__ inline double K_fp64(double x, double y) {
   const double x2 = x * x;
   const double r = x2 + y * y;
+  if (r >= kFp64Gh2R2) return K_gh2_64(x2, y);
   if (r >= kFp64Gh4R2) return K_gh4_64(x2, y);
   if (r >= kFp64Gh8R2) return K_gh8_64(x2, y);
   return K_cr64(x, y);

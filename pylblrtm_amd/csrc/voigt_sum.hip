@@ -71,9 +71,11 @@ __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
 // ------------------------------------------------------------------ FP64 --
 
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_voigt_sum_fp64(const SumArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_voigt_sum_fp64(const SumArgs a) {
   __shared__ double s_c[kChunk], s_s[kChunk], s_y[kChunk], s_a[kChunk];
   __shared__ double s_y2[kChunk], s_w0[kChunk], s_w1[kChunk];
+  __shared__ double s_cw[kChunk], s_ay[kChunk];
 
   const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
   const int64_t tile = wg / a.slices_per_tile;
@@ -112,15 +114,28 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_fp64(const SumArgs a) {
         }
         const int q = threadIdx.x;
         const double w = am * 2.0 * y * kInvPi;
+        const double y2 = y * y;
         s_c[q] = c; s_s[q] = s; s_y[q] = y; s_a[q] = am;
-        s_y2[q] = y * y; s_w0[q] = w * kGh4W[0]; s_w1[q] = w * kGh4W[1];
+        s_y2[q] = y2; s_w0[q] = w * kGh4W[0]; s_w1[q] = w * kGh4W[1];
+        s_cw[q] = y2 - 0.5; s_ay[q] = am * y * kInvSqrtPi;
       }
       __syncthreads();
       for (int q = 0; q < cnt; ++q) {
         const double c = s_c[q], s = s_s[q], y2 = s_y2[q];
         const double xa = (nu[0] - c) * s, xb = (nu[P - 1] - c) * s;
         const double mn = xa > 0.0 ? xa : (xb < 0.0 ? -xb : 0.0);
-        if (fma(mn, mn, y2) >= kFp64Gh4R2) {
+        const double rmin = fma(mn, mn, y2);
+        if (rmin >= kFp64Gh2R2) {
+          // two-node form (|z| >= 700)
+          const double cw = s_cw[q], d2 = 2.0 * y2, ay = s_ay[q];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const double x = (nu[j] - c) * s;
+            const double w = fma(x, x, cw);
+            lay[j] = fma(fma(ay, w, ay), fast_rcp64(fma(w, w, d2)), lay[j]);
+          }
+        } else if (rmin >= kFp64Gh4R2) {
+          // four-node form (|z| >= 35)
           const double w0 = s_w0[q], w1 = s_w1[q], fy2 = 4.0 * y2;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
@@ -134,10 +149,14 @@ __global__ __launch_bounds__(kBlock) void k_voigt_sum_fp64(const SumArgs a) {
           }
         } else {
           const double y = s_y[q], am = s_a[q];
-#pragma unroll
+#pragma unroll 1
           for (int j = 0; j < P; ++j) {
-            const double x = (nu[j] - c) * s;
-            lay[j] = fma(am, K_fp64(x, y), lay[j]);
+            double nj = nu[0], lj = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < P; ++jj) nj = (jj == j) ? nu[jj] : nj;
+            lj = am * K_fp64((nj - c) * s, y);
+#pragma unroll
+            for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? lj : 0.0;
           }
         }
       }

@@ -41,7 +41,7 @@ def test_voigt_fp64_known_answers():
     z = np.array([complex(*v) for v in kat["z"]])
     w = np.array([complex(*v) for v in kat["w"]])
     K = voigt_eval(z.real, z.imag, "fp64")
-    np.testing.assert_allclose(K, w.real, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(K, w.real, rtol=2e-11, atol=0)
 
 
 def test_voigt_fp64_against_scipy_plane():
@@ -50,7 +50,7 @@ def test_voigt_fp64_against_scipy_plane():
     K = voigt_eval(x, y, "fp64")
     ref = wofz(x + 1j * y).real
     rel = np.abs(K - ref) / ref
-    assert rel.max() < 1e-12, (rel.max(), x[rel.argmax()], y[rel.argmax()])
+    assert rel.max() < 2e-11, (rel.max(), x[rel.argmax()], y[rel.argmax()])
 
 
 def test_voigt_fp64_real_axis_and_imag_axis():
@@ -59,8 +59,8 @@ def test_voigt_fp64_real_axis_and_imag_axis():
     x = np.linspace(-5.5, 5.5, 1001)
     np.testing.assert_allclose(voigt_eval(x, np.zeros_like(x), "fp64"), np.exp(-x * x),
                                rtol=1e-12, atol=1e-300)
-    y = np.geomspace(1e-6, 1e3, 500)
-    np.testing.assert_allclose(voigt_eval(np.zeros_like(y), y, "fp64"), erfcx(y), rtol=1e-12)
+    y = np.geomspace(1e-6, 1e3, 500)       # y >= 700 takes the 2-node form (<= 1.04e-11)
+    np.testing.assert_allclose(voigt_eval(np.zeros_like(y), y, "fp64"), erfcx(y), rtol=2e-11)
 
 
 def test_voigt_mixed_against_scipy_plane():
