@@ -191,11 +191,17 @@ def adjoint_bench(args, dev, torch):
     """Batched gradients: seeded perturbations of the five-gas layers (T_k +
     N(0, 2 K), vmr x lognormal(0, 0.1)) over 1000-1100 cm^-1 at 1e-4 cm^-1
     (1,000,001 points); per profile one mixed forward and one adjoint
-    (dT/dT_k, dT/dvmr_{k,g}).  Checked on the first profile against central
-    differences (h = 1e-3 relative) of the FP64 forward for a few parameters."""
+    (dT/dT_k, dT/dvmr_{k,g}).  Line strengths are scaled by 1e-6 so the column
+    is translucent (at full strength T = 0 and every gradient is exactly 0);
+    the kernels do the same work either way.  Checked on the first profile
+    against central differences of the FP64 forward for a few parameters."""
     from pylblrtm_amd import synth
     from pylblrtm_amd.device import OpticalDepthProblem
+    from pylblrtm_amd.spectral_data import SpectralData
     sd, prof = synth.five_gas()
+    lines = dict(sd.lines)
+    lines["Sref"] = np.asarray(sd.lines["Sref"]) * 1e-6
+    sd = SpectralData(lines=lines, tips=sd.tips, molar_masses=sd.molar_masses)
     lay = prof.build_layers()
     prob = OpticalDepthProblem(sd, lay, 1000.0, 1100.0, 1e-4, 25.0, 2.0, device=dev)
     E = prob.count_evaluations()
@@ -216,37 +222,51 @@ def adjoint_bench(args, dev, torch):
         grads.append((gt, gv))
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.adjoint
-    # finite-difference spot check on profile 0 (layers with most signal)
+    # Finite-difference spot checks on profile 0: the three layer temperatures
+    # with the largest |dL/dT| whose +-0.01 K step stays inside one 1 K TIPS
+    # segment (Q(T) is piecewise linear), and the three mixing ratios with the
+    # largest log-sensitivity |v dL/dv| (h = 1e-3 v).  The two transmittance
+    # spectra are differenced before the sum with G, so the loss's own
+    # magnitude does not round the difference away.
     prob.set_layers(T_p[0], V(0))
     gt0, gv0 = (x.cpu().numpy() for x in grads[0])
-    def loss(tt, vv):
+    Gd = G.double()
+
+    def spectrum(tt, vv):
         prob.set_layers(tt, vv)
-        return float((prob.run("fp64", mus=mus)["transmittance"] * G.double()).sum())
+        return prob.run("fp64", mus=mus)["transmittance"]
+
+    def fd(tp, vp, tm, vm, h):
+        return float(((spectrum(tp, vp) - spectrum(tm, vm)) * Gd).sum()) / (2 * h)
+
     checks = []
-    for k in np.argsort(-np.abs(gt0))[:3]:
-        h = 1e-3 * T_p[0][k]
+    h_t = 0.01
+    ok = np.floor(T_p[0] - h_t) == np.floor(T_p[0] + h_t)
+    for k in [k for k in np.argsort(-np.abs(gt0)) if ok[k]][:3]:
         tp, tm = T_p[0].copy(), T_p[0].copy()
-        tp[k] += h
-        tm[k] -= h
-        fd = (loss(tp, V(0)) - loss(tm, V(0))) / (2 * h)
-        checks.append({"param": f"T[{int(k)}]", "adjoint": float(gt0[k]), "fd": fd,
-                       "rel": abs(gt0[k] - fd) / max(abs(fd), 1e-300)})
-    g_idx = np.unravel_index(np.argsort(-np.abs(gv0), axis=None)[:3], gv0.shape)
+        tp[k] += h_t
+        tm[k] -= h_t
+        d = fd(tp, V(0), tm, V(0), h_t)
+        checks.append({"param": f"T[{int(k)}]", "adjoint": float(gt0[k]), "fd": d,
+                       "rel": abs(gt0[k] - d) / max(abs(d), 1e-300)})
+    g_idx = np.unravel_index(np.argsort(-np.abs(gv0 * V(0)), axis=None)[:3], gv0.shape)
     for g, k in zip(*g_idx):
         h = 1e-3 * V(0)[g, k]
         vp, vm = V(0), V(0)
         vp[g, k] += h
         vm[g, k] -= h
-        fd = (loss(T_p[0], vp) - loss(T_p[0], vm)) / (2 * h)
+        d = fd(T_p[0], vp, T_p[0], vm, h)
         checks.append({"param": f"vmr[{prob.gases[g]}][{int(k)}]", "adjoint": float(gv0[g, k]),
-                       "fd": fd, "rel": abs(gv0[g, k] - fd) / max(abs(fd), 1e-300)})
+                       "fd": d, "rel": abs(gv0[g, k] - d) / max(abs(d), 1e-300)})
     res = {"metric": "profiles/s (mixed forward + adjoint, 50 layers x 5 gases)",
            "value": 1.0 / dt, "unit": "profiles/s", "n_gpus": 1, "steps": args.adjoint,
            "ms_per_profile": dt * 1e3, "forward_evals_per_profile": E,
            "evals_per_s_forward_plus_adjoint": 2 * E / dt, "higher_is_better": True,
            "dtype": "f32", "data": "synthetic (seeded PCG64)",
            "config": {"workload": f"five-gas lines, {prob.n_lines} in window, 50 layers, 1000-1100 "
-                                  f"cm-1 at 1e-4 ({prob.n_nu:,} points), {args.adjoint} perturbed profiles"},
+                                  f"cm-1 at 1e-4 ({prob.n_nu:,} points), line strengths x 1e-6 (translucent), "
+                                  f"{args.adjoint} perturbed profiles"},
+           "median_T_last_profile": float(T[0].median()),
            "fd_checks_profile0": checks}
     log(f"[bench] adjoint: {dt * 1e3:.1f} ms/profile; fd checks {checks}")
     print(json.dumps(res), flush=True)

@@ -19,8 +19,9 @@
 // The backward sum recomputes K and its derivatives (no lines x grid storage):
 //   far (|z| >= 25, the forward's two-node form, w = x^2 + y^2 - 1/2, D = w^2 + 2y^2):
 //     K = y (w+1)/(sqrt(pi) D),  dK/dx = 2xy (2y^2 - w^2 - 2w)/(sqrt(pi) D^2),
-//     dK/dy = ((w + 1 + 2y^2) D - 4 y^2 (w+1)^2)/(sqrt(pi) D^2)
-//   near: K, L = Re, Im w from W_mixed, dK/dx = -2(xK - yL), dK/dy = 2(xL + yK) - 2/sqrt(pi).
+//     dK/dy = ((w + 1 + 2y^2) D - 4 y^2 (w+1)^2)/(sqrt(pi) D^2),
+//   accumulated per record as five point sums weighted once per record (VjpFar)
+//   near: K, dK/dx, dK/dy from kgrad_mixed (each region's approximant differentiated).
 #include <math.h>
 
 #include "faddeeva.h"
@@ -129,13 +130,27 @@ struct VjpArgs {
   double* grad_vmr;            // [n_gases][n_layers]
 };
 
-struct alignas(16) VjpRec {
+// Far-field record of one (layer, line) pair as staged in LDS.  Away from the
+// centre K = ay F with F = (w+1)/D, and per grid point
+//   dK/dx = 2 ay x F',  dK/dy = aq (F + d2 F' - 2 d2 F/D),  F' = (d2 - w^2 - 2w)/D^2,
+// so a record's contribution to both gradients is a fixed combination of five
+// running sums over its points: S1 = sum W F, S2 = sum W F', S3 = sum W x F',
+// S4 = sum W x^2 F', S5 = sum W F/D (W = the weighted upstream gradient).
+// The coefficients below are those combinations' weights; dx/dT = -x/(2T)
+// multiplies S4, dx/dv = bx multiplies S3.
+struct alignas(16) VjpFar {
   double c;
   float s, hs;
-  float cw, d2, ay, aq;      // y^2 - 1/2, 2 y^2, A y/sqrt(pi), A/sqrt(pi)
-  float hw, y, a, at;        // |x| threshold, y, A, d ln A/dT
-  float dyt, av, bxv, dyv;
-  int32_t gas, pad0, pad1, pad2;
+  float cw, d2, hw, ay;      // y^2 - 1/2, 2 y^2, |x| threshold, A y/sqrt(pi)
+  float tF, tX2, tD, tQ;     // temperature: weights of S1, S4, S2, S5
+  float vF, vX, vD, vQ;      // mixing ratio: weights of S1, S3, S2, S5
+};
+
+// What the near-centre branch (complex Faddeeva, per point) needs.
+struct alignas(16) VjpNear {
+  float y, a, at, dyt;
+  float av, bxv, dyv;
+  int32_t gas;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -146,7 +161,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
-  __shared__ VjpRec s_rec[kVjpChunk];
+  __shared__ VjpFar s_far[kVjpChunk];
+  __shared__ VjpNear s_near[kVjpChunk];
   const int64_t wg = blockIdx.x;
   const int64_t tile = wg / a.slices_per_tile;
   const int slice = (int)(wg - tile * a.slices_per_tile);
@@ -195,31 +211,39 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
         const MixedRec& mr = recs[r];
         double c = mr.c;
         float s = mr.s, y = mr.y, am = mr.a;
-        VjpRec q;
-        q.at = a.at[r]; q.dyt = a.dyt[r]; q.av = a.av[r]; q.bxv = a.bxv[r]; q.dyv = a.dyv[r];
+        float at = a.at[r], dyt = a.dyt[r], av = a.av[r], bxv = a.bxv[r], dyv = a.dyv[r];
         if (!(c >= lo_b && c <= hi_b)) {
+          // outside this tile's window: a record that adds exactly zero
           c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
-          q.at = q.dyt = q.av = q.bxv = q.dyv = 0.0f;
+          at = dyt = av = bxv = dyv = 0.0f;
         }
         const float y2 = y * y;
-        q.c = c; q.s = s; q.hs = stepf * s;
-        q.cw = y2 - 0.5f; q.d2 = 2.0f * y2;
-        q.ay = am * y * (float)kInvSqrtPi; q.aq = am * (float)kInvSqrtPi;
-        q.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); q.y = y; q.a = am;
-        q.gas = a.gas_of_group[a.group[base + threadIdx.x]];
-        q.pad0 = q.pad1 = q.pad2 = 0;
-        s_rec[threadIdx.x] = q;
+        const float d2 = 2.0f * y2;
+        const float ay = am * y * (float)kInvSqrtPi;
+        const float aq = am * (float)kInvSqrtPi;
+        const float cty = dyt * aq, cvy = dyv * aq;
+        VjpFar f;
+        f.c = c; f.s = s; f.hs = stepf * s;
+        f.cw = y2 - 0.5f; f.d2 = d2;
+        f.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
+        f.tF = at * ay + cty; f.tX2 = 2.0f * ax_t * ay; f.tD = cty * d2; f.tQ = -2.0f * d2 * cty;
+        f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
+        s_far[threadIdx.x] = f;
+        VjpNear n;
+        n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
+        n.gas = a.gas_of_group[a.group[base + threadIdx.x]];
+        s_near[threadIdx.x] = n;
       }
       __syncthreads();
       for (int qi = 0; qi < cnt; ++qi) {
-        const VjpRec& R = s_rec[qi];
+        const VjpFar& R = s_far[qi];
         const float x0 = (float)(nu_l - R.c) * R.s;
         const float hs = R.hs;
         const float xb = fmaf((float)(P - 1), hs, x0);
         float acc_t = 0.0f, acc_v = 0.0f;
         if (x0 >= R.hw || xb <= -R.hw) {
-          const float cw = R.cw, d2 = R.d2, ay = R.ay, aq = R.aq;
-          const float at = R.at, dyt = R.dyt, av = R.av, bxv = R.bxv, dyv = R.dyv;
+          const float cw = R.cw, d2 = R.d2, ay = R.ay;
+          float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f, S5 = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, hs, x0);
@@ -227,35 +251,41 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
             const float D = fmaf(w, w, d2);
             const float rD = __builtin_amdgcn_rcpf(D);
             const float w1 = w + 1.0f;
-            const float AK = ay * w1 * rD;
-            const float r2 = rD * rD;
-            const float Kx = 2.0f * ay * x * (d2 - w * (w + 2.0f)) * r2;
-            const float Ky = aq * ((w1 + d2) * D - 2.0f * d2 * w1 * w1) * r2;
-            part[j] += AK;
-            acc_t = fmaf(WR[j], at * AK + Kx * (ax_t * x) + Ky * dyt, acc_t);
-            acc_v = fmaf(WR[j], av * AK + Kx * bxv + Ky * dyv, acc_v);
+            const float F = w1 * rD;
+            part[j] = fmaf(ay, F, part[j]);
+            S1 = fmaf(WR[j], F, S1);
+            const float q = (WR[j] * rD) * rD;
+            const float qe = q * fmaf(-w, w + 2.0f, d2);
+            const float xqe = x * qe;
+            S2 += qe;
+            S3 += xqe;
+            S4 = fmaf(x, xqe, S4);
+            S5 = fmaf(q, w1, S5);
           }
+          acc_t = fmaf(R.tF, S1, fmaf(R.tX2, S4, fmaf(R.tD, S2, R.tQ * S5)));
+          acc_v = fmaf(R.vF, S1, fmaf(R.vX, S3, fmaf(R.vD, S2, R.vQ * S5)));
         } else {
+          const VjpNear& N = s_near[qi];
 #pragma unroll 1
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, hs, x0);
-            float K, L;
-            W_mixed(x, R.y, K, L);
-            const float AK = R.a * K;
-            const float Kx = -2.0f * R.a * (x * K - R.y * L);
-            const float Ky = R.a * (2.0f * (x * L + R.y * K) - 2.0f * (float)kInvSqrtPi);
+            float K, Kx, Ky;
+            kgrad_mixed(x, N.y, K, Kx, Ky);
+            const float AK = N.a * K;
+            Kx *= N.a;
+            Ky *= N.a;
             float wr = 0.0f;
 #pragma unroll
             for (int jj = 0; jj < P; ++jj) {
               wr = (jj == j) ? WR[jj] : wr;
               part[jj] += (jj == j) ? AK : 0.0f;
             }
-            acc_t = fmaf(wr, R.at * AK + Kx * (ax_t * x) + Ky * R.dyt, acc_t);
-            acc_v = fmaf(wr, R.av * AK + Kx * R.bxv + Ky * R.dyv, acc_v);
+            acc_t = fmaf(wr, N.at * AK + Kx * (ax_t * x) + Ky * N.dyt, acc_t);
+            acc_v = fmaf(wr, N.av * AK + Kx * N.bxv + Ky * N.dyv, acc_v);
           }
         }
         g_t += (double)acc_t;
-        const int gas = __builtin_amdgcn_readfirstlane(R.gas);
+        const int gas = __builtin_amdgcn_readfirstlane(s_near[qi].gas);
 #pragma unroll
         for (int g = 0; g < 8; ++g)
           if (g == gas) g_v[g] += (double)acc_v;

@@ -217,56 +217,65 @@ __device__ inline float K_mixed(float x, float y) {
   return K_cr32(x, y);
 }
 
-// ------------------------------------------- FP32, both parts of w(z) ----
-// K = Re w and L = Im w with the same region split as K_mixed; used by the
-// adjoint, whose derivatives are dK/dx = -2(xK - yL), dK/dy = 2(xL + yK) - 2/sqrt(pi).
-// Imaginary part of a symmetric node pair: (2 x W / pi) (r - t^2) / D.
-
-__device__ inline void W_gh_pairs32(float x, float y, const double* T, const double* Wt, int n,
-                                    float& K, float& L) {
-  const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
-  float sk = 0.0f, sl = 0.0f;
-  for (int k = n - 1; k >= 0; --k) {
-    const float t2 = (float)(T[k] * T[k]);
-    const float u = xmy - t2;
-    const float inv = 1.0f / fmaf(u, u, fxy);
-    sk = fmaf((float)Wt[k] * (r + t2), inv, sk);
-    sl = fmaf((float)Wt[k] * (r - t2), inv, sl);
+// ------------------------------------------- FP32, K and its gradient ----
+// K, dK/dx and dK/dy with the same region split as K_mixed, each region's
+// approximant differentiated analytically (used by the adjoint).  The
+// identity w' = -2 z w + 2i/sqrt(pi) is not used: for |z| >= 8 it cancels
+// to O(1/|z|^2) and would multiply the approximant's error by ~2|z|^2.
+//
+// Symmetric node pair (node t, weight W), w = x^2 + y^2 - t^2, d = 4 t^2 y^2,
+// D = w^2 + d (= (x^2 - y^2 - t^2)^2 + 4 x^2 y^2):
+//   K  += (2W/pi) y F,            F  = (w + 2t^2)/D,
+//   Kx += (2W/pi) 2 x y F',       F' = dF/dw = (4 t^2 (t^2 - x^2) - w^2)/D^2,
+//   Ky += (2W/pi) (F + 2 y^2 F' - 2 d F/D).
+template <int N>
+__device__ inline void kgrad_pairs32(float x, float y, const float (&t2s)[N],
+                                     const float (&cs)[N], float& K, float& Kx, float& Ky) {
+  const float x2 = x * x, y2 = y * y, r = x2 + y2;
+  float sf = 0.0f, sp = 0.0f, sy = 0.0f;
+#pragma unroll
+  for (int k = N - 1; k >= 0; --k) {
+    const float t2 = t2s[k], c = cs[k];
+    const float w = r - t2;
+    const float d = 4.0f * t2 * y2;
+    const float rD = 1.0f / fmaf(w, w, d);
+    const float F = (w + 2.0f * t2) * rD;
+    const float Fp = fmaf(4.0f * t2, t2 - x2, -w * w) * rD * rD;
+    sf = fmaf(c, F, sf);
+    sp = fmaf(c, Fp, sp);
+    sy = fmaf(c, F - 2.0f * d * F * rD, sy);
   }
-  K = 2.0f * y * (float)kInvPi * sk;
-  L = 2.0f * x * (float)kInvPi * sl;
+  K = y * sf;
+  Kx = 2.0f * x * y * sp;
+  Ky = fmaf(2.0f * y2, sp, sy);
 }
 
-__device__ inline void W_cr32(float x, float y, float& K, float& L) {
+// Trapezoid + residue, |z| < 8.  The pole terms are node pairs with weight
+// h exp(-t^2) (plus (h/pi) y/r for the pole at 0 on the integer lattice);
+// the residue C = 2 exp(-z^2) sg p/(1 + sg p), p = exp(2 pi i z/h), has
+//   dC/dz = -2 z C + (2 pi i/h) C/(1 + sg p),
+// and dK/dx = Re dC/dz, dK/dy = -Im dC/dz for its share.
+__device__ inline void kgrad_cr32(float x, float y, float& K, float& Kx, float& Ky) {
   const float h = kCr32H;
   const float q = fabsf(x) * (1.0f / h);
   const float fr = q - floorf(q);
   const bool half = !(fr >= 0.25f && fr <= 0.75f);
-  const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
-  float sk = 0.0f, sl = 0.0f;
-  if (half) {
+  float t2s[5], cs[5];
 #pragma unroll
-    for (int n = 4; n >= 0; --n) {
-      const float t = (n + 0.5f) * h, t2 = t * t, u = xmy - t2;
-      const float e = kCr32E5[n] / fmaf(u, u, fxy);
-      sk = fmaf(e, r + t2, sk);
-      sl = fmaf(e, r - t2, sl);
-    }
-    sk *= 2.0f * y;
-    sl *= 2.0f * x;
-  } else {
-#pragma unroll
-    for (int n = 5; n >= 1; --n) {
-      const float t = n * h, t2 = t * t, u = xmy - t2;
-      const float e = kCr32E0[n - 1] / fmaf(u, u, fxy);
-      sk = fmaf(e, r + t2, sk);
-      sl = fmaf(e, r - t2, sl);
-    }
-    sk = 2.0f * y * sk + y / r;
-    sl = 2.0f * x * sl + x / r;
+  for (int n = 0; n < 5; ++n) {
+    const float t = half ? (n + 0.5f) * h : (n + 1) * h;
+    t2s[n] = t * t;
+    cs[n] = (float)(2.0 * kInvPi) * h * (half ? kCr32E5[n] : kCr32E0[n]);
   }
-  K = sk * h * (float)kInvPi;
-  L = sl * h * (float)kInvPi;
+  kgrad_pairs32<5>(x, y, t2s, cs, K, Kx, Ky);
+  const float x2 = x * x, y2 = y * y;
+  if (!half) {
+    const float c = h * (float)kInvPi;
+    const float rr = 1.0f / (x2 + y2);
+    K = fmaf(c * y, rr, K);
+    Kx = fmaf(-2.0f * c * x * y, rr * rr, Kx);
+    Ky = fmaf(c * (x2 - y2), rr * rr, Ky);
+  }
   if (y < (float)kPi / h) {
     const float sg = half ? 1.0f : -1.0f;
     const float a = 2.0f * (float)kPi / h;
@@ -277,21 +286,31 @@ __device__ inline void W_cr32(float x, float y, float& K, float& L) {
     sincosf(2.0f * x * y, &s2, &c2);
     const float cphi = ct * c2 + st * s2;   // cos(theta - 2xy)
     const float sphi = st * c2 - ct * s2;   // sin(theta - 2xy)
-    const float g = 2.0f * M / (1.0f + 2.0f * sg * rho * ct + rho * rho);
-    K += g * (sg * cphi + rho * c2);
-    L += g * (sg * sphi - rho * s2);
+    const float rden = 1.0f / (1.0f + 2.0f * sg * rho * ct + rho * rho);
+    const float g = 2.0f * M * rden;
+    const float cr = g * (sg * cphi + rho * c2);
+    const float ci = g * (sg * sphi - rho * s2);
+    // u = 1/(1 + sg p) = (1 + sg rho cos theta - i sg rho sin theta) rden
+    const float ur = (1.0f + sg * rho * ct) * rden, ui = -sg * rho * st * rden;
+    const float cur = cr * ur - ci * ui, cui = cr * ui + ci * ur;
+    K += cr;
+    Kx += -2.0f * (x * cr - y * ci) - a * cui;
+    Ky += 2.0f * (x * ci + y * cr) - a * cur;
   }
 }
 
-__device__ inline void W_mixed(float x, float y, float& K, float& L) {
+__device__ inline void kgrad_mixed(float x, float y, float& K, float& Kx, float& Ky) {
   const float r = x * x + y * y;
   if (r >= kF32Gh2R2) {
-    const double t[1] = {kGh2T}, w[1] = {kGh2W};
-    W_gh_pairs32(x, y, t, w, 1, K, L);
+    const float t2s[1] = {(float)(kGh2T * kGh2T)};
+    const float cs[1] = {(float)(2.0 * kGh2W * kInvPi)};
+    kgrad_pairs32<1>(x, y, t2s, cs, K, Kx, Ky);
   } else if (r >= kF32Gh4R2) {
-    W_gh_pairs32(x, y, kGh4T, kGh4W, 2, K, L);
+    const float t2s[2] = {(float)(kGh4T[0] * kGh4T[0]), (float)(kGh4T[1] * kGh4T[1])};
+    const float cs[2] = {(float)(2.0 * kGh4W[0] * kInvPi), (float)(2.0 * kGh4W[1] * kInvPi)};
+    kgrad_pairs32<2>(x, y, t2s, cs, K, Kx, Ky);
   } else {
-    W_cr32(x, y, K, L);
+    kgrad_cr32(x, y, K, Kx, Ky);
   }
 }
 

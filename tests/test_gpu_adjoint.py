@@ -15,7 +15,7 @@ from test_adjoint_oracle import translucent_case
 
 pytestmark = pytest.mark.gpu
 
-GRAD_RTOL = 2e-3      # FP32 adjoint vs FP64 references, relative to the gradient norm
+GRAD_RTOL = 1e-5      # FP32 adjoint vs FP64 references, relative to the gradient norm
 
 
 def say(*a):
@@ -52,9 +52,11 @@ def test_vjp_matches_fp64_analytic_adjoint():
     G = rng.normal(size=tuple(T.shape))
     gt, gv = prob.vjp(T, torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
     rt, rv = A.vjp(nu, lines, tips, mm, p, t, vmr, air, 25.0, 0.5, mus, G)
-    assert _rel(gt.cpu().numpy(), rt) < GRAD_RTOL
     ref_v = np.stack([rv[g] for g in prob.gases])
-    assert _rel(gv.cpu().numpy(), ref_v) < GRAD_RTOL
+    et, ev = _rel(gt.cpu().numpy(), rt), _rel(gv.cpu().numpy(), ref_v)
+    say(f"  rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
+    assert et < GRAD_RTOL
+    assert ev < GRAD_RTOL
 
 
 def test_vjp_matches_central_differences_of_fp64_forward():
@@ -90,8 +92,10 @@ def test_vjp_matches_central_differences_of_fp64_forward():
     prob.set_layers(t, V)
     T = prob.run("mixed", mus=mus)["transmittance"]
     gt, gv = prob.vjp(T, torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
-    assert _rel(gt.cpu().numpy(), fd_t) < GRAD_RTOL
-    assert _rel(gv.cpu().numpy(), fd_v) < GRAD_RTOL
+    et, ev = _rel(gt.cpu().numpy(), fd_t), _rel(gv.cpu().numpy(), fd_v)
+    say(f"  rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
+    assert et < GRAD_RTOL
+    assert ev < GRAD_RTOL
 
 
 def test_autograd_function_and_batched_profiles():
@@ -113,5 +117,8 @@ def test_autograd_function_and_batched_profiles():
         Gw = w.cpu().numpy()[None, :].astype(np.float64)
         rt, rv = A.vjp(nu, lines, tips, mm, p, tk, {g: vk[j] for j, g in enumerate(prob.gases)},
                        air, 25.0, 0.5, mus, Gw)
-        assert _rel(tt.grad.cpu().numpy(), rt) < GRAD_RTOL, i
-        assert _rel(vv.grad.cpu().numpy(), np.stack([rv[g] for g in prob.gases])) < GRAD_RTOL, i
+        et = _rel(tt.grad.cpu().numpy(), rt)
+        ev = _rel(vv.grad.cpu().numpy(), np.stack([rv[g] for g in prob.gases]))
+        say(f"  profile {i} rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
+        assert et < GRAD_RTOL, i
+        assert ev < GRAD_RTOL, i
