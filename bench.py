@@ -212,13 +212,15 @@ def adjoint_bench(args, dev, torch):
     G = torch.as_tensor(rng.normal(size=(1, prob.n_nu)), dtype=torch.float32, device=dev)
     grads = []
     prob.set_layers(T_p[0], V(0))
-    prob.vjp(prob.run("mixed", mus=mus)["transmittance"], G, mus)   # warm-up
+    fw = prob.run("mixed", mus=mus, want_layers=True)
+    prob.vjp(fw["transmittance"], fw["optical_depth_layers"], G, mus)   # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.adjoint):
         prob.set_layers(T_p[i], V(i))
-        T = prob.run("mixed", mus=mus)["transmittance"]
-        gt, gv = prob.vjp(T, G, mus)
+        fw = prob.run("mixed", mus=mus, want_layers=True)
+        T = fw["transmittance"]
+        gt, gv = prob.vjp(T, fw["optical_depth_layers"], G, mus)
         grads.append((gt, gv))
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.adjoint

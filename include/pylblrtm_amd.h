@@ -152,9 +152,10 @@ int lbl_voigt_eval(int64_t n, const void *x, const void *y, void *k_out, int pre
                    void *stream);
 
 /* Adjoint of the transmittance (no counterpart in the reference: the
- * gradients retrievals need).  Given the forward transmittance T_m(nu)
- * (float, n_mu x n_nu, from lbl_optical_depth with LBL_MIXED) and an upstream
- * gradient G_m(nu) = dL/dT_m(nu) (float, same shape), returns
+ * gradients retrievals need).  Given the forward transmittance T_m(nu) and
+ * per-layer optical depth od_k(nu) (float, n_mu x n_nu and n_layers x n_nu,
+ * both from one lbl_optical_depth call with LBL_MIXED on the same inputs) and
+ * an upstream gradient G_m(nu) = dL/dT_m(nu) (float, n_mu x n_nu), returns
  *   grad_t[k]          = dL/dT_k            (double, n_layers)
  *   grad_vmr[g][k]     = dL/dvmr_{k,g}      (double, n_gases x n_layers)
  * with the column of gas g in layer k taken as vmr_{k,g} * (air column of
@@ -165,12 +166,14 @@ int lbl_vjp_workspace_bytes(const lbl_lines_t *lines, const lbl_layers_t *layers
 int lbl_transmittance_vjp(const lbl_lines_t *lines, const lbl_groups_t *groups,
                           const lbl_layers_t *layers, const lbl_grid_t *grid, int32_t n_mu,
                           const double *mu, const float *transmittance,
-                          const float *grad_transmittance, double *grad_t, double *grad_vmr,
-                          void *workspace, size_t workspace_bytes, void *stream);
+                          const float *optical_depth_layers, const float *grad_transmittance,
+                          double *grad_t, double *grad_vmr, void *workspace,
+                          size_t workspace_bytes, void *stream);
 
 /* Learned Voigt surrogate (no counterpart in the reference; its Voigt is
  * scipy.special.wofz at forward.py:62).  MLP 2-64-64-64-64-1 (SiLU) on
- * u = x/50, v = (log10 y + 1)/3 predicting t = log(K / K_ref),
+ * u = 2 asinh(|x|)/asinh(50) - 1 (K is even in x), v = (log10 y + 1)/3
+ * predicting t = log(K / K_ref),
  * K_ref = exp(-x^2)/(1 + sqrt(pi) y) + y/(sqrt(pi)(x^2 + y^2 + 1/2)).
  * Parameters: FP32 vector of lbl_mlp_num_params() entries (layout in
  * pylblrtm_amd/csrc/surrogate.h); wbf16 = BF16 copies of the three hidden

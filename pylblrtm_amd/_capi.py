@@ -94,7 +94,7 @@ def load() -> ctypes.CDLL:
     lib.lbl_vjp_workspace_bytes.restype = ctypes.c_int
     lib.lbl_transmittance_vjp.argtypes = [P(LblLines), P(LblGroups), P(LblLayers), P(LblGrid),
                                           ctypes.c_int32, P(ctypes.c_double), _vp, _vp, _vp, _vp,
-                                          _vp, ctypes.c_size_t, _vp]
+                                          _vp, _vp, ctypes.c_size_t, _vp]
     lib.lbl_transmittance_vjp.restype = ctypes.c_int
     lib.lbl_mlp_num_params.argtypes = []
     lib.lbl_mlp_num_params.restype = ctypes.c_int

@@ -26,19 +26,20 @@ class LayerTransmittance(torch.autograd.Function):
     def forward(ctx, t_layers: torch.Tensor, vmr_layers: torch.Tensor,
                 problem: OpticalDepthProblem, mus: Sequence[float] = (1.0,)):
         problem.set_layers(t_layers.detach(), vmr_layers.detach())
-        out = problem.run("mixed", mus=tuple(mus))
+        out = problem.run("mixed", mus=tuple(mus), want_layers=True)
         T = out["transmittance"]
         ctx.problem = problem
         ctx.mus = tuple(mus)
-        ctx.save_for_backward(t_layers.detach().clone(), vmr_layers.detach().clone(), T)
+        ctx.save_for_backward(t_layers.detach().clone(), vmr_layers.detach().clone(), T,
+                              out["optical_depth_layers"])
         return T
 
     @staticmethod
     def backward(ctx, grad_T: torch.Tensor):
-        t_layers, vmr_layers, T = ctx.saved_tensors
+        t_layers, vmr_layers, T, od_layers = ctx.saved_tensors
         prob = ctx.problem
         prob.set_layers(t_layers, vmr_layers)
-        gt, gv = prob.vjp(T, grad_T, ctx.mus)
+        gt, gv = prob.vjp(T, od_layers, grad_T, ctx.mus)
         return (gt.to(t_layers.dtype), gv.to(vmr_layers.dtype), None, None)
 
 

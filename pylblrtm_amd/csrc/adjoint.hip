@@ -126,6 +126,7 @@ struct VjpArgs {
   double inv_mu[kMaxMu];
   const float* trans;          // [n_mu][n_nu] forward transmittance
   const float* grad_trans;     // [n_mu][n_nu] upstream gradient
+  const float* od_layers;      // [n_layers][n_nu] forward per-layer optical depth
   double* grad_t;              // [n_layers]
   double* grad_vmr;            // [n_gases][n_layers]
 };
@@ -159,90 +160,129 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// One workgroup per (tile slice, layer): P consecutive points per thread,
+// the layer's records of the tile staged through LDS in chunks.  Each chunk
+// is counting-sorted by gas (ballot ranks, so the order is deterministic) so
+// the per-gas gradient is summed in FP32 over runs of one gas and folded into
+// FP64 every kVjpFold records.
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
+  constexpr int kWaves = kBlock / 64;
   __shared__ VjpFar s_far[kVjpChunk];
   __shared__ VjpNear s_near[kVjpChunk];
-  const int64_t wg = blockIdx.x;
+  __shared__ int32_t s_wcnt[kWaves][8];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
+  const int k = (int)(lw / a.n_wg);
+  const int64_t wg = lw - (int64_t)k * a.n_wg;
   const int64_t tile = wg / a.slices_per_tile;
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
-  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)threadIdx.x * P;
+  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)tid * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
   const double nu_l = grid_nu(a.nu_min, a.step, i0);
   const float stepf = (float)a.step;
   const MixedRec* __restrict__ recs = a.rec.mix;
+  const int n_gas = a.n_gases;
+  const float T = (float)a.t_k[k];
+  const float ax_t = -0.5f / T;
 
-  float W[P];
-#pragma unroll
-  for (int j = 0; j < P; ++j) {
-    const int64_t i = i0 + j;
-    float w = 0.0f;
-    if (i < t1)
-      for (int m = 0; m < a.n_mu; ++m)
-        w -= a.grad_trans[(int64_t)m * a.n_nu + i] * a.trans[(int64_t)m * a.n_nu + i] *
-             (float)a.inv_mu[m];
-    W[j] = w;
-  }
-
-  for (int k = 0; k < a.n_layers; ++k) {
-    const float T = (float)a.t_k[k];
-    const float ax_t = -0.5f / T;
-    float WR[P], lay[P], part[P];
+  // W(nu) = -sum_m G_m T_m / mu_m and WR = W R(nu, T_k) at this thread's
+  // points; the radiation term's own derivative W (d ln R/dT) od_k(nu) needs
+  // only the forward's layer optical depth.
+  float WR[P];
+  double g_t;
+  {
+    float rad = 0.0f;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      const float nu = (float)grid_nu(a.nu_min, a.step, i0 + j);
-      WR[j] = W[j] * nu * tanhf(0.5f * (float)kC2 * nu / T);
-      lay[j] = part[j] = 0.0f;
-    }
-    double g_t = 0.0;
-    double g_v[8];
-#pragma unroll
-    for (int g = 0; g < 8; ++g) g_v[g] = 0.0;
-    const int64_t kb = (int64_t)k * a.n_lines;
-    for (int32_t base = rlo; base < rhi; base += kVjpChunk) {
-      const int cnt = min(kVjpChunk, rhi - base);
-      __syncthreads();
-      if ((int)threadIdx.x < cnt) {
-        const int64_t r = kb + base + threadIdx.x;
-        const MixedRec& mr = recs[r];
-        double c = mr.c;
-        float s = mr.s, y = mr.y, am = mr.a;
-        float at = a.at[r], dyt = a.dyt[r], av = a.av[r], bxv = a.bxv[r], dyv = a.dyv[r];
-        if (!(c >= lo_b && c <= hi_b)) {
-          // outside this tile's window: a record that adds exactly zero
-          c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
-          at = dyt = av = bxv = dyv = 0.0f;
-        }
-        const float y2 = y * y;
-        const float d2 = 2.0f * y2;
-        const float ay = am * y * (float)kInvSqrtPi;
-        const float aq = am * (float)kInvSqrtPi;
-        const float cty = dyt * aq, cvy = dyv * aq;
-        VjpFar f;
-        f.c = c; f.s = s; f.hs = stepf * s;
-        f.cw = y2 - 0.5f; f.d2 = d2;
-        f.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
-        f.tF = at * ay + cty; f.tX2 = 2.0f * ax_t * ay; f.tD = cty * d2; f.tQ = -2.0f * d2 * cty;
-        f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
-        s_far[threadIdx.x] = f;
-        VjpNear n;
-        n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
-        n.gas = a.gas_of_group[a.group[base + threadIdx.x]];
-        s_near[threadIdx.x] = n;
+      const int64_t i = i0 + j;
+      float w = 0.0f, od = 0.0f;
+      if (i < t1) {
+        for (int m = 0; m < a.n_mu; ++m)
+          w -= a.grad_trans[(int64_t)m * a.n_nu + i] * a.trans[(int64_t)m * a.n_nu + i] *
+               (float)a.inv_mu[m];
+        od = a.od_layers[(int64_t)k * a.n_nu + i];
       }
-      __syncthreads();
-      for (int qi = 0; qi < cnt; ++qi) {
+      const float nu = (float)grid_nu(a.nu_min, a.step, i);
+      const float u = (float)kC2 * nu / T;
+      WR[j] = w * nu * tanhf(0.5f * u);
+      rad = fmaf(w * od, -(u / T) / sinhf(u), rad);
+    }
+    g_t = (double)rad;
+  }
+  double g_v[8];
+#pragma unroll
+  for (int g = 0; g < 8; ++g) g_v[g] = 0.0;
+
+  const int64_t kb = (int64_t)k * a.n_lines;
+  for (int32_t base = rlo; base < rhi; base += kVjpChunk) {
+    const int cnt = min(kVjpChunk, rhi - base);
+    __syncthreads();   // previous chunk consumed (s_far, s_near, s_wcnt)
+    VjpFar f;
+    VjpNear n;
+    int gas = -1;
+    if (tid < cnt) {
+      const int64_t r = kb + base + tid;
+      const MixedRec& mr = recs[r];
+      double c = mr.c;
+      float s = mr.s, y = mr.y, am = mr.a;
+      float at = a.at[r], dyt = a.dyt[r], av = a.av[r], bxv = a.bxv[r], dyv = a.dyv[r];
+      if (!(c >= lo_b && c <= hi_b)) {
+        // outside this tile's window: a record that adds exactly zero
+        c = 0.0; s = 0.0f; y = 1.0e3f; am = 0.0f;
+        at = dyt = av = bxv = dyv = 0.0f;
+      }
+      const float y2 = y * y;
+      const float d2 = 2.0f * y2;
+      const float ay = am * y * (float)kInvSqrtPi;
+      const float aq = am * (float)kInvSqrtPi;
+      const float cty = dyt * aq, cvy = dyv * aq;
+      f.c = c; f.s = s; f.hs = stepf * s;
+      f.cw = y2 - 0.5f; f.d2 = d2;
+      f.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
+      f.tF = at * ay + cty; f.tX2 = 2.0f * ax_t * ay; f.tD = cty * d2; f.tQ = -2.0f * d2 * cty;
+      f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
+      n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
+      gas = a.gas_of_group[a.group[base + tid]];
+      if (gas >= n_gas) gas = -1;   // cannot happen for validated inputs
+      n.gas = gas;
+    }
+    // Counting sort by gas: rank inside the wave, then per-wave counts.
+    int rank = 0;
+    for (int g = 0; g < n_gas; ++g) {
+      const unsigned long long m = __ballot(gas == g);
+      if (gas == g) rank = __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) s_wcnt[wv][g] = __popcll(m);
+    }
+    __syncthreads();
+    if (gas >= 0) {
+      int off = rank;
+      for (int g = 0; g < gas; ++g)
+        for (int w = 0; w < kWaves; ++w) off += s_wcnt[w][g];
+      for (int w = 0; w < wv; ++w) off += s_wcnt[w][gas];
+      s_far[off] = f;
+      s_near[off] = n;
+    }
+    __syncthreads();
+    int q0 = 0;
+    for (int g = 0; g < n_gas; ++g) {
+      int len = 0;
+      for (int w = 0; w < kWaves; ++w) len += s_wcnt[w][g];
+      const int q1 = q0 + len;
+      float ft = 0.0f, fv = 0.0f;
+      double dv = 0.0;
+      for (int qi = q0; qi < q1; ++qi) {
         const VjpFar& R = s_far[qi];
         const float x0 = (float)(nu_l - R.c) * R.s;
         const float hs = R.hs;
         const float xb = fmaf((float)(P - 1), hs, x0);
-        float acc_t = 0.0f, acc_v = 0.0f;
+        float acc_t, acc_v;
         if (x0 >= R.hw || xb <= -R.hw) {
-          const float cw = R.cw, d2 = R.d2, ay = R.ay;
+          const float cw = R.cw, d2 = R.d2;
           float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f, S5 = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
@@ -251,9 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
             const float D = fmaf(w, w, d2);
             const float rD = __builtin_amdgcn_rcpf(D);
             const float w1 = w + 1.0f;
-            const float F = w1 * rD;
-            part[j] = fmaf(ay, F, part[j]);
-            S1 = fmaf(WR[j], F, S1);
+            S1 = fmaf(WR[j], w1 * rD, S1);
             const float q = (WR[j] * rD) * rD;
             const float qe = q * fmaf(-w, w + 2.0f, d2);
             const float xqe = x * qe;
@@ -266,6 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
           acc_v = fmaf(R.vF, S1, fmaf(R.vX, S3, fmaf(R.vD, S2, R.vQ * S5)));
         } else {
           const VjpNear& N = s_near[qi];
+          acc_t = acc_v = 0.0f;
 #pragma unroll 1
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, hs, x0);
@@ -276,41 +315,32 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
             Ky *= N.a;
             float wr = 0.0f;
 #pragma unroll
-            for (int jj = 0; jj < P; ++jj) {
-              wr = (jj == j) ? WR[jj] : wr;
-              part[jj] += (jj == j) ? AK : 0.0f;
-            }
+            for (int jj = 0; jj < P; ++jj) wr = (jj == j) ? WR[jj] : wr;
             acc_t = fmaf(wr, N.at * AK + Kx * (ax_t * x) + Ky * N.dyt, acc_t);
             acc_v = fmaf(wr, N.av * AK + Kx * N.bxv + Ky * N.dyv, acc_v);
           }
         }
-        g_t += (double)acc_t;
-        const int gas = __builtin_amdgcn_readfirstlane(s_near[qi].gas);
-#pragma unroll
-        for (int g = 0; g < 8; ++g)
-          if (g == gas) g_v[g] += (double)acc_v;
-        if ((qi & (kVjpFold - 1)) == kVjpFold - 1) {
-#pragma unroll
-          for (int j = 0; j < P; ++j) { lay[j] += part[j]; part[j] = 0.0f; }
+        ft += acc_t;
+        fv += acc_v;
+        if (((qi - q0) & (kVjpFold - 1)) == kVjpFold - 1) {
+          g_t += (double)ft;
+          dv += (double)fv;
+          ft = fv = 0.0f;
         }
       }
+      g_t += (double)ft;
+      dv += (double)fv;
 #pragma unroll
-      for (int j = 0; j < P; ++j) { lay[j] += part[j]; part[j] = 0.0f; }
+      for (int gg = 0; gg < 8; ++gg)
+        if (gg == g) g_v[gg] += dv;
+      q0 = q1;
     }
-    // radiation-term derivative at the grid points
-#pragma unroll
-    for (int j = 0; j < P; ++j) {
-      const float nu = (float)grid_nu(a.nu_min, a.step, i0 + j);
-      const float u = (float)kC2 * nu / T;
-      const float dlnr = -(u / T) / sinhf(u);
-      g_t += (double)(WR[j] * dlnr * lay[j]);
-    }
-    g_t = wave_sum(g_t);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&a.grad_t[k], g_t);
-    for (int g = 0; g < a.n_gases && g < 8; ++g) {
-      const double sv = wave_sum(g_v[g]);
-      if ((threadIdx.x & 63) == 0) atomicAdd(&a.grad_vmr[(int64_t)g * a.n_layers + k], sv);
-    }
+  }
+  g_t = wave_sum(g_t);
+  if (lane == 0) atomicAdd(&a.grad_t[k], g_t);
+  for (int g = 0; g < n_gas && g < 8; ++g) {
+    const double sv = wave_sum(g_v[g]);
+    if (lane == 0) atomicAdd(&a.grad_vmr[(int64_t)g * a.n_layers + k], sv);
   }
 }
 
@@ -338,8 +368,8 @@ hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, con
                       const int32_t* gas_of_group, const int32_t* lo, const int32_t* hi,
                       const double* t_k, int64_t n_lines, int n_layers, int n_gases,
                       const lbl_grid_t& grid, int64_t n_tiles, int n_mu, const double* mu,
-                      const float* trans, const float* grad_trans, double* grad_t,
-                      double* grad_vmr, hipStream_t s) {
+                      const float* trans, const float* grad_trans, const float* od_layers,
+                      double* grad_t, double* grad_vmr, hipStream_t s) {
   const int64_t nt = (int64_t)n_layers * (1 + n_gases);
   if (n_layers > 0) {
     k_fill_f64<<<(unsigned)((n_layers + 255) / 256), 256, 0, s>>>(grad_t, n_layers);
@@ -373,9 +403,13 @@ hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, con
   for (int m = 0; m < n_mu; ++m) a.inv_mu[m] = 1.0 / mu[m];
   a.trans = trans;
   a.grad_trans = grad_trans;
+  a.od_layers = od_layers;
   a.grad_t = grad_t;
   a.grad_vmr = grad_vmr;
-  k_vjp_mixed<kVjpPoints><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+  // one workgroup per (slice, layer); a 1-D grid holds at most 2^32 work-items
+  const int64_t n_blocks = a.n_wg * (int64_t)n_layers;
+  if (n_blocks * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
+  k_vjp_mixed<kVjpPoints><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 

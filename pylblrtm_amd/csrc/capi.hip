@@ -282,13 +282,14 @@ int lbl_vjp_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers
 int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
                           const lbl_layers_t* layers, const lbl_grid_t* grid, int32_t n_mu,
                           const double* mu, const float* transmittance,
-                          const float* grad_transmittance, double* grad_t, double* grad_vmr,
-                          void* workspace, size_t workspace_bytes, void* stream) {
+                          const float* optical_depth_layers, const float* grad_transmittance,
+                          double* grad_t, double* grad_vmr, void* workspace,
+                          size_t workspace_bytes, void* stream) {
   g_last_error[0] = '\0';
   int rc = check_inputs(lines, groups, layers, grid, LBL_MIXED);
   if (rc != LBL_OK) return rc;
-  if (n_mu < 1 || n_mu > lbl::kMaxMu || !mu || !transmittance || !grad_transmittance ||
-      !grad_t || !grad_vmr)
+  if (n_mu < 1 || n_mu > lbl::kMaxMu || !mu || !transmittance || !optical_depth_layers ||
+      !grad_transmittance || !grad_t || !grad_vmr)
     return fail(LBL_EINVAL, "vjp: need 1..%d viewing cosines and all gradient buffers",
                 lbl::kMaxMu);
   for (int m = 0; m < n_mu; ++m)
@@ -326,8 +327,8 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   HIP_TRY(lbl::launch_line_grad(*lines, *groups, *layers, gc[0], gc[1], gc[2], gc[3], gc[4], s));
   HIP_TRY(lbl::launch_vjp(rec, gc[0], gc[1], gc[2], gc[3], gc[4], lines->group, groups->gas, lo,
                           hi, layers->t_k, lines->n_lines, layers->n_layers, layers->n_gases,
-                          *grid, L.n_tiles, n_mu, mu, transmittance, grad_transmittance, grad_t,
-                          grad_vmr, s));
+                          *grid, L.n_tiles, n_mu, mu, transmittance, grad_transmittance,
+                          optical_depth_layers, grad_t, grad_vmr, s));
   uint32_t code = lbl::kErrNone;
   HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));

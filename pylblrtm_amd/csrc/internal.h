@@ -25,6 +25,15 @@ constexpr int kBlock = 256;        // threads per workgroup of the line sum
 constexpr uint32_t kErrNone = 0xFFFFFFFFu;
 constexpr int kErrGroupBits = 11;
 
+// Hardware assigns workgroup b to XCD b % 8.  Give every XCD a contiguous
+// run of logical workgroups so neighbouring tiles (which share most of their
+// line records) share an L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
+  const int64_t per = n >> 3, rem = n & 7;
+  const int64_t xcd = b & 7, idx = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
 // Grid point i exactly as numpy builds it: nu_min + (double(i) * step),
 // rounded after the multiply and after the add (no fused multiply-add).
 __device__ __forceinline__ double grid_nu(double nu_min, double step, int64_t i) {
@@ -114,8 +123,8 @@ hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, con
                       const int32_t* gas_of_group, const int32_t* lo, const int32_t* hi,
                       const double* t_k, int64_t n_lines, int n_layers, int n_gases,
                       const lbl_grid_t& grid, int64_t n_tiles, int n_mu, const double* mu,
-                      const float* trans, const float* grad_trans, double* grad_t,
-                      double* grad_vmr, hipStream_t s);
+                      const float* trans, const float* grad_trans, const float* od_layers,
+                      double* grad_t, double* grad_vmr, hipStream_t s);
 int vjp_points_per_thread();
 
 int mlp_num_params();

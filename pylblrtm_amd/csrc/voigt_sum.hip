@@ -40,15 +40,6 @@ constexpr int kChunk = kBlock;  // records staged per LDS round
 constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
 constexpr int kQueueCap = 8192; // near-centre (record, thread) pairs per chunk (16-bit entries)
 
-// Hardware assigns workgroup b to XCD b % 8.  Give every XCD a contiguous
-// run of logical workgroups so neighbouring tiles (which share most of their
-// line records) share an L2.
-__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
-  const int64_t per = n >> 3, rem = n & 7;
-  const int64_t xcd = b & 7, idx = b >> 3;
-  return xcd * per + (xcd < rem ? xcd : rem) + idx;
-}
-
 __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
   const float y = v - comp;
   const float t = sum + y;

@@ -212,15 +212,20 @@ class OpticalDepthProblem:
         return (self._t["t_k"][: self.n_layers].clone(),
                 self._t["vmr"][: len(self.gases), : self.n_layers].clone())
 
-    def vjp(self, transmittance, grad_transmittance, mus: Sequence[float]):
+    def vjp(self, transmittance, optical_depth_layers, grad_transmittance, mus: Sequence[float]):
         """dL/dT_k [n_layers] and dL/dvmr [n_gases, n_layers] (float64 device
-        tensors) for the mixed-precision forward ``transmittance`` [n_mu, n_nu]
-        and upstream gradient ``grad_transmittance`` of the same shape."""
+        tensors) for a mixed-precision forward (``transmittance`` [n_mu, n_nu]
+        and ``optical_depth_layers`` [n_layers, n_nu] of one
+        ``run("mixed", want_layers=True)``) and the upstream gradient
+        ``grad_transmittance`` [n_mu, n_nu]."""
         torch = _torch()
         T = transmittance.to(torch.float32).contiguous()
+        OD = optical_depth_layers.to(torch.float32).contiguous()
         G = grad_transmittance.to(device=self.device, dtype=torch.float32).contiguous()
         if tuple(T.shape) != (len(mus), self.n_nu) or tuple(G.shape) != tuple(T.shape):
             raise ValueError("transmittance / gradient must be [n_mu, n_nu]")
+        if tuple(OD.shape) != (self.n_layers, self.n_nu):
+            raise ValueError("optical_depth_layers must be [n_layers, n_nu]")
         if "vjp" not in self._ws:
             nbytes = ctypes.c_size_t(0)
             st = self.lib.lbl_vjp_workspace_bytes(ctypes.byref(self.c_lines),
@@ -238,8 +243,8 @@ class OpticalDepthProblem:
         with torch.cuda.device(self.device):
             st = self.lib.lbl_transmittance_vjp(
                 ctypes.byref(self.c_lines), ctypes.byref(self.c_groups), ctypes.byref(self.c_layers),
-                ctypes.byref(self.c_grid), len(mus), mu_arr, T.data_ptr(), G.data_ptr(),
-                gt.data_ptr(), gv.data_ptr(), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                ctypes.byref(self.c_grid), len(mus), mu_arr, T.data_ptr(), OD.data_ptr(),
+                G.data_ptr(), gt.data_ptr(), gv.data_ptr(), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
                 ctypes.c_void_p(stream))
         _capi.raise_for(st, "lbl_transmittance_vjp")
         return gt[: self.n_layers], gv[: len(self.gases), : self.n_layers]

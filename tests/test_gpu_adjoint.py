@@ -47,10 +47,12 @@ def test_vjp_matches_fp64_analytic_adjoint():
     prob, (nu, lines, tips, mm, p, t, vmr, air) = _problem()
     mus = (1.0, 0.5)
     np.testing.assert_allclose(prob.air_cm2, air, rtol=1e-14)
-    T = prob.run("mixed", mus=mus)["transmittance"]
+    fw = prob.run("mixed", mus=mus, want_layers=True)
+    T = fw["transmittance"]
     rng = np.random.default_rng(3)
     G = rng.normal(size=tuple(T.shape))
-    gt, gv = prob.vjp(T, torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
+    gt, gv = prob.vjp(T, fw["optical_depth_layers"],
+                      torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
     rt, rv = A.vjp(nu, lines, tips, mm, p, t, vmr, air, 25.0, 0.5, mus, G)
     ref_v = np.stack([rv[g] for g in prob.gases])
     et, ev = _rel(gt.cpu().numpy(), rt), _rel(gv.cpu().numpy(), ref_v)
@@ -90,8 +92,10 @@ def test_vjp_matches_central_differences_of_fp64_forward():
             vm[g, k] -= h
             fd_v[g, k] = (loss(t, vp) - loss(t, vm)) / (2 * h)
     prob.set_layers(t, V)
-    T = prob.run("mixed", mus=mus)["transmittance"]
-    gt, gv = prob.vjp(T, torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
+    fw = prob.run("mixed", mus=mus, want_layers=True)
+    T = fw["transmittance"]
+    gt, gv = prob.vjp(T, fw["optical_depth_layers"],
+                      torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
     et, ev = _rel(gt.cpu().numpy(), fd_t), _rel(gv.cpu().numpy(), fd_v)
     say(f"  rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
     assert et < GRAD_RTOL
