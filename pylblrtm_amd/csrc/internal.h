@@ -40,17 +40,25 @@ __device__ __forceinline__ double grid_nu(double nu_min, double step, int64_t i)
   return __dadd_rn(nu_min, __dmul_rn((double)i, step));
 }
 
+// Grid points per thread of the mixed-precision line sums.
+constexpr int kPointsMixed = 8;
+
 // Per-(layer, line) record of the mixed-precision paths (48 bytes, one
-// 16-byte-aligned struct so the line sum stages it with three 16-byte loads):
-//   c  shifted centre (FP64), s = sqrt(ln2)/alpha_D, hs = step * s,
-//   cw = y^2 - 1/2, d2 = 2 y^2, ay = A y / sqrt(pi)   (two-node far form),
-//   hw = sqrt(max(25^2 - y^2, 0))  (|x| beyond which the far form holds),
+// 16-byte-aligned struct so the line sum stages it with three 16-byte loads).
+// The first 16 bytes are what the line sum's per-record test reads, the next
+// 16 what its far-field branch reads:
+//   c  shifted centre (FP64), s = sqrt(ln2)/alpha_D,
+//   hw = sqrt(max(25^2 - y^2, 0)) + (kPointsMixed - 1)/2 * hs: a thread's
+//        kPointsMixed points all lie where the far form holds (|z| >= 25)
+//        iff |x| at the middle of the run is >= hw,
+//   hs = step * s, cw = y^2 - 1/2, d2 = 2 y^2, ay = A y / sqrt(pi)
+//        (two-node far form),
 //   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D.
 // A skipped (zero-column) record has c = NaN, ay = a = 0, d2 = 1.
 struct alignas(16) MixedRec {
   double c;
-  float s, hs;
-  float cw, d2, ay, hw;
+  float s, hw;
+  float hs, cw, d2, ay;
   float y, a, pad0, pad1;
 };
 static_assert(sizeof(MixedRec) == 48, "record layout");

@@ -66,7 +66,7 @@ __device__ __forceinline__ void store_mixed(const Records& r, int64_t i, double 
   m.cw = y2 - 0.5f;
   m.d2 = 2.0f * y2;
   m.ay = a * y * (float)kInvSqrtPi;
-  m.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
+  m.hw = fmaf(0.5f * (float)(kPointsMixed - 1), m.hs, sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)));
   m.y = y;
   m.a = a;
   m.pad0 = m.pad1 = 0.0f;

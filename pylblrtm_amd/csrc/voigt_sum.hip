@@ -40,6 +40,10 @@ constexpr int kChunk = kBlock;  // records staged per LDS round
 constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
 constexpr int kQueueCap = 8192; // near-centre (record, thread) pairs per chunk (16-bit entries)
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+typedef __attribute__((address_space(3))) MixedRec lds_mixed_rec;
+
 __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
   const float y = v - comp;
   const float t = sum + y;
@@ -217,9 +221,13 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
-  const double nu_l = grid_nu(a.nu_min, a.step, i0);
+  static_assert(P == kPointsMixed, "records carry hw for kPointsMixed points per thread");
+  // middle of this thread's run of P points; point j sits at x_mid + (j - (P-1)/2) hs
+  const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
   const bool live = i0 < t1;   // lanes wholly past the tile end never write
   const MixedRec* __restrict__ recs = a.rec.mix;
+  // s_rec as an LDS-qualified pointer, so record reads stay ds_read_b128
+  lds_mixed_rec* const s_rec_lds = (lds_mixed_rec*)s_rec;
 
   for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0f;
   if (threadIdx.x == 0) s_qn = 0;
@@ -249,19 +257,26 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         }
       }
       __syncthreads();
-      for (int q = 0; q < cnt; ++q) {
-        const MixedRec& R = s_rec[q];
-        const double c = R.c;
-        const float s = R.s, hs = R.hs;
-        const float x0 = (float)(nu_l - c) * s;
-        const float xb = fmaf((float)(P - 1), hs, x0);
-        const float hw = R.hw;
-        if (x0 >= hw || xb <= -hw) {
+      // Per record: c, s, hw in one 16-byte LDS read (prefetched one record
+      // ahead, so its latency hides behind the current record), and hs, cw,
+      // d2, ay in the next 16 bytes, read only on the far branch; volatile
+      // keeps each a single ds_read_b128.
+      const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
+      const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
+      u32x4 hn = rb[0];
+      for (int q = 0; q < ncnt; ++q) {
+        const u32x4 h0 = hn;
+        if (q + 1 < ncnt) hn = rb[3 * (q + 1)];
+        const double c = __hiloint2double((int)h0.y, (int)h0.x);
+        const float xm = (float)(nu_m - c) * __uint_as_float(h0.z);
+        if (fabsf(xm) >= __uint_as_float(h0.w)) {
           // two-node form: K = (y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
-          const float cw = R.cw, d2 = R.d2, ay = R.ay;
+          const u32x4 h1 = rb[3 * q + 1];
+          const float hs = __uint_as_float(h1.x), cw = __uint_as_float(h1.y);
+          const float d2 = __uint_as_float(h1.z), ay = __uint_as_float(h1.w);
 #pragma unroll
           for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j, hs, x0);
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
             const float w = fmaf(x, x, cw);
             const float D = fmaf(w, w, d2);
             const float N = fmaf(ay, w, ay);
@@ -279,10 +294,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           if (pos < kQueueCap) {
             s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
           } else {  // queue full: evaluate here
-            const float y = R.y, am = R.a;
+            const float y = s_rec[q].y, am = s_rec[q].a, hs = s_rec[q].hs;
 #pragma unroll 1
             for (int j = 0; j < P; ++j) {
-              const float v = am * K_mixed(fmaf((float)j, hs, x0), y);
+              const float v = am * K_mixed(fmaf((float)j - 0.5f * (float)(P - 1), hs, xm), y);
 #pragma unroll
               for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
             }
@@ -420,7 +435,6 @@ __global__ void k_voigt_eval32(int64_t n, const float* x, const float* y, float*
 }
 
 constexpr int kPointsFp64 = 4;
-constexpr int kPointsMixed = 8;
 
 }  // namespace
 
