@@ -179,6 +179,9 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  // a slice wholly past the end of a short (last) tile has no points: the
+  // whole workgroup leaves before its first barrier
+  if (t0 + (int64_t)slice * (kBlock * P) >= t1) return;
   const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)tid * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
