@@ -126,3 +126,107 @@ def test_autograd_function_and_batched_profiles():
         say(f"  profile {i} rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
         assert et < GRAD_RTOL, i
         assert ev < GRAD_RTOL, i
+
+
+# Levels whose layer means are the translucent case's layers: temperatures
+# 220.3, 235.7, 258.4, 281.1 K (a 0.1 K level step moves a layer by 0.05 K,
+# inside its 1 K TIPS segment) and H2O 5e-6, 2e-4, 2e-3, 8e-3.
+T_LEVELS = np.array([215.0, 225.6, 245.8, 271.0, 291.2])
+H2O_LEVELS = np.array([2e-6, 8e-6, 3.92e-4, 3.608e-3, 1.2392e-2])
+
+
+def _level_state(prob):
+    rows = {"CO2": np.full(5, 4.15e-4), "H2O": H2O_LEVELS}
+    return T_LEVELS.copy(), np.stack([rows[g] for g in prob.gases])
+
+
+def test_level_gradients_chain_rule_and_central_differences():
+    say("case: level gradients through the device layer means")
+    import torch
+    from pylblrtm_amd.autograd import level_layers, level_transmittance
+    prob, _ = _problem()
+    mus = (1.0,)
+    tl, vl = _level_state(prob)
+    dev = prob.device
+    tt = torch.tensor(tl, dtype=torch.float64, device=dev, requires_grad=True)
+    vv = torch.tensor(vl, dtype=torch.float64, device=dev, requires_grad=True)
+    T = level_transmittance(prob, tt, vv, mus)
+    rng = np.random.default_rng(11)
+    G = rng.normal(size=tuple(T.shape))
+    Gt = torch.as_tensor(G, dtype=torch.float32, device=dev)
+    (T * Gt).sum().backward()
+    g_tl, g_vl = tt.grad.cpu().numpy(), vv.grad.cpu().numpy()
+
+    # closed form: level i feeds layers i-1 and i with weight 1/2 each
+    t_lay, v_lay = level_layers(torch.as_tensor(tl), torch.as_tensor(vl))
+    prob.set_layers(t_lay.numpy(), v_lay.numpy())
+    fw = prob.run("mixed", mus=mus, want_layers=True)
+    gt, gv = prob.vjp(fw["transmittance"], fw["optical_depth_layers"], Gt, mus)
+    gt, gv = gt.cpu().numpy(), gv.cpu().numpy()
+    ct = np.zeros(5)
+    cv = np.zeros_like(vl)
+    ct[:-1] += 0.5 * gt
+    ct[1:] += 0.5 * gt
+    cv[:, :-1] += 0.5 * gv
+    cv[:, 1:] += 0.5 * gv
+    # (two separate forward + adjoint runs: equal up to the rounding of the
+    # atomic reductions)
+    assert _rel(g_tl, ct) < 1e-6
+    assert _rel(g_vl, cv) < 1e-6
+
+    # central differences of the FP64 forward through the level means
+    def loss(t_lev, v_lev):
+        a, b = level_layers(torch.as_tensor(t_lev), torch.as_tensor(v_lev))
+        prob.set_layers(a.numpy(), b.numpy())
+        return prob.run("fp64", mus=mus)["transmittance"]
+
+    fd_t, ad_t = [], []
+    for i in range(5):
+        tp, tm = tl.copy(), tl.copy()
+        tp[i] += 0.1
+        tm[i] -= 0.1
+        fd_t.append(float(((loss(tp, vl) - loss(tm, vl)) * Gt.double()).sum()) / 0.2)
+        ad_t.append(g_tl[i])
+    gi = prob.gases.index("H2O")
+    fd_v, ad_v = [], []
+    for i in range(5):
+        h = 1e-3 * vl[gi, i]
+        vp, vm = vl.copy(), vl.copy()
+        vp[gi, i] += h
+        vm[gi, i] -= h
+        fd_v.append(float(((loss(tl, vp) - loss(tl, vm)) * Gt.double()).sum()) / (2 * h))
+        ad_v.append(g_vl[gi, i])
+    et, ev = _rel(ad_t, fd_t), _rel(ad_v, fd_v)
+    say(f"  level gradients vs central differences: temperature {et:.2e}, H2O {ev:.2e}")
+    assert et < GRAD_RTOL
+    assert ev < GRAD_RTOL
+
+
+def test_batched_profiles_match_one_by_one():
+    say("case: batch of 3 profiles vs one profile at a time")
+    import torch
+    from pylblrtm_amd.autograd import batch_layer_transmittance, layer_transmittance
+    prob, (nu, lines, tips, mm, p, t, vmr, air) = _problem()
+    mus = (1.0, 0.5)
+    V0 = _vmr_matrix(prob, vmr)
+    rng = np.random.default_rng(21)
+    tb = np.stack([t + rng.normal(0.0, 2.0, 4) for _ in range(3)])
+    vb = np.stack([V0 * rng.lognormal(0.0, 0.1, V0.shape) for _ in range(3)])
+    dev = prob.device
+    G = torch.as_tensor(rng.normal(size=(3, len(mus), prob.n_nu)), dtype=torch.float32, device=dev)
+    tt = torch.tensor(tb, dtype=torch.float64, device=dev, requires_grad=True)
+    vv = torch.tensor(vb, dtype=torch.float64, device=dev, requires_grad=True)
+    T = batch_layer_transmittance(prob, tt, vv, mus)
+    (T * G).sum().backward()
+    for b in range(3):
+        t1 = torch.tensor(tb[b], dtype=torch.float64, device=dev, requires_grad=True)
+        v1 = torch.tensor(vb[b], dtype=torch.float64, device=dev, requires_grad=True)
+        T1 = layer_transmittance(prob, t1, v1, mus)
+        (T1 * G[b]).sum().backward()
+        # same kernels on the same inputs; the near-centre drain (LDS float
+        # atomics) and the gradient reduction (FP64 atomics) add in hardware
+        # order, so agreement is to rounding, not bitwise
+        np.testing.assert_allclose(T[b].detach().cpu().numpy(), T1.detach().cpu().numpy(),
+                                   rtol=1e-6, atol=1e-7)
+        assert _rel(tt.grad[b].cpu().numpy(), t1.grad.cpu().numpy()) < 1e-6, b
+        assert _rel(vv.grad[b].cpu().numpy(), v1.grad.cpu().numpy()) < 1e-6, b
