@@ -199,6 +199,20 @@ int lbl_mlp_train_step(float *params, float *adam_m, float *adam_v, unsigned sho
 int lbl_mlp_eval(int64_t n, const float *x, const float *log10y, const float *params,
                  const unsigned short *wbf16, float *t_out, void *stream);
 
+/* HITRAN-2004 .par line-list reader (host code; replaces the line loop of
+ * par_to_dict, pylblrtm/hitran_manager.py:816-872).  Text lines are split as
+ * Python's universal-newline mode splits them (\n, \r\n, \r).  Call once
+ * with capacity 0 to get *n_text_lines, then with arrays of that length:
+ * status[i] = -1 for a line shorter than 155 characters (skipped), -2 for a
+ * line holding non-ASCII bytes (left to the caller, which decodes UTF-8 with
+ * replacement as Python does), else the number 0..9 of leading fields
+ * mol_id, iso_id, nu0, Sref, g_air, g_self, Elow, n_air, shift that convert
+ * under Python's int()/float() rules (the reference appends field by field,
+ * so a malformed line keeps the fields before the bad one); ids[2i..2i+1]
+ * and values[7i..7i+6] hold them.  LBL_EINVAL if the file cannot be read. */
+int lbl_par_read(const char *path, int64_t capacity, int64_t *n_text_lines, int8_t *status,
+                 int32_t *ids, double *values);
+
 /* Kernel timing (no counterpart in the reference; for benchmarks).  When
  * enabled, lbl_optical_depth records HIP events on its stream around the
  * per-(layer, line) kernel and the line-sum kernel; lbl_last_timings returns

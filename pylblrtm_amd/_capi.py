@@ -21,7 +21,7 @@ EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_l
             "lbl_enable_timing", "lbl_last_timings", "lbl_vjp_workspace_bytes",
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
             "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
-            "lbl_optical_depth_surrogate")
+            "lbl_optical_depth_surrogate", "lbl_par_read")
 
 _vp = ctypes.c_void_p
 
@@ -115,6 +115,9 @@ def load() -> ctypes.CDLL:
     lib.lbl_enable_timing.restype = None
     lib.lbl_last_timings.argtypes = [P(ctypes.c_double), P(ctypes.c_double)]
     lib.lbl_last_timings.restype = ctypes.c_int
+    lib.lbl_par_read.argtypes = [ctypes.c_char_p, ctypes.c_int64, P(ctypes.c_int64), _vp, _vp,
+                                 _vp]
+    lib.lbl_par_read.restype = ctypes.c_int
     lib.lbl_last_error.argtypes = []
     lib.lbl_last_error.restype = ctypes.c_char_p
     _lib = lib

@@ -18,7 +18,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libpylblrtm_amd.so"
-SOURCES = ("line_params.hip", "voigt_sum.hip", "adjoint.hip", "surrogate.hip", "capi.hip")
+SOURCES = ("line_params.hip", "voigt_sum.hip", "adjoint.hip", "surrogate.hip", "capi.hip",
+           "par_reader.cpp")
 ARCH = os.environ.get("PYLBLRTM_AMD_ARCH", "gfx950")
 
 

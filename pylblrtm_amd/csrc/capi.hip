@@ -44,6 +44,22 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
+}  // namespace
+
+namespace lbl {
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+}  // namespace lbl
+
+namespace {
+
 #define HIP_TRY(expr)                                                              \
   do {                                                                             \
     hipError_t e_ = (expr);                                                        \

@@ -40,6 +40,10 @@ __device__ __forceinline__ double grid_nu(double nu_min, double step, int64_t i)
   return __dadd_rn(nu_min, __dmul_rn((double)i, step));
 }
 
+// Sets the calling thread's lbl_last_error() text (printf format) and
+// returns code.
+int set_error(int code, const char* fmt, ...);
+
 // Grid points per thread of the mixed-precision line sums.
 constexpr int kPointsMixed = 8;
 
