@@ -51,13 +51,6 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
   sum = t;
 }
 
-__device__ __forceinline__ double fast_rcp64(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
 
 __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
   atomicMin(err, (uint32_t)(2 * k + 1) << kErrGroupBits);
@@ -65,36 +58,70 @@ __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
 
 // ------------------------------------------------------------------ FP64 --
 
+// v_rcp_f64 (relative error about 2^-23) and one Newton step: error about
+// 2^-46, far below the 1e-9 target of the FP64 mode.
+__device__ __forceinline__ double rcp64_1nr(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// The full FP64 dispatch, called (not inlined) from the rare near-centre
+// paths so that its registers do not crowd the far-field loop.
+__device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x, y); }
+
+// Record of the FP64 line sum as staged in LDS (one per (layer, line)); the
+// queue drain reads y and A from the global records.
+struct alignas(16) Rec64 {
+  double c, s;        // shifted centre, sqrt(ln2)/alpha_D
+  double hw2, hw4;    // sqrt(max(700^2 - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
+  double hs, cw;      // step*s; y^2 - 1/2
+  double d2, ay;      // 2 y^2; A y/sqrt(pi)
+};
+// 4 workgroups per CU need under 40 KB of LDS each: 16 KB of records, 16 KB
+// of FP64 partial sums and this many 16-bit queue entries.
+constexpr int kQueueCap64 = 3840;
+
+// FP64 line sum.  Same tiling as the mixed kernel: per record and thread
+// the middle x of the thread's P points decides, for all of them at once,
+// between the 2-node form (|z| >= 700), the 4-node form (|z| >= 35) and the
+// near-centre LDS queue, which the whole workgroup drains at the end of the
+// chunk with the full FP64 dispatch (8-node form and trapezoid+residue),
+// adding the results into the owner's sums with LDS FP64 atomics.
 template <int P>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_fp64(const SumArgs a) {
-  __shared__ double s_c[kChunk], s_s[kChunk], s_y[kChunk], s_a[kChunk];
-  __shared__ double s_y2[kChunk], s_w0[kChunk], s_w1[kChunk];
-  __shared__ double s_cw[kChunk], s_ay[kChunk];
+  __shared__ Rec64 s_rec[kChunk];
+  __shared__ uint16_t s_q[kQueueCap64];
+  __shared__ double s_acc[kBlock * P];
+  __shared__ int s_qn;
 
   const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
   const int64_t tile = wg / a.slices_per_tile;
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  const int64_t s0 = t0 + (int64_t)slice * (kBlock * P);
   // a slice wholly past the end of a short (last) tile has no points: the
   // whole workgroup leaves before its first barrier
-  if (t0 + (int64_t)slice * (kBlock * P) >= t1) return;
-  const int64_t i0 = t0 + (int64_t)slice * (kBlock * P) + (int64_t)threadIdx.x * P;
+  if (s0 >= t1) return;
+  const int64_t i0 = s0 + (int64_t)threadIdx.x * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
-
-  double nu[P], tau[P];
-#pragma unroll
-  for (int j = 0; j < P; ++j) {
-    nu[j] = grid_nu(a.nu_min, a.step, i0 + j);
-    tau[j] = 0.0;
-  }
-  const double t0sq = kGh4T[0] * kGh4T[0], t1sq = kGh4T[1] * kGh4T[1];
+  const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
+  const bool live = i0 < t1;
   const double* rs = static_cast<const double*>(a.rec.s);
   const double* ry = static_cast<const double*>(a.rec.y);
   const double* ra = static_cast<const double*>(a.rec.a);
+  const double half = 0.5 * (P - 1);
+  const double t0sq = kGh4T[0] * kGh4T[0], t1sq = kGh4T[1] * kGh4T[1];
+
+  for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0;
+  if (threadIdx.x == 0) s_qn = 0;
+
+  double tau[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) tau[j] = 0.0;
 
   for (int k = 0; k < a.n_layers; ++k) {
     double lay[P];
@@ -106,57 +133,112 @@ void k_voigt_sum_fp64(const SumArgs a) {
       __syncthreads();
       if ((int)threadIdx.x < cnt) {
         const int64_t r = kb + base + threadIdx.x;
-        double c = a.rec.c[r], s = rs[r], y = ry[r], am = ra[r];
-        if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped)
-          c = 0.0; s = 0.0; y = 1.0e3; am = 0.0;
+        double c = a.rec.c[r], sv = rs[r], y = ry[r], am = ra[r];
+        if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped): adds 0
+          c = 0.0; sv = 0.0; y = 1.0e3; am = 0.0;
         }
-        const int q = threadIdx.x;
-        const double w = am * 2.0 * y * kInvPi;
-        const double y2 = y * y;
-        s_c[q] = c; s_s[q] = s; s_y[q] = y; s_a[q] = am;
-        s_y2[q] = y2; s_w0[q] = w * kGh4W[0]; s_w1[q] = w * kGh4W[1];
-        s_cw[q] = y2 - 0.5; s_ay[q] = am * y * kInvSqrtPi;
+        Rec64 R;
+        const double y2 = y * y, hs = a.step * sv;
+        R.c = c; R.s = sv; R.hs = hs;
+        R.hw2 = fma(half, hs, sqrt(fmax(kFp64Gh2R2 - y2, 0.0)));
+        R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
+        R.cw = y2 - 0.5; R.d2 = 2.0 * y2; R.ay = am * y * kInvSqrtPi;
+        s_rec[threadIdx.x] = R;
       }
       __syncthreads();
-      for (int q = 0; q < cnt; ++q) {
-        const double c = s_c[q], s = s_s[q], y2 = s_y2[q];
-        const double xa = (nu[0] - c) * s, xb = (nu[P - 1] - c) * s;
-        const double mn = xa > 0.0 ? xa : (xb < 0.0 ? -xb : 0.0);
-        const double rmin = fma(mn, mn, y2);
-        if (rmin >= kFp64Gh2R2) {
-          // two-node form (|z| >= 700)
-          const double cw = s_cw[q], d2 = 2.0 * y2, ay = s_ay[q];
+      const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
+      for (int q = 0; q < ncnt; ++q) {
+        const Rec64& R = s_rec[q];
+        const double rc = R.c, rsv = R.s;
+        const double xm = (nu_m - rc) * rsv;
+        const double ax = fabs(xm);
+        if (ax >= R.hw2) {
+          // two-node form (|z| >= 700 at every point); points step from the
+          // first one by integer multiples of hs (inline constants of v_fma_f64)
+          const double hs = R.hs, cw = R.cw, d2 = R.d2, ay = R.ay;
+          const double x0 = fma(-half, hs, xm);
+          double w[P], D[P];
 #pragma unroll
           for (int j = 0; j < P; ++j) {
-            const double x = (nu[j] - c) * s;
-            const double w = fma(x, x, cw);
-            lay[j] = fma(fma(ay, w, ay), fast_rcp64(fma(w, w, d2)), lay[j]);
+            const double x = fma((double)j, hs, x0);
+            w[j] = fma(x, x, cw);
+            D[j] = fma(w[j], w[j], d2);
           }
-        } else if (rmin >= kFp64Gh4R2) {
-          // four-node form (|z| >= 35)
-          const double w0 = s_w0[q], w1 = s_w1[q], fy2 = 4.0 * y2;
+          // four reciprocals from one v_rcp_f64 (Montgomery's batch inversion;
+          // D in [2.4e11, ~1e19] here, so a product of four cannot overflow)
+          static_assert(P % 4 == 0, "batches of four points");
+#pragma unroll
+          for (int g = 0; g < P; g += 4) {
+            const double p01 = D[g] * D[g + 1], p23 = D[g + 2] * D[g + 3];
+            const double r = rcp64_1nr(p01 * p23);
+            const double q01 = r * p23, q23 = r * p01;
+            lay[g] = fma(fma(ay, w[g], ay), q01 * D[g + 1], lay[g]);
+            lay[g + 1] = fma(fma(ay, w[g + 1], ay), q01 * D[g], lay[g + 1]);
+            lay[g + 2] = fma(fma(ay, w[g + 2], ay), q23 * D[g + 3], lay[g + 2]);
+            lay[g + 3] = fma(fma(ay, w[g + 3], ay), q23 * D[g + 2], lay[g + 3]);
+          }
+        } else if (ax >= R.hw4) {
+          // four-node form (|z| >= 35 at every point); A (2y/pi) W = ay (2/sqrt(pi)) W
+          const double hs = R.hs, y2 = 0.5 * R.d2, fy2 = 2.0 * R.d2;
+          const double w0 = R.ay * (2.0 * kInvSqrtPi * kGh4W[0]);
+          const double w1 = R.ay * (2.0 * kInvSqrtPi * kGh4W[1]);
+          const double x0 = fma(-half, hs, xm);
 #pragma unroll
           for (int j = 0; j < P; ++j) {
-            const double x = (nu[j] - c) * s;
+            const double x = fma((double)j, hs, x0);
             const double x2 = x * x;
             const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
             const double u0 = xmy - t0sq, u1 = xmy - t1sq;
             const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
             const double N = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
-            lay[j] = fma(N, fast_rcp64(D0 * D1), lay[j]);
+            lay[j] = fma(N, rcp64_1nr(D0 * D1), lay[j]);
           }
-        } else {
-          const double y = s_y[q], am = s_a[q];
+        } else if (live) {
+          // append (q, thread) with one LDS atomic per wavefront
+          const uint64_t msk = __ballot(1);
+          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+          int qb = 0;
+          if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(msk));
+          qb = __builtin_amdgcn_readfirstlane(qb);
+          const int pos = qb + (int)rank;
+          if (pos < kQueueCap64) {
+            s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
+          } else {  // queue full: evaluate here
+            const int64_t r = kb + base + q;
+            const double y = ry[r], am = ra[r];
 #pragma unroll 1
-          for (int j = 0; j < P; ++j) {
-            double nj = nu[0], lj = 0.0;
+            for (int j = 0; j < P; ++j) {
+              const double v =
+                  am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - rc) * rsv, y);
 #pragma unroll
-            for (int jj = 0; jj < P; ++jj) nj = (jj == j) ? nu[jj] : nj;
-            lj = am * K_fp64((nj - c) * s, y);
-#pragma unroll
-            for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? lj : 0.0;
+              for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
+            }
           }
         }
+      }
+      __syncthreads();
+      const int nq = min(s_qn, kQueueCap64);
+      if (nq > 0) {
+        // dense drain: entry e = (record, thread); points at their exact grid values
+        for (int e = threadIdx.x; e < nq; e += kBlock) {
+          const uint32_t ent = s_q[e];
+          const int q = (int)(ent >> 8), t = (int)(ent & 0xFF);
+          const Rec64& R = s_rec[q];
+          const int64_t r = kb + base + q;
+          const double y = ry[r], am = ra[r];
+          for (int j = 0; j < P; ++j) {
+            const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P + j) - R.c) * R.s;
+            atomicAdd(&s_acc[t * P + j], am * K_fp64_call(x, y));
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          lay[j] += s_acc[threadIdx.x * P + j];
+          s_acc[threadIdx.x * P + j] = 0.0;
+        }
+        if (threadIdx.x == 0) s_qn = 0;
       }
     }
     const double T = a.t_k[k];
@@ -165,7 +247,8 @@ void k_voigt_sum_fp64(const SumArgs a) {
     for (int j = 0; j < P; ++j) {
       const int64_t i = i0 + j;
       if (i < t1) {
-        const double v = lay[j] * (nu[j] * tanh(0.5 * kC2 * nu[j] / T));
+        const double nu = grid_nu(a.nu_min, a.step, i);
+        const double v = lay[j] * (nu * tanh(0.5 * kC2 * nu / T));
         bad |= !isfinite(v);
         tau[j] += v;
         if (a.tau_layers) static_cast<double*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
@@ -440,7 +523,7 @@ __global__ void k_voigt_eval32(int64_t n, const float* x, const float* y, float*
   if (i < n) k[i] = K_mixed(x[i], y[i]);
 }
 
-constexpr int kPointsFp64 = 4;
+constexpr int kPointsFp64 = 8;
 
 }  // namespace
 
