@@ -193,3 +193,27 @@ def test_co2_window_full_size_tiles_vs_oracle():
     om = prob.run("mixed", mus=(1.0,), want_tau=True, want_layers=True)
     relm = np.abs(om["optical_depth_layers"].double().cpu().numpy()[:, m] - od[:, m]) / od[:, m]
     assert relm.max() < 2e-5, relm.max()
+
+
+def test_dense_lines_queue_overflow_vs_oracle():
+    """400 lines within 0.01 cm-1 on a 1e-5 grid, 2 layers: nearly every
+    thread of both workgroups is near-centre for nearly every line, tens of
+    thousands of (record, thread) pairs per chunk against queue capacities of
+    8192 (mixed) and 3840 (FP64), so the queue-full path evaluates in line."""
+    say("case: 400 lines in 0.01 cm-1, step 1e-5, 2 layers (near-centre queue overflow)")
+    rng = np.random.default_rng(17)
+    lines = synth.random_lines(rng, 400, 700.000, 700.010, 2)
+    sd = SpectralData(lines=lines, tips={(2, 1): synth.tips_table()}, molar_masses={(2, 1): 43.99})
+    p, t, _ = synth.levels(3)
+    lay = AtmosphericProfile(p, t, {"CO2": np.full(3, 4.15e-4)}).build_layers()
+    nu, od = _oracle_tau(sd, lay, 699.99, 700.02, 1e-5, 25.0, 2.0)
+    prob = OpticalDepthProblem(sd, lay, 699.99, 700.02, 1e-5, 25.0, 2.0)
+    assert prob.n_nu == 3001 and prob.tile_samples == 200_000
+    o64 = prob.run("fp64", mus=(1.0,), want_layers=True)["optical_depth_layers"].cpu().numpy()
+    rel = np.abs(o64 - od) / od
+    say(f"  fp64 max rel tau {rel.max():.2e}")
+    assert rel.max() < TAU_RTOL_FP64
+    om = prob.run("mixed", mus=(1.0,), want_layers=True)["optical_depth_layers"]
+    relm = np.abs(om.double().cpu().numpy() - od) / od
+    say(f"  mixed max rel tau {relm.max():.2e}")
+    assert relm.max() < 2e-5
