@@ -160,7 +160,9 @@ int lbl_voigt_eval(int64_t n, const void *x, const void *y, void *k_out, int pre
  *   grad_vmr[g][k]     = dL/dvmr_{k,g}      (double, n_gases x n_layers)
  * with the column of gas g in layer k taken as vmr_{k,g} * (air column of
  * layer k), i.e. lbl_layers.col must equal vmr * air.  The Voigt values are
- * recomputed (FP32), nothing of size lines x grid is stored. */
+ * recomputed (FP32), nothing of size lines x grid is stored.  A mixing ratio
+ * that is exactly 0 gets gradient 0: the forward skips that gas in that
+ * layer (forward.py:253-255), so its lines are not visited. */
 int lbl_vjp_workspace_bytes(const lbl_lines_t *lines, const lbl_layers_t *layers,
                             const lbl_grid_t *grid, size_t *bytes);
 int lbl_transmittance_vjp(const lbl_lines_t *lines, const lbl_groups_t *groups,
