@@ -217,3 +217,18 @@ def test_dense_lines_queue_overflow_vs_oracle():
     relm = np.abs(om.double().cpu().numpy() - od) / od
     say(f"  mixed max rel tau {relm.max():.2e}")
     assert relm.max() < 2e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_window_without_lines_and_zero_columns(precision):
+    """No line reaches the window (the reference returns od = 0, forward.py:245-246),
+    and a gas whose mixing ratio is 0 in every layer (each group skipped,
+    forward.py:253-255): T = 1 exactly."""
+    say(f"case: empty window and zero columns ({precision})")
+    sd, prof = synth.co2_window(n_lines=200)
+    nu, T = compute_transmittance(sd, prof, 900.0, 901.0, precision=precision)
+    assert nu.size == 1001 and np.all(T == 1.0)
+    zero = AtmosphericProfile(prof.pressure_hpa, prof.temperature_k,
+                              {"CO2": np.zeros_like(prof.pressure_hpa)})
+    nu, T = compute_transmittance(sd, zero, 720.0, 722.0, precision=precision)
+    assert nu.size == 2001 and np.all(T == 1.0)
