@@ -71,9 +71,10 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-  size_t c, s, y, a, lo, hi, err, total;
+  size_t c, s, y, a, lo, hi, err, tau_part, total;
   size_t g[5];   // adjoint coefficient arrays (only with_grad)
-  int64_t n_tiles;
+  int64_t n_tiles, n_wg;
+  int32_t slices_per_tile, layer_groups;
 };
 
 bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl_grid_t* grid,
@@ -84,6 +85,10 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
                       (size_t)std::max<int32_t>(layers->n_layers, 0);
   const bool fp64 = precision == LBL_FP64;
   L->n_tiles = (grid->n_nu + grid->tile_samples - 1) / grid->tile_samples;
+  const int64_t pts = (int64_t)lbl::sum_block(precision) * lbl::sum_points_per_thread(precision);
+  L->slices_per_tile = (int32_t)((grid->tile_samples + pts - 1) / pts);
+  L->n_wg = L->n_tiles * L->slices_per_tile;
+  L->layer_groups = lbl::layer_groups_for(L->n_wg, std::max<int32_t>(layers->n_layers, 0));
   size_t off = 0;
   // FP64: SoA c, s, y, a (8 B each); mixed: one 48-byte MixedRec per record at L->c
   L->c = off; off = align_up(off + nrec * (fp64 ? 8 : sizeof(lbl::MixedRec)));
@@ -93,6 +98,9 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->lo = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->hi = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->err = off; off = align_up(off + 4);
+  // partial tau per layer group (float sum + compensation, or double): 8 B
+  L->tau_part = off;
+  if (L->layer_groups > 1) off = align_up(off + (size_t)L->layer_groups * (size_t)grid->n_nu * 8);
   for (int i = 0; i < 5; ++i) {
     L->g[i] = off;
     if (with_grad) off = align_up(off + nrec * 4);
@@ -223,10 +231,11 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   sa.n_nu = grid->n_nu;
   sa.ts = grid->tile_samples;
   sa.n_tiles = L.n_tiles;
-  const int64_t pts = (int64_t)lbl::sum_block(precision) * lbl::sum_points_per_thread(precision);
-  sa.slices_per_tile = (int32_t)((grid->tile_samples + pts - 1) / pts);
-  sa.n_wg = L.n_tiles * sa.slices_per_tile;
+  sa.slices_per_tile = L.slices_per_tile;
+  sa.n_wg = L.n_wg;
   if (sa.n_wg > INT32_MAX) return fail(LBL_EINVAL, "grid too large");
+  sa.layer_groups = L.layer_groups;
+  sa.tau_part = ws + L.tau_part;
   sa.n_mu = out->n_mu;
   for (int m = 0; m < out->n_mu; ++m) sa.inv_mu[m] = 1.0 / out->mu[m];
   sa.trans = out->transmittance;
@@ -242,6 +251,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   }
   if (ev) HIP_TRY(hipEventRecord(ev[2], s));
   HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));
+  HIP_TRY(lbl::launch_tau_epilogue(sa, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[3], s));
 
   uint32_t code = lbl::kErrNone;

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "../../include/pylblrtm_amd.h"
@@ -118,12 +120,33 @@ struct SumArgs {
   float mlp_gate;   // a point uses the network iff A * c * K_ref(x, y) <= mlp_gate
   unsigned long long* mlp_count;     // network evaluations (nullable)
   unsigned long long* exact_count;   // exact near-centre evaluations (nullable)
+  // Layer groups: when the grid has too few tile slices to fill the GPU, the
+  // layers are split into layer_groups ranges, one workgroup per (slice,
+  // range); each writes its partial tau to tau_part[group][point] (mixed:
+  // float sum and Kahan compensation; FP64: double) and launch_tau_epilogue
+  // adds them and writes T (and tau).  layer_groups == 1: T is written by the
+  // line sum itself.
+  int32_t layer_groups;
+  void* tau_part;
 };
+
+// Workgroups a line-sum launch should have to keep every CU busy (4 resident
+// per CU on 256 CUs).
+constexpr int64_t kTargetWorkgroups = 1024;
+
+// Number of layer ranges for a grid of n_wg tile slices.
+inline int32_t layer_groups_for(int64_t n_wg, int32_t n_layers) {
+  if (n_wg <= 0 || n_layers <= 1 || n_wg >= kTargetWorkgroups) return 1;
+  const int64_t g = (kTargetWorkgroups + n_wg - 1) / n_wg;
+  return (int32_t)std::min<int64_t>(g, n_layers);
+}
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_line_params(const LineParamsArgs& a, int precision, hipStream_t s);
 hipError_t launch_tile_ranges(const TileRangeArgs& a, hipStream_t s);
 hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s);
+// Sum of the layer-group partials, T = exp(-tau/mu) (only when layer_groups > 1).
+hipError_t launch_tau_epilogue(const SumArgs& a, int precision, hipStream_t s);
 hipError_t launch_voigt_eval(int64_t n, const void* x, const void* y, void* k, int precision,
                              hipStream_t s);
 

@@ -95,7 +95,13 @@ void k_voigt_sum_fp64(const SumArgs a) {
   __shared__ double s_acc[kBlock * P];
   __shared__ int s_qn;
 
-  const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
+  // (layer group, tile slice), group-major so that an XCD holds neighbouring
+  // tiles of one layer range
+  const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.layer_groups);
+  const int grp = (int)(lw / a.n_wg);
+  const int64_t wg = lw - (int64_t)grp * a.n_wg;
+  const int k_begin = (int)((int64_t)a.n_layers * grp / a.layer_groups);
+  const int k_end = (int)((int64_t)a.n_layers * (grp + 1) / a.layer_groups);
   const int64_t tile = wg / a.slices_per_tile;
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
@@ -123,7 +129,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
 #pragma unroll
   for (int j = 0; j < P; ++j) tau[j] = 0.0;
 
-  for (int k = 0; k < a.n_layers; ++k) {
+  for (int k = k_begin; k < k_end; ++k) {
     double lay[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) lay[j] = 0.0;
@@ -260,6 +266,10 @@ void k_voigt_sum_fp64(const SumArgs a) {
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
     if (i < t1) {
+      if (a.layer_groups > 1) {
+        static_cast<double*>(a.tau_part)[(int64_t)grp * a.n_nu + i] = tau[j];
+        continue;
+      }
       for (int m = 0; m < a.n_mu; ++m)
         static_cast<double*>(a.trans)[(int64_t)m * a.n_nu + i] = exp(-tau[j] * a.inv_mu[m]);
       if (a.tau) static_cast<double*>(a.tau)[i] = tau[j];
@@ -297,7 +307,13 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ float s_acc[kMixedBlock * P];
   __shared__ int s_qn;
 
-  const int64_t wg = xcd_remap(blockIdx.x, a.n_wg);
+  // (layer group, tile slice), group-major so that an XCD holds neighbouring
+  // tiles of one layer range
+  const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.layer_groups);
+  const int grp = (int)(lw / a.n_wg);
+  const int64_t wg = lw - (int64_t)grp * a.n_wg;
+  const int k_begin = (int)((int64_t)a.n_layers * grp / a.layer_groups);
+  const int k_end = (int)((int64_t)a.n_layers * (grp + 1) / a.layer_groups);
   const int64_t tile = wg / a.slices_per_tile;
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
@@ -325,7 +341,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
 #pragma unroll
   for (int j = 0; j < P; ++j) tau[j] = tauc[j] = 0.0f;
 
-  for (int k = 0; k < a.n_layers; ++k) {
+  for (int k = k_begin; k < k_end; ++k) {
     float lay[P], layc[P], part[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) lay[j] = layc[j] = part[j] = 0.0f;
@@ -485,6 +501,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
     if (i < t1) {
+      if (a.layer_groups > 1) {
+        static_cast<float2*>(a.tau_part)[(int64_t)grp * a.n_nu + i] = make_float2(tau[j], tauc[j]);
+        continue;
+      }
       const float t = tau[j] - tauc[j];
       for (int m = 0; m < a.n_mu; ++m)
         static_cast<float*>(a.trans)[(int64_t)m * a.n_nu + i] = expf(-t * (float)a.inv_mu[m]);
@@ -523,6 +543,34 @@ __global__ void k_voigt_eval32(int64_t n, const float* x, const float* y, float*
   if (i < n) k[i] = K_mixed(x[i], y[i]);
 }
 
+// Layer-group epilogue: tau = sum over groups of the partial taus (in group,
+// i.e. layer, order), T = exp(-tau/mu); one thread per grid point.
+__global__ void k_tau_epilogue64(const SumArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_nu) return;
+  const double* part = static_cast<const double*>(a.tau_part);
+  double t = 0.0;
+  for (int g = 0; g < a.layer_groups; ++g) t += part[(int64_t)g * a.n_nu + i];
+  for (int m = 0; m < a.n_mu; ++m)
+    static_cast<double*>(a.trans)[(int64_t)m * a.n_nu + i] = exp(-t * a.inv_mu[m]);
+  if (a.tau) static_cast<double*>(a.tau)[i] = t;
+}
+
+__global__ void k_tau_epilogue32(const SumArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_nu) return;
+  const float2* part = static_cast<const float2*>(a.tau_part);
+  float t = 0.0f, c = 0.0f;
+  for (int g = 0; g < a.layer_groups; ++g) {
+    const float2 p = part[(int64_t)g * a.n_nu + i];
+    kahan_add(t, c, p.x - p.y);
+  }
+  t -= c;
+  for (int m = 0; m < a.n_mu; ++m)
+    static_cast<float*>(a.trans)[(int64_t)m * a.n_nu + i] = expf(-t * (float)a.inv_mu[m]);
+  if (a.tau) static_cast<float*>(a.tau)[i] = t;
+}
+
 constexpr int kPointsFp64 = 8;
 
 }  // namespace
@@ -535,12 +583,24 @@ int sum_block(int precision) { return precision == LBL_FP64 ? kBlock : kMixedBlo
 
 hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0) return hipSuccess;
+  const int64_t n_blocks = a.n_wg * a.layer_groups;
+  if (a.layer_groups < 1 || n_blocks * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
   if (precision == LBL_FP64)
-    k_voigt_sum_fp64<kPointsFp64><<<(unsigned)a.n_wg, kBlock, 0, s>>>(a);
+    k_voigt_sum_fp64<kPointsFp64><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
   else if (precision == LBL_SURROGATE)
-    k_voigt_sum_surrogate<kPointsMixed><<<(unsigned)a.n_wg, kMixedBlock, 0, s>>>(a);
+    k_voigt_sum_surrogate<kPointsMixed><<<(unsigned)n_blocks, kMixedBlock, 0, s>>>(a);
   else
-    k_voigt_sum_mixed<kPointsMixed><<<(unsigned)a.n_wg, kMixedBlock, 0, s>>>(a);
+    k_voigt_sum_mixed<kPointsMixed><<<(unsigned)n_blocks, kMixedBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tau_epilogue(const SumArgs& a, int precision, hipStream_t s) {
+  if (a.layer_groups <= 1 || a.n_nu <= 0) return hipSuccess;
+  const unsigned g = (unsigned)((a.n_nu + 255) / 256);
+  if (precision == LBL_FP64)
+    k_tau_epilogue64<<<g, 256, 0, s>>>(a);
+  else
+    k_tau_epilogue32<<<g, 256, 0, s>>>(a);
   return hipGetLastError();
 }
 
