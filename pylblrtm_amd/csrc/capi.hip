@@ -88,7 +88,7 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   const int64_t pts = (int64_t)lbl::sum_block(precision) * lbl::sum_points_per_thread(precision);
   L->slices_per_tile = (int32_t)((grid->tile_samples + pts - 1) / pts);
   L->n_wg = L->n_tiles * L->slices_per_tile;
-  L->layer_groups = lbl::layer_groups_for(L->n_wg, std::max<int32_t>(layers->n_layers, 0));
+  L->layer_groups = lbl::layer_groups_for(L->n_wg, std::max<int32_t>(layers->n_layers, 0), precision);
   size_t off = 0;
   // FP64: SoA c, s, y, a (8 B each); mixed: one 48-byte MixedRec per record at L->c
   L->c = off; off = align_up(off + nrec * (fp64 ? 8 : sizeof(lbl::MixedRec)));
