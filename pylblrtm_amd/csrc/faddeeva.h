@@ -45,6 +45,19 @@ constexpr double kFp64Gh8R2 = 225.0;    // |z| >= 15 : 8 nodes, rel err <= 1e-15
 constexpr float kF32Gh2R2 = 625.0f;     // |z| >= 25: 2 nodes, rel err <= 6.4e-6
 constexpr float kF32Gh4R2 = 64.0f;      // |z| >= 8 : 4 nodes, rel err <= 1e-6
 
+// Shifted one-node far form (mixed line sum): K ~= y / (sqrt(pi) (x^2 + y^2 - 3/2)).
+// The asymptotic expansion K = (1/sqrt(pi)) sum_n (2n-1)!!/2^n sin((2n+1) theta)/|z|^(2n+1)
+// (theta = arg z) is y/(sqrt(pi)|z|^2) (1 + 3/(2|z|^2) + (15/4 - 2y^2)/|z|^4 + ...);
+// the shifted term reproduces the first two orders, so its relative error is
+// (2 y^2 - 3/2)/|z|^4 + O(|z|^-6): measured |err| <= 1.013 (3/2 + 2 y^2)/|z|^4 for
+// |z| >= 20 against the series summed to 40 terms.  It is used where
+// 1.05 (3/2 + 2 y^2)/|z|^4 <= 6.4e-6 (the two-node form's bound at |z| = 25)
+// and |z| >= 25, i.e. r >= max(625, sqrt(kF32S1Coef (3/2 + 2 y^2))).
+constexpr float kF32S1Coef = 164062.5f;  // 1.05 / 6.4e-6
+__device__ __forceinline__ float s1_r2_min(float y2) {
+  return fmaxf(kF32Gh2R2, sqrtf(kF32S1Coef * fmaf(2.0f, y2, 1.5f)));
+}
+
 // Gauss-Hermite positive nodes / weights (numpy.polynomial.hermite.hermgauss).
 constexpr double kGh2T = 0.7071067811865475, kGh2W = 0.8862269254527579;
 constexpr double kGh4T[2] = {0.5246476232752904, 1.6506801238857847};

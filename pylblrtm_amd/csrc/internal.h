@@ -52,19 +52,21 @@ constexpr int kPointsMixed = 8;
 // Per-(layer, line) record of the mixed-precision paths (48 bytes, one
 // 16-byte-aligned struct so the line sum stages it with three 16-byte loads).
 // The first 16 bytes are what the line sum's per-record test reads, the next
-// 16 what its far-field branch reads:
+// 16 what its far-field branches read:
 //   c  shifted centre (FP64), s = sqrt(ln2)/alpha_D,
-//   hw = sqrt(max(25^2 - y^2, 0)) + (kPointsMixed - 1)/2 * hs: a thread's
-//        kPointsMixed points all lie where the far form holds (|z| >= 25)
-//        iff |x| at the middle of the run is >= hw,
-//   hs = step * s, cw = y^2 - 1/2, d2 = 2 y^2, ay = A y / sqrt(pi)
-//        (two-node far form),
+//   hw1 = sqrt(max(s1_r2_min(y^2) - y^2, 0)) + (kPointsMixed - 1)/2 * hs: a
+//        thread's kPointsMixed points all lie where the shifted one-node form
+//        holds (faddeeva.h) iff |x| at the middle of the run is >= hw1,
+//   hs = step * s, c1 = y^2 - 3/2, ay = A y / sqrt(pi) (one-node form; the
+//        two-node form's y^2 - 1/2 = c1 + 1 and 2 y^2 = 2 c1 + 3 are exact),
+//   hw2 = sqrt(max(25^2 - y^2, 0)) + (kPointsMixed - 1)/2 * hs: the same for
+//        the two-node form (|z| >= 25),
 //   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D.
-// A skipped (zero-column) record has c = NaN, ay = a = 0, d2 = 1.
+// A skipped (zero-column) record has c = NaN, ay = a = 0.
 struct alignas(16) MixedRec {
   double c;
-  float s, hw;
-  float hs, cw, d2, ay;
+  float s, hw1;
+  float hs, c1, ay, hw2;
   float y, a, pad0, pad1;
 };
 static_assert(sizeof(MixedRec) == 48, "record layout");

@@ -63,10 +63,11 @@ __device__ __forceinline__ void store_mixed(const Records& r, int64_t i, double 
   m.c = c;
   m.s = s;
   m.hs = step * s;
-  m.cw = y2 - 0.5f;
-  m.d2 = 2.0f * y2;
+  m.c1 = y2 - 1.5f;
   m.ay = a * y * (float)kInvSqrtPi;
-  m.hw = fmaf(0.5f * (float)(kPointsMixed - 1), m.hs, sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)));
+  const float half_run = 0.5f * (float)(kPointsMixed - 1) * m.hs;
+  m.hw1 = half_run + sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
+  m.hw2 = half_run + sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
   m.y = y;
   m.a = a;
   m.pad0 = m.pad1 = 0.0f;

@@ -14,12 +14,13 @@
 // the kernel writes exp(-tau/mu) for every requested mu.
 //
 // Region split.  Almost every evaluation is far from its line centre, where
-// the two-node (FP32) / four-node (FP64) Gauss-Hermite form is accurate.  A
-// thread decides once per record whether all of its P points are in that
-// region (a per-lane test on the range of x it covers); lanes that are not
-// take the full region dispatch of faddeeva.h.  Both branches are plain
-// divergent ifs, so a wavefront only pays for the slow branch when one of
-// its lanes needs it.
+// a short rational form is accurate: in mixed mode the shifted one-node form
+// (4 VALU operations per point) and, closer in for broad lines, the two-node
+// Gauss-Hermite form; in FP64 the two- and four-node forms.  A thread decides
+// once per record which form holds for all of its P points (a compare of |x|
+// at the middle of its run with a per-record threshold); pairs (record,
+// thread) that need the full region dispatch of faddeeva.h go to an LDS work
+// queue that the workgroup drains densely after each chunk of records.
 //
 // Precision.  FP64 mode: every operation in FP64.  Mixed mode: nu - nu_c is
 // formed in FP64 from the exact grid point and the FP64 centre, rounded to
@@ -354,31 +355,38 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         const int q = threadIdx.x + h * kMixedBlock;
         if (q < cnt) {
           MixedRec m = recs[kb + base + q];
-          if (!(m.c >= lo_b && m.c <= hi_b)) {  // outside the tile window (or skipped)
-            m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.cw = 0.0f; m.d2 = 1.0f;
-            m.ay = 0.0f; m.hw = 0.0f; m.a = 0.0f;
+          if (!(m.c >= lo_b && m.c <= hi_b)) {  // outside the tile window (or skipped):
+            // x = 0 passes the far test and adds ay / (0 + c1) = 0 exactly
+            m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.c1 = 1.0f;
+            m.ay = 0.0f; m.hw1 = 0.0f; m.a = 0.0f;
           }
           s_rec[q] = m;
         }
       }
       __syncthreads();
-      // Per record: c, s, hw in one 16-byte LDS read (prefetched one record
-      // ahead, so its latency hides behind the current record), and hs, cw,
-      // d2, ay in the next 16 bytes, read only on the far branch; volatile
-      // keeps each a single ds_read_b128.
+      // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
+      // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
+      // keeps each a single ds_read_b128.  Three tiers per (record, thread):
+      // the shifted one-node form, the two-node form, and the work queue.
       const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
       const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
-      u32x4 hn = rb[0];
-      for (int q = 0; q < ncnt; ++q) {
-        const u32x4 h0 = hn;
-        if (q + 1 < ncnt) hn = rb[3 * (q + 1)];
+      auto record = [&](const int q, const u32x4 h0, const volatile lds_u32x4* const body) {
         const double c = __hiloint2double((int)h0.y, (int)h0.x);
         const float xm = (float)(nu_m - c) * __uint_as_float(h0.z);
+        const u32x4 h1 = *body;
+        const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
+        const float ay = __uint_as_float(h1.z);
         if (fabsf(xm) >= __uint_as_float(h0.w)) {
-          // two-node form: K = (y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
-          const u32x4 h1 = rb[3 * q + 1];
-          const float hs = __uint_as_float(h1.x), cw = __uint_as_float(h1.y);
-          const float d2 = __uint_as_float(h1.z), ay = __uint_as_float(h1.w);
+          // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
+            const float D = fmaf(x, x, c1);
+            part[j] = fmaf(ay, __builtin_amdgcn_rcpf(D), part[j]);
+          }
+        } else if (fabsf(xm) >= __uint_as_float(h1.w)) {
+          // two-node form: K A = (A y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
+          const float cw = c1 + 1.0f, d2 = fmaf(2.0f, c1, 3.0f);
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
@@ -399,7 +407,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           if (pos < kQueueCap) {
             s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
           } else {  // queue full: evaluate here
-            const float y = s_rec[q].y, am = s_rec[q].a, hs = s_rec[q].hs;
+            const float y = s_rec[q].y, am = s_rec[q].a;
 #pragma unroll 1
             for (int j = 0; j < P; ++j) {
               const float v = am * K_mixed(fmaf((float)j - 0.5f * (float)(P - 1), hs, xm), y);
@@ -408,7 +416,23 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             }
           }
         }
-        if ((q & (kFoldEvery - 1)) == kFoldEvery - 1) {
+      };
+      // Two records per iteration, each header read two records ahead into
+      // its own registers (no copies between iterations); kChunk entries
+      // exist, so record 1 is readable even when cnt == 1.  The record
+      // address lives in a VGPR (the empty asm makes it one) so every LDS
+      // read uses an immediate offset instead of a copy from an SGPR.
+      u32x4 ha = rb[0], hb = rb[3];
+      const volatile lds_u32x4* pr = rb;
+      asm volatile("" : "+v"(pr));
+      for (int q = 0; q < ncnt; q += 2, pr += 6) {
+        record(q, ha, pr + 1);
+        if (q + 2 < ncnt) ha = pr[6];
+        if (q + 1 < ncnt) {
+          record(q + 1, hb, pr + 4);
+          if (q + 3 < ncnt) hb = pr[9];
+        }
+        if (((q + 2) & (kFoldEvery - 1)) == 0) {   // after records 31, 63, ...
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             kahan_add(lay[j], layc[j], part[j]);
