@@ -42,7 +42,7 @@ constexpr double kInvSqrtPi = 0.56418958354775628695;
 constexpr double kFp64Gh2R2 = 490000.0; // |z| >= 700: 2 nodes, rel err <= 1.04e-11
 constexpr double kFp64Gh4R2 = 1225.0;   // |z| >= 35 : 4 nodes, rel err <= 5.1e-12
 constexpr double kFp64Gh8R2 = 225.0;    // |z| >= 15 : 8 nodes, rel err <= 1e-15
-constexpr float kF32Gh2R2 = 625.0f;     // |z| >= 25: 2 nodes, rel err <= 6.4e-6
+constexpr float kF32Gh2R2 = 625.0f;     // |z| >= 25: 2 nodes, rel err <= 6.41e-6
 constexpr float kF32Gh4R2 = 64.0f;      // |z| >= 8 : 4 nodes, rel err <= 1e-6
 
 // Shifted one-node far form (mixed line sum): K ~= y / (sqrt(pi) (x^2 + y^2 - 3/2)).

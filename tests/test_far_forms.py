@@ -1,0 +1,135 @@
+"""The far-field forms of the mixed-precision line sum (csrc/faddeeva.h,
+csrc/voigt_sum.hip) checked against independent references on the CPU.
+
+K(x, y) = Re w(x + iy).  References: scipy's ``wofz`` where it is accurate
+(moderate |z|), and, for large |z|, the asymptotic series
+K = (1/sqrt(pi)) sum_n (2n-1)!!/2^n sin((2n+1) theta) / |z|^(2n+1), theta = arg z,
+whose real part is formed from sin((2n+1) theta) directly, so it keeps full
+relative accuracy when y << x (where Re w is a tiny part of |w|).
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy.special import wofz
+
+SQRT_PI = math.sqrt(math.pi)
+S1_COEF = np.float32(164062.5)      # 1.05 / 6.4e-6, faddeeva.h kF32S1Coef
+EPS_FAR = 6.4e-6                    # target of the one-node threshold
+EPS_GH2 = 6.42e-6                   # the two-node form's error at |z| = 25 (max 6.41e-6)
+P = 8                               # points per thread (kPointsMixed)
+
+
+def k_series(x, y, terms=40):
+    r = np.hypot(x, y)
+    th = np.arctan2(y, x)
+    k = np.zeros_like(r)
+    coef = 1.0
+    rp = r.copy()
+    with np.errstate(over="ignore"):
+        for n in range(terms):
+            k += coef * np.sin((2 * n + 1) * th) / rp
+            coef *= (2 * n + 1) / 2.0
+            rp = rp * r * r
+    return k / SQRT_PI
+
+
+def s1_r2_min(y2):
+    """faddeeva.h s1_r2_min, in float32 as the kernel evaluates it."""
+    y2 = np.float32(y2)
+    return np.maximum(np.float32(625.0), np.sqrt(S1_COEF * (np.float32(2.0) * y2 + np.float32(1.5))))
+
+
+def k_s1(x, y):
+    return y / (SQRT_PI * (x * x + y * y - 1.5))
+
+
+def k_gh2(x, y):
+    w = x * x + y * y - 0.5
+    return y * (w + 1.0) / (SQRT_PI * (w * w + 2.0 * y * y))
+
+
+def test_series_reference_agrees_with_wofz():
+    rng = np.random.default_rng(11)
+    x = rng.uniform(20.0, 200.0, 20000)
+    y = 10.0 ** rng.uniform(-1.0, 2.0, 20000)
+    ref = wofz(x + 1j * y).real
+    assert np.max(np.abs(k_series(x, y) / ref - 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("ylo,yhi", [(1e-4, 1e-2), (1e-2, 1.0), (1.0, 10.0), (10.0, 100.0),
+                                     (100.0, 1000.0)])
+def test_shifted_one_node_within_bound_where_used(ylo, yhi):
+    """Where the kernel uses it (|z|^2 >= s1_r2_min(y^2)) the shifted one-node
+    form is within 6.4e-6 relative of K, the two-node form's bound."""
+    rng = np.random.default_rng(int(ylo * 1e4) + 7)
+    n = 200000
+    y = np.exp(rng.uniform(math.log(ylo), math.log(yhi), n))
+    r_min = np.sqrt(s1_r2_min(y * y).astype(np.float64))
+    r = r_min * np.exp(rng.uniform(0.0, math.log(200.0), n))
+    r[: n // 10] = r_min[: n // 10]          # the boundary itself
+    ok = r > y
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y) * rng.choice([-1.0, 1.0], y.size)
+    err = np.abs(k_s1(x, y) / k_series(x, y) - 1.0)
+    assert err.max() <= EPS_FAR
+    # and it is the form the thresholds were derived for: error ~ (2y^2 - 3/2)/|z|^4
+    near = r < 1.01 * r_min[ok]
+    assert err[near].max() > 0.2 * EPS_FAR
+
+
+def test_shifted_one_node_error_model():
+    """|err| <= 1.05 (3/2 + 2 y^2) / |z|^4 for |z| >= 20 (the bound the
+    threshold is built from); a dropped or wrong shift fails it."""
+    rng = np.random.default_rng(5)
+    n = 400000
+    y = np.exp(rng.uniform(math.log(1e-3), math.log(300.0), n))
+    r = np.exp(rng.uniform(math.log(20.0), math.log(2000.0), n))
+    ok = r > y * 1.0001
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    k = k_series(x, y)
+    bound = 1.05 * (1.5 + 2.0 * y * y) / r ** 4
+    assert np.all(np.abs(k_s1(x, y) / k - 1.0) <= bound)
+    unshifted = y / (SQRT_PI * (x * x + y * y))
+    assert np.any(np.abs(unshifted / k - 1.0) > bound)
+
+
+def test_two_node_within_bound_at_25():
+    rng = np.random.default_rng(9)
+    n = 200000
+    th = rng.uniform(0.0, math.pi / 2, n)
+    r = 25.0 * np.exp(rng.uniform(0.0, math.log(40.0), n))
+    x, y = r * np.cos(th), np.maximum(r * np.sin(th), 1e-6)
+    assert np.max(np.abs(k_gh2(x, y) / k_series(x, y) - 1.0)) <= EPS_GH2
+
+
+@pytest.mark.parametrize("which", ["s1", "gh2"])
+def test_mid_point_threshold_covers_the_whole_run(which):
+    """|x_mid| >= hw (line_params.hip store_mixed) puts all P points of the
+    run x_mid + (j - 3.5) hs inside the form's region."""
+    rng = np.random.default_rng(3 if which == "s1" else 4)
+    n = 100000
+    y = np.exp(rng.uniform(math.log(1e-3), math.log(300.0), n)).astype(np.float32)
+    hs = np.exp(rng.uniform(math.log(1e-3), math.log(30.0), n)).astype(np.float32)
+    y2 = y * y
+    r2 = s1_r2_min(y2) if which == "s1" else np.float32(625.0)
+    hw = (np.float32(0.5 * (P - 1)) * hs + np.sqrt(np.maximum(r2 - y2, np.float32(0.0)))).astype(np.float32)
+    xm = (hw * np.float32(1.0) + np.float32(1e-3) * hs * rng.uniform(0, 1, n).astype(np.float32))
+    xm *= rng.choice([-1.0, 1.0], n).astype(np.float32)
+    j = np.arange(P, dtype=np.float32) - np.float32(0.5 * (P - 1))
+    x = xm[:, None] + j[None, :] * hs[:, None]
+    z2 = x.astype(np.float64) ** 2 + y2[:, None].astype(np.float64)
+    assert np.all(z2 >= np.asarray(r2, np.float64).reshape(-1, 1) * (1.0 - 1e-6))
+
+
+def test_two_node_constants_from_c1_are_exact():
+    """cw = c1 + 1 and d2 = 2 c1 + 3 reproduce y^2 - 1/2 and 2 y^2 exactly in
+    float32 (c1 = y^2 - 3/2 as stored) for 3 <= y^2 < 2^23."""
+    rng = np.random.default_rng(1)
+    y2 = np.exp(rng.uniform(math.log(3.0), math.log(2.0 ** 23), 1000000)).astype(np.float32)
+    c1 = (y2 - np.float32(1.5)).astype(np.float32)
+    assert np.array_equal((c1 + np.float32(1.0)).astype(np.float32), (y2 - np.float32(0.5)).astype(np.float32))
+    d2 = (np.float32(2.0) * c1 + np.float32(3.0)).astype(np.float32)
+    assert np.array_equal(d2, (np.float32(2.0) * y2).astype(np.float32))
