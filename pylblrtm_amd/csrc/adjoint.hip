@@ -16,11 +16,14 @@
 //                d ln A/dv = 1/v + delta P (1/c + (c2/T)/sinh(c2 c/T))
 //                ax = 0, bx = s delta P, dy/dv = sqrt(ln2) (296/T)^n P (g_self - g_air)/alpha_D
 // and, at every grid point, dR/dT = -R (c2 nu/T^2)/sinh(c2 nu/T).
-// The backward sum recomputes K and its derivatives (no lines x grid storage):
-//   far (|z| >= 25, the forward's two-node form, w = x^2 + y^2 - 1/2, D = w^2 + 2y^2):
+// The backward sum recomputes K and its derivatives (no lines x grid storage),
+// with the forward's region split:
+//   far, shifted one-node form (faddeeva.h, D1 = x^2 + y^2 - 3/2):
+//     K = y/(sqrt(pi) D1),  dK/dx = -2xy/(sqrt(pi) D1^2),  dK/dy = (D1 - 2y^2)/(sqrt(pi) D1^2),
+//   far, two-node form (|z| >= 25, w = x^2 + y^2 - 1/2, D = w^2 + 2y^2):
 //     K = y (w+1)/(sqrt(pi) D),  dK/dx = 2xy (2y^2 - w^2 - 2w)/(sqrt(pi) D^2),
 //     dK/dy = ((w + 1 + 2y^2) D - 4 y^2 (w+1)^2)/(sqrt(pi) D^2),
-//   accumulated per record as five point sums weighted once per record (VjpFar)
+//   both accumulated per record as point sums weighted once per record (VjpFar);
 //   near: K, dK/dx, dK/dy from kgrad_mixed (each region's approximant differentiated).
 #include <math.h>
 
@@ -131,18 +134,21 @@ struct VjpArgs {
   double* grad_vmr;            // [n_gases][n_layers]
 };
 
-// Far-field record of one (layer, line) pair as staged in LDS.  Away from the
-// centre K = ay F with F = (w+1)/D, and per grid point
+// Far-field record of one (layer, line) pair as staged in LDS.  Two-node form:
+// K = ay F with F = (w+1)/D, and per grid point
 //   dK/dx = 2 ay x F',  dK/dy = aq (F + d2 F' - 2 d2 F/D),  F' = (d2 - w^2 - 2w)/D^2,
 // so a record's contribution to both gradients is a fixed combination of five
 // running sums over its points: S1 = sum W F, S2 = sum W F', S3 = sum W x F',
 // S4 = sum W x^2 F', S5 = sum W F/D (W = the weighted upstream gradient).
 // The coefficients below are those combinations' weights; dx/dT = -x/(2T)
-// multiplies S4, dx/dv = bx multiplies S3.
+// multiplies S4, dx/dv = bx multiplies S3.  Shifted one-node form (r = 1/D1):
+// K = ay r, dK/dx = -2 ay x r^2, dK/dy = aq (r - d2 r^2), so with S1 = sum W r,
+// S2 = sum W r^2, S3 = sum W x r^2, S4 = sum W x^2 r^2 the same coefficients
+// apply as tF S1 - tX2 S4 - tD S2 and vF S1 - vX S3 - vD S2.
 struct alignas(16) VjpFar {
   double c;
   float s, hs;
-  float cw, d2, hw, ay;      // y^2 - 1/2, 2 y^2, |x| threshold, A y/sqrt(pi)
+  float c1, hw1, hw2, ay;    // y^2 - 3/2, |x| thresholds (one-node, two-node), A y/sqrt(pi)
   float tF, tX2, tD, tQ;     // temperature: weights of S1, S4, S2, S5
   float vF, vX, vD, vQ;      // mixing ratio: weights of S1, S3, S2, S5
 };
@@ -245,8 +251,9 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       const float aq = am * (float)kInvSqrtPi;
       const float cty = dyt * aq, cvy = dyv * aq;
       f.c = c; f.s = s; f.hs = stepf * s;
-      f.cw = y2 - 0.5f; f.d2 = d2;
-      f.hw = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
+      f.c1 = y2 - 1.5f;
+      f.hw1 = sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
+      f.hw2 = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
       f.tF = at * ay + cty; f.tX2 = 2.0f * ax_t * ay; f.tD = cty * d2; f.tQ = -2.0f * d2 * cty;
       f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
       n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
@@ -284,8 +291,25 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
         const float hs = R.hs;
         const float xb = fmaf((float)(P - 1), hs, x0);
         float acc_t, acc_v;
-        if (x0 >= R.hw || xb <= -R.hw) {
-          const float cw = R.cw, d2 = R.d2;
+        if (x0 >= R.hw1 || xb <= -R.hw1) {
+          const float c1 = R.c1;
+          float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f;
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, hs, x0);
+            const float r = __builtin_amdgcn_rcpf(fmaf(x, x, c1));
+            const float q = WR[j] * r;
+            const float q2 = q * r;
+            const float xq2 = x * q2;
+            S1 += q;
+            S2 += q2;
+            S3 += xq2;
+            S4 = fmaf(x, xq2, S4);
+          }
+          acc_t = fmaf(R.tF, S1, -fmaf(R.tX2, S4, R.tD * S2));
+          acc_v = fmaf(R.vF, S1, -fmaf(R.vX, S3, R.vD * S2));
+        } else if (x0 >= R.hw2 || xb <= -R.hw2) {
+          const float cw = R.c1 + 1.0f, d2 = fmaf(2.0f, R.c1, 3.0f);
           float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f, S5 = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {

@@ -159,10 +159,12 @@ __device__ inline double K_fp64(double x, double y) {
 }
 
 // ---------------------------------------------------------------- FP32 ----
+// Quotients use v_rcp_f32 (1 ulp) and a multiply instead of the correctly
+// rounded division sequence: 2 ulp, far inside the FP32 forms' own error.
 
 __device__ __forceinline__ float gh_pair32(float r, float xmy, float fxy, float t2, float w) {
   const float u = xmy - t2;
-  return w * (r + t2) / fmaf(u, u, fxy);
+  return w * (r + t2) * __builtin_amdgcn_rcpf(fmaf(u, u, fxy));
 }
 
 // 2-node form: K = y (r + 1/2) / (sqrt(pi) D), D = x^4 + x^2 (2y^2 - 1) + (y^2 + 1/2)^2.
@@ -170,7 +172,7 @@ __device__ __forceinline__ float K_gh2_32(float x2, float y) {
   const float y2 = y * y;
   const float c = y2 + 0.5f;
   const float D = fmaf(x2, x2 + (2.0f * y2 - 1.0f), c * c);
-  return y * (float)kInvSqrtPi * (x2 + c) / D;
+  return y * (float)kInvSqrtPi * (x2 + c) * __builtin_amdgcn_rcpf(D);
 }
 
 // 4-node form, |z| >= 8.
@@ -180,7 +182,7 @@ __device__ __forceinline__ float K_gh4_32(float x2, float y) {
   const float u0 = xmy - t0, u1 = xmy - t1;
   const float D0 = fmaf(u0, u0, fxy), D1 = fmaf(u1, u1, fxy);
   const float N = (float)kGh4W[0] * (r + t0) * D1 + (float)kGh4W[1] * (r + t1) * D0;
-  return 2.0f * y * (float)kInvPi * N / (D0 * D1);
+  return 2.0f * y * (float)kInvPi * N * __builtin_amdgcn_rcpf(D0 * D1);
 }
 
 // Trapezoid + residue, |z| < 8.
@@ -204,7 +206,7 @@ __device__ inline float K_cr32(float x, float y) {
       const float t = n * h;
       s += gh_pair32(r, xmy, fxy, t * t, kCr32E0[n - 1]);
     }
-    s = 2.0f * y * s + y / r;
+    s = fmaf(2.0f * y, s, y * __builtin_amdgcn_rcpf(r));
   }
   float K = s * h * (float)kInvPi;
   if (y < (float)kPi / h) {
@@ -216,7 +218,7 @@ __device__ inline float K_cr32(float x, float y) {
     sincospif(2.0f * x / h, &st, &ct);
     sincosf(2.0f * x * y, &s2, &c2);
     const float cphi = ct * c2 + st * s2;
-    K += 2.0f * M * (sg * cphi + rho * c2) / (1.0f + 2.0f * sg * rho * ct + rho * rho);
+    K += 2.0f * M * (sg * cphi + rho * c2) * __builtin_amdgcn_rcpf(1.0f + 2.0f * sg * rho * ct + rho * rho);
   }
   return K;
 }
