@@ -57,6 +57,12 @@ constexpr float kF32S1Coef = 164062.5f;  // 1.05 / 6.4e-6
 __device__ __forceinline__ float s1_r2_min(float y2) {
   return fmaxf(kF32Gh2R2, sqrtf(kF32S1Coef * fmaf(2.0f, y2, 1.5f)));
 }
+// FP64 mode: the same form where 1.05 (3/2 + 2 y^2)/|z|^4 <= 1e-11 (|z| >= 630
+// for small y, the two-node form's 1.04e-11 at |z| = 700).
+constexpr double kFp64S1Coef = 1.05e11;   // 1.05 / 1e-11
+__device__ __forceinline__ double s1_r2_min64(double y2) {
+  return sqrt(kFp64S1Coef * fma(2.0, y2, 1.5));
+}
 
 // Gauss-Hermite positive nodes / weights (numpy.polynomial.hermite.hermgauss).
 constexpr double kGh2T = 0.7071067811865475, kGh2W = 0.8862269254527579;

@@ -71,12 +71,15 @@ __device__ __forceinline__ double rcp64_1nr(double d) {
 __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x, y); }
 
 // Record of the FP64 line sum as staged in LDS (one per (layer, line)); the
-// queue drain reads y and A from the global records.
+// queue drain reads y and A from the global records.  hw1, hw2, hw4: |x| at
+// the middle of a thread's run above which all P points are in the shifted
+// one-node, two-node and four-node regions.  The two-node form's y^2 - 1/2
+// and 2 y^2 are c1 + 1 and 2 c1 + 3 (exact).
 struct alignas(16) Rec64 {
   double c, s;        // shifted centre, sqrt(ln2)/alpha_D
-  double hw2, hw4;    // sqrt(max(700^2 - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
-  double hs, cw;      // step*s; y^2 - 1/2
-  double d2, ay;      // 2 y^2; A y/sqrt(pi)
+  double hw1, hw2;    // sqrt(max(s1_r2_min64(y^2) - y^2, 0)), sqrt(max(700^2 - y^2, 0)), + (P-1)/2 hs
+  double hw4, hs;     // sqrt(max(35^2 - y^2, 0)) + (P-1)/2 hs; step*s
+  double c1, ay;      // y^2 - 3/2; A y/sqrt(pi)
 };
 // 4 workgroups per CU need under 40 KB of LDS each: 16 KB of records, 16 KB
 // of FP64 partial sums and this many 16-bit queue entries.
@@ -84,8 +87,8 @@ constexpr int kQueueCap64 = 3840;
 
 // FP64 line sum.  Same tiling as the mixed kernel: per record and thread
 // the middle x of the thread's P points decides, for all of them at once,
-// between the 2-node form (|z| >= 700), the 4-node form (|z| >= 35) and the
-// near-centre LDS queue, which the whole workgroup drains at the end of the
+// between the shifted one-node form (faddeeva.h, error <= 1e-11), the 2-node
+// form (|z| >= 700), the 4-node form (|z| >= 35) and the near-centre LDS queue, which the whole workgroup drains at the end of the
 // chunk with the full FP64 dispatch (8-node form and trapezoid+residue),
 // adding the results into the owner's sums with LDS FP64 atomics.
 template <int P>
@@ -147,9 +150,10 @@ void k_voigt_sum_fp64(const SumArgs a) {
         Rec64 R;
         const double y2 = y * y, hs = a.step * sv;
         R.c = c; R.s = sv; R.hs = hs;
+        R.hw1 = fma(half, hs, sqrt(fmax(s1_r2_min64(y2) - y2, 0.0)));
         R.hw2 = fma(half, hs, sqrt(fmax(kFp64Gh2R2 - y2, 0.0)));
         R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
-        R.cw = y2 - 0.5; R.d2 = 2.0 * y2; R.ay = am * y * kInvSqrtPi;
+        R.c1 = y2 - 1.5; R.ay = am * y * kInvSqrtPi;
         s_rec[threadIdx.x] = R;
       }
       __syncthreads();
@@ -159,10 +163,37 @@ void k_voigt_sum_fp64(const SumArgs a) {
         const double rc = R.c, rsv = R.s;
         const double xm = (nu_m - rc) * rsv;
         const double ax = fabs(xm);
-        if (ax >= R.hw2) {
-          // two-node form (|z| >= 700 at every point); points step from the
-          // first one by integer multiples of hs (inline constants of v_fma_f64)
-          const double hs = R.hs, cw = R.cw, d2 = R.d2, ay = R.ay;
+        if (ax >= R.hw1) {
+          // shifted one-node form, A K = ay / D, D = x^2 + y^2 - 3/2; points step
+          // from the first one by integer multiples of hs (inline constants of
+          // v_fma_f64); the eight quotients share one v_rcp_f64 (Montgomery's
+          // batch inversion: D >= 3.9e5 and a product of eight stays far
+          // inside the FP64 range)
+          static_assert(P == 8, "one batch of eight points");
+          const double hs = R.hs, c1 = R.c1;
+          const double x0 = fma(-half, hs, xm);
+          double D[P];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const double x = fma((double)j, hs, x0);
+            D[j] = fma(x, x, c1);
+          }
+          const double p01 = D[0] * D[1], p23 = D[2] * D[3], p45 = D[4] * D[5], p67 = D[6] * D[7];
+          const double p03 = p01 * p23, p47 = p45 * p67;
+          const double r = R.ay * rcp64_1nr(p03 * p47);     // ay / (D0 ... D7)
+          const double r03 = r * p47, r47 = r * p03;
+          const double r01 = r03 * p23, r23 = r03 * p01, r45 = r47 * p67, r67 = r47 * p45;
+          lay[0] = fma(r01, D[1], lay[0]);
+          lay[1] = fma(r01, D[0], lay[1]);
+          lay[2] = fma(r23, D[3], lay[2]);
+          lay[3] = fma(r23, D[2], lay[3]);
+          lay[4] = fma(r45, D[5], lay[4]);
+          lay[5] = fma(r45, D[4], lay[5]);
+          lay[6] = fma(r67, D[7], lay[6]);
+          lay[7] = fma(r67, D[6], lay[7]);
+        } else if (ax >= R.hw2) {
+          // two-node form (|z| >= 700 at every point)
+          const double hs = R.hs, cw = R.c1 + 1.0, d2 = fma(2.0, R.c1, 3.0), ay = R.ay;
           const double x0 = fma(-half, hs, xm);
           double w[P], D[P];
 #pragma unroll
@@ -186,7 +217,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
           }
         } else if (ax >= R.hw4) {
           // four-node form (|z| >= 35 at every point); A (2y/pi) W = ay (2/sqrt(pi)) W
-          const double hs = R.hs, y2 = 0.5 * R.d2, fy2 = 2.0 * R.d2;
+          const double hs = R.hs, y2 = R.c1 + 1.5, fy2 = 4.0 * y2;
           const double w0 = R.ay * (2.0 * kInvSqrtPi * kGh4W[0]);
           const double w1 = R.ay * (2.0 * kInvSqrtPi * kGh4W[1]);
           const double x0 = fma(-half, hs, xm);

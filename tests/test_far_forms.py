@@ -96,6 +96,23 @@ def test_shifted_one_node_error_model():
     assert np.any(np.abs(unshifted / k - 1.0) > bound)
 
 
+def test_shifted_one_node_fp64_tier():
+    """FP64 mode: where |z|^2 >= sqrt(1.05e11 (2 y^2 + 3/2)) (faddeeva.h
+    s1_r2_min64) the shifted one-node form is within 1e-11 of K."""
+    rng = np.random.default_rng(21)
+    n = 400000
+    y = np.exp(rng.uniform(math.log(1e-4), math.log(300.0), n))
+    r_min = np.sqrt(np.sqrt(1.05e11 * (2.0 * y * y + 1.5)))
+    r = r_min * np.exp(rng.uniform(0.0, math.log(50.0), n))
+    r[: n // 10] = r_min[: n // 10]
+    ok = r > y
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    err = np.abs(k_s1(x, y) / k_series(x, y) - 1.0)
+    assert err.max() <= 1e-11
+    assert err[r < 1.01 * r_min[ok]].max() > 2e-12
+
+
 def test_two_node_within_bound_at_25():
     rng = np.random.default_rng(9)
     n = 200000
