@@ -59,13 +59,6 @@ __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
 
 // ------------------------------------------------------------------ FP64 --
 
-// v_rcp_f64 (relative error about 2^-23) and one Newton step: error about
-// 2^-46, far below the 1e-9 target of the FP64 mode.
-__device__ __forceinline__ double rcp64_1nr(double d) {
-  const double r = __builtin_amdgcn_rcp(d);
-  return fma(r, fma(-d, r, 1.0), r);
-}
-
 // The full FP64 dispatch, called (not inlined) from the rare near-centre
 // paths so that its registers do not crowd the far-field loop.
 __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x, y); }
