@@ -143,8 +143,8 @@ struct VjpArgs {
 // The coefficients below are those combinations' weights; dx/dT = -x/(2T)
 // multiplies S4, dx/dv = bx multiplies S3.  Shifted one-node form (r = 1/D1):
 // K = ay r, dK/dx = -2 ay x r^2, dK/dy = aq (r - d2 r^2), so with S1 = sum W r,
-// S2 = sum W r^2, S3 = sum W x r^2, S4 = sum W x^2 r^2 the same coefficients
-// apply as tF S1 - tX2 S4 - tD S2 and vF S1 - vX S3 - vD S2.
+// S2 = sum W r^2, S3 = sum W x r^2, S4 = sum W x^2 r^2 = S1 - c1 S2 the same
+// coefficients apply as tF S1 - tX2 S4 - tD S2 and vF S1 - vX S3 - vD S2.
 struct alignas(16) VjpFar {
   double c;
   float s, hs;
@@ -293,19 +293,20 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
         float acc_t, acc_v;
         if (x0 >= R.hw1 || xb <= -R.hw1) {
           const float c1 = R.c1;
-          float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f;
+          float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, hs, x0);
             const float r = __builtin_amdgcn_rcpf(fmaf(x, x, c1));
             const float q = WR[j] * r;
             const float q2 = q * r;
-            const float xq2 = x * q2;
             S1 += q;
             S2 += q2;
-            S3 += xq2;
-            S4 = fmaf(x, xq2, S4);
+            S3 = fmaf(x, q2, S3);
           }
+          // x^2 r^2 = r - c1 r^2, so sum W x^2 r^2 = S1 - c1 S2; its rounding error
+          // is at most that of S1, the size of the tF S1 term's own
+          const float S4 = fmaf(-c1, S2, S1);
           acc_t = fmaf(R.tF, S1, -fmaf(R.tX2, S4, R.tD * S2));
           acc_v = fmaf(R.vF, S1, -fmaf(R.vX, S3, R.vD * S2));
         } else if (x0 >= R.hw2 || xb <= -R.hw2) {
