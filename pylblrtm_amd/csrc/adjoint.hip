@@ -148,7 +148,7 @@ struct VjpArgs {
 struct alignas(16) VjpFar {
   double c;
   float s, hs;
-  float c1, hw1, hw2, ay;    // y^2 - 3/2, |x| thresholds (one-node, two-node), A y/sqrt(pi)
+  float c1, hw1, hw2, ay;    // y^2 - 3/2, mid-run |x| thresholds (one-node, two-node), A y/sqrt(pi)
   float tF, tX2, tD, tQ;     // temperature: weights of S1, S4, S2, S5
   float vF, vX, vD, vQ;      // mixing ratio: weights of S1, S3, S2, S5
 };
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
-  const double nu_l = grid_nu(a.nu_min, a.step, i0);
+  const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
   const float stepf = (float)a.step;
   const MixedRec* __restrict__ recs = a.rec.mix;
   const int n_gas = a.n_gases;
@@ -252,8 +252,10 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       const float cty = dyt * aq, cvy = dyv * aq;
       f.c = c; f.s = s; f.hs = stepf * s;
       f.c1 = y2 - 1.5f;
-      f.hw1 = sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
-      f.hw2 = sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
+      // thresholds on |x| at the middle of a thread's run of P points
+      const float half_run = 0.5f * (float)(P - 1) * f.hs;
+      f.hw1 = half_run + sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
+      f.hw2 = half_run + sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
       f.tF = at * ay + cty; f.tX2 = 2.0f * ax_t * ay; f.tD = cty * d2; f.tQ = -2.0f * d2 * cty;
       f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
       n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
@@ -282,21 +284,22 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
     for (int g = 0; g < n_gas; ++g) {
       int len = 0;
       for (int w = 0; w < kWaves; ++w) len += s_wcnt[w][g];
-      const int q1 = q0 + len;
+      // uniform across the workgroup: keep the record loop on the scalar unit
+      const int q1 = q0 + __builtin_amdgcn_readfirstlane(len);
       float ft = 0.0f, fv = 0.0f;
       double dv = 0.0;
       for (int qi = q0; qi < q1; ++qi) {
         const VjpFar& R = s_far[qi];
-        const float x0 = (float)(nu_l - R.c) * R.s;
+        // x at the middle of the run; point j at xm + (j - (P-1)/2) hs
+        const float xm = (float)(nu_m - R.c) * R.s;
         const float hs = R.hs;
-        const float xb = fmaf((float)(P - 1), hs, x0);
         float acc_t, acc_v;
-        if (x0 >= R.hw1 || xb <= -R.hw1) {
+        if (fabsf(xm) >= R.hw1) {
           const float c1 = R.c1;
           float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j, hs, x0);
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
             const float r = __builtin_amdgcn_rcpf(fmaf(x, x, c1));
             const float q = WR[j] * r;
             const float q2 = q * r;
@@ -309,12 +312,12 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
           const float S4 = fmaf(-c1, S2, S1);
           acc_t = fmaf(R.tF, S1, -fmaf(R.tX2, S4, R.tD * S2));
           acc_v = fmaf(R.vF, S1, -fmaf(R.vX, S3, R.vD * S2));
-        } else if (x0 >= R.hw2 || xb <= -R.hw2) {
+        } else if (fabsf(xm) >= R.hw2) {
           const float cw = R.c1 + 1.0f, d2 = fmaf(2.0f, R.c1, 3.0f);
           float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f, S5 = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j, hs, x0);
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
             const float w = fmaf(x, x, cw);
             const float D = fmaf(w, w, d2);
             const float rD = __builtin_amdgcn_rcpf(D);
@@ -335,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
           acc_t = acc_v = 0.0f;
 #pragma unroll 1
           for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j, hs, x0);
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
             float K, Kx, Ky;
             kgrad_mixed(x, N.y, K, Kx, Ky);
             const float AK = N.a * K;
