@@ -70,10 +70,12 @@ __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x
 // and 2 y^2 are c1 + 1 and 2 c1 + 3 (exact).
 struct alignas(16) Rec64 {
   double c, s;        // shifted centre, sqrt(ln2)/alpha_D
-  double hw1, hw2;    // sqrt(max(s1_r2_min64(y^2) - y^2, 0)), sqrt(max(700^2 - y^2, 0)), + (P-1)/2 hs
-  double hw4, hs;     // sqrt(max(35^2 - y^2, 0)) + (P-1)/2 hs; step*s
+  double hw1, hs;     // sqrt(max(s1_r2_min64(y^2) - y^2, 0)) + (P-1)/2 hs; step*s
   double c1, ay;      // y^2 - 3/2; A y/sqrt(pi)
+  double hw2, hw4;    // sqrt(max(700^2 - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
 };
+static_assert(sizeof(Rec64) == 64, "four 16-byte LDS reads per record");
+typedef __attribute__((address_space(3))) Rec64 lds_rec64;
 // 4 workgroups per CU need under 40 KB of LDS each: 16 KB of records, 16 KB
 // of FP64 partial sums and this many 16-bit queue entries.
 constexpr int kQueueCap64 = 3840;
@@ -151,19 +153,26 @@ void k_voigt_sum_fp64(const SumArgs a) {
       }
       __syncthreads();
       const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
-      for (int q = 0; q < ncnt; ++q) {
-        const Rec64& R = s_rec[q];
-        const double rc = R.c, rsv = R.s;
+      // Records are read from LDS as 16-byte pieces: (c, s) and (hw1, hs) two
+      // records ahead into their own registers (two records per iteration),
+      // (c1, ay) at the record and (hw2, hw4) only off the one-node branch.
+      const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)(lds_rec64*)s_rec;
+      auto f64 = [](unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); };
+      auto record = [&](const int q, const u32x4 h0, const u32x4 h1,
+                        const volatile lds_u32x4* const more) {
+        const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
+        const double hw1 = f64(h1.x, h1.y), hs = f64(h1.z, h1.w);
         const double xm = (nu_m - rc) * rsv;
         const double ax = fabs(xm);
-        if (ax >= R.hw1) {
+        const u32x4 h2 = more[0];
+        const double c1 = f64(h2.x, h2.y), ay = f64(h2.z, h2.w);
+        if (ax >= hw1) {
           // shifted one-node form, A K = ay / D, D = x^2 + y^2 - 3/2; points step
           // from the first one by integer multiples of hs (inline constants of
           // v_fma_f64); the eight quotients share one v_rcp_f64 (Montgomery's
           // batch inversion: D >= 3.9e5 and a product of eight stays far
           // inside the FP64 range)
           static_assert(P == 8, "one batch of eight points");
-          const double hs = R.hs, c1 = R.c1;
           const double x0 = fma(-half, hs, xm);
           double D[P];
 #pragma unroll
@@ -173,7 +182,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
           }
           const double p01 = D[0] * D[1], p23 = D[2] * D[3], p45 = D[4] * D[5], p67 = D[6] * D[7];
           const double p03 = p01 * p23, p47 = p45 * p67;
-          const double r = R.ay * rcp64_1nr(p03 * p47);     // ay / (D0 ... D7)
+          const double r = ay * rcp64_1nr(p03 * p47);     // ay / (D0 ... D7)
           const double r03 = r * p47, r47 = r * p03;
           const double r01 = r03 * p23, r23 = r03 * p01, r45 = r47 * p67, r67 = r47 * p45;
           lay[0] = fma(r01, D[1], lay[0]);
@@ -184,68 +193,83 @@ void k_voigt_sum_fp64(const SumArgs a) {
           lay[5] = fma(r45, D[4], lay[5]);
           lay[6] = fma(r67, D[7], lay[6]);
           lay[7] = fma(r67, D[6], lay[7]);
-        } else if (ax >= R.hw2) {
-          // two-node form (|z| >= 700 at every point)
-          const double hs = R.hs, cw = R.c1 + 1.0, d2 = fma(2.0, R.c1, 3.0), ay = R.ay;
-          const double x0 = fma(-half, hs, xm);
-          double w[P], D[P];
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const double x = fma((double)j, hs, x0);
-            w[j] = fma(x, x, cw);
-            D[j] = fma(w[j], w[j], d2);
-          }
-          // four reciprocals from one v_rcp_f64 (Montgomery's batch inversion;
-          // D in [2.4e11, ~1e19] here, so a product of four cannot overflow)
-          static_assert(P % 4 == 0, "batches of four points");
-#pragma unroll
-          for (int g = 0; g < P; g += 4) {
-            const double p01 = D[g] * D[g + 1], p23 = D[g + 2] * D[g + 3];
-            const double r = rcp64_1nr(p01 * p23);
-            const double q01 = r * p23, q23 = r * p01;
-            lay[g] = fma(fma(ay, w[g], ay), q01 * D[g + 1], lay[g]);
-            lay[g + 1] = fma(fma(ay, w[g + 1], ay), q01 * D[g], lay[g + 1]);
-            lay[g + 2] = fma(fma(ay, w[g + 2], ay), q23 * D[g + 3], lay[g + 2]);
-            lay[g + 3] = fma(fma(ay, w[g + 3], ay), q23 * D[g + 2], lay[g + 3]);
-          }
-        } else if (ax >= R.hw4) {
-          // four-node form (|z| >= 35 at every point); A (2y/pi) W = ay (2/sqrt(pi)) W
-          const double hs = R.hs, y2 = R.c1 + 1.5, fy2 = 4.0 * y2;
-          const double w0 = R.ay * (2.0 * kInvSqrtPi * kGh4W[0]);
-          const double w1 = R.ay * (2.0 * kInvSqrtPi * kGh4W[1]);
-          const double x0 = fma(-half, hs, xm);
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const double x = fma((double)j, hs, x0);
-            const double x2 = x * x;
-            const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
-            const double u0 = xmy - t0sq, u1 = xmy - t1sq;
-            const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
-            const double N = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
-            lay[j] = fma(N, rcp64_1nr(D0 * D1), lay[j]);
-          }
-        } else if (live) {
-          // append (q, thread) with one LDS atomic per wavefront
-          const uint64_t msk = __ballot(1);
-          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-          int qb = 0;
-          if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(msk));
-          qb = __builtin_amdgcn_readfirstlane(qb);
-          const int pos = qb + (int)rank;
-          if (pos < kQueueCap64) {
-            s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
-          } else {  // queue full: evaluate here
-            const int64_t r = kb + base + q;
-            const double y = ry[r], am = ra[r];
-#pragma unroll 1
+        } else {
+          const u32x4 h3 = more[1];
+          const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
+          if (ax >= hw2) {
+            // two-node form (|z| >= 700 at every point)
+            const double cw = c1 + 1.0, d2 = fma(2.0, c1, 3.0);
+            const double x0 = fma(-half, hs, xm);
+            double w[P], D[P];
+  #pragma unroll
             for (int j = 0; j < P; ++j) {
-              const double v =
-                  am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - rc) * rsv, y);
-#pragma unroll
-              for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
+              const double x = fma((double)j, hs, x0);
+              w[j] = fma(x, x, cw);
+              D[j] = fma(w[j], w[j], d2);
+            }
+            // four reciprocals from one v_rcp_f64 (Montgomery's batch inversion;
+            // D in [2.4e11, ~1e19] here, so a product of four cannot overflow)
+            static_assert(P % 4 == 0, "batches of four points");
+  #pragma unroll
+            for (int g = 0; g < P; g += 4) {
+              const double p01 = D[g] * D[g + 1], p23 = D[g + 2] * D[g + 3];
+              const double r = rcp64_1nr(p01 * p23);
+              const double q01 = r * p23, q23 = r * p01;
+              lay[g] = fma(fma(ay, w[g], ay), q01 * D[g + 1], lay[g]);
+              lay[g + 1] = fma(fma(ay, w[g + 1], ay), q01 * D[g], lay[g + 1]);
+              lay[g + 2] = fma(fma(ay, w[g + 2], ay), q23 * D[g + 3], lay[g + 2]);
+              lay[g + 3] = fma(fma(ay, w[g + 3], ay), q23 * D[g + 2], lay[g + 3]);
+            }
+          } else if (ax >= hw4) {
+            // four-node form (|z| >= 35 at every point); A (2y/pi) W = ay (2/sqrt(pi)) W
+            const double y2 = c1 + 1.5, fy2 = 4.0 * y2;
+            const double w0 = ay * (2.0 * kInvSqrtPi * kGh4W[0]);
+            const double w1 = ay * (2.0 * kInvSqrtPi * kGh4W[1]);
+            const double x0 = fma(-half, hs, xm);
+  #pragma unroll
+            for (int j = 0; j < P; ++j) {
+              const double x = fma((double)j, hs, x0);
+              const double x2 = x * x;
+              const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+              const double u0 = xmy - t0sq, u1 = xmy - t1sq;
+              const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
+              const double N = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
+              lay[j] = fma(N, rcp64_1nr(D0 * D1), lay[j]);
+            }
+          } else if (live) {
+            // append (q, thread) with one LDS atomic per wavefront
+            const uint64_t msk = __ballot(1);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+            int qb = 0;
+            if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(msk));
+            qb = __builtin_amdgcn_readfirstlane(qb);
+            const int pos = qb + (int)rank;
+            if (pos < kQueueCap64) {
+              s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
+            } else {  // queue full: evaluate here
+              const int64_t r = kb + base + q;
+              const double y = ry[r], am = ra[r];
+  #pragma unroll 1
+              for (int j = 0; j < P; ++j) {
+                const double v =
+                    am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - rc) * rsv, y);
+  #pragma unroll
+                for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
+              }
             }
           }
+        }
+      };
+      u32x4 ha0 = rb[0], ha1 = rb[1], hb0 = rb[4], hb1 = rb[5];
+      const volatile lds_u32x4* pr = rb;
+      asm volatile("" : "+v"(pr));
+      for (int q = 0; q < ncnt; q += 2, pr += 8) {
+        record(q, ha0, ha1, pr + 2);
+        if (q + 2 < ncnt) { ha0 = pr[8]; ha1 = pr[9]; }
+        if (q + 1 < ncnt) {
+          record(q + 1, hb0, hb1, pr + 6);
+          if (q + 3 < ncnt) { hb0 = pr[12]; hb1 = pr[13]; }
         }
       }
       __syncthreads();
