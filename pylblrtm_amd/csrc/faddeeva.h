@@ -64,6 +64,18 @@ __device__ __forceinline__ double s1_r2_min64(double y2) {
   return sqrt(kFp64S1Coef * fma(2.0, y2, 1.5));
 }
 
+// Second-order shifted form (FP64 line sum), r = |z|^2:
+//   K ~= y r / (sqrt(pi) (r^2 - 3/2 r + 2 y^2 - 3/2)),
+// i.e. y / (sqrt(pi) (r - 3/2 + (2y^2 - 3/2)/r)): the shift's own correction
+// makes the first three orders of the expansion above exact, leaving
+// (9 y^2 - 21/4)/|z|^6 + O(|z|^-8); measured |err| <= 1.1 (21/4 + 9 y^2)/|z|^6
+// for |z| >= 20 against the series.  Used where that is <= 1e-11 (|z| >= 91
+// for small y, 368 at y = 50).
+constexpr double kFp64S2Coef = 1.1e11;    // 1.1 / 1e-11
+__device__ __forceinline__ double s2_r2_min64(double y2) {
+  return cbrt(kFp64S2Coef * fma(9.0, y2, 5.25));
+}
+
 // Gauss-Hermite positive nodes / weights (numpy.polynomial.hermite.hermgauss).
 constexpr double kGh2T = 0.7071067811865475, kGh2W = 0.8862269254527579;
 constexpr double kGh4T[2] = {0.5246476232752904, 1.6506801238857847};

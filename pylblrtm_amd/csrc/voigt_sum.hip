@@ -72,7 +72,7 @@ struct alignas(16) Rec64 {
   double c, s;        // shifted centre, sqrt(ln2)/alpha_D
   double hw1, hs;     // sqrt(max(s1_r2_min64(y^2) - y^2, 0)) + (P-1)/2 hs; step*s
   double c1, ay;      // y^2 - 3/2; A y/sqrt(pi)
-  double hw2, hw4;    // sqrt(max(700^2 - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
+  double hw2, hw4;    // sqrt(max(s2_r2_min64(y^2) - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
 };
 static_assert(sizeof(Rec64) == 64, "four 16-byte LDS reads per record");
 typedef __attribute__((address_space(3))) Rec64 lds_rec64;
@@ -82,8 +82,9 @@ constexpr int kQueueCap64 = 3840;
 
 // FP64 line sum.  Same tiling as the mixed kernel: per record and thread
 // the middle x of the thread's P points decides, for all of them at once,
-// between the shifted one-node form (faddeeva.h, error <= 1e-11), the 2-node
-// form (|z| >= 700), the 4-node form (|z| >= 35) and the near-centre LDS queue, which the whole workgroup drains at the end of the
+// between the shifted one-node form and its second-order refinement
+// (faddeeva.h, error <= 1e-11), the 4-node form (|z| >= 35) and the
+// near-centre LDS queue, which the whole workgroup drains at the end of the
 // chunk with the full FP64 dispatch (8-node form and trapezoid+residue),
 // adding the results into the owner's sums with LDS FP64 atomics.
 template <int P>
@@ -146,7 +147,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
         const double y2 = y * y, hs = a.step * sv;
         R.c = c; R.s = sv; R.hs = hs;
         R.hw1 = fma(half, hs, sqrt(fmax(s1_r2_min64(y2) - y2, 0.0)));
-        R.hw2 = fma(half, hs, sqrt(fmax(kFp64Gh2R2 - y2, 0.0)));
+        R.hw2 = fma(half, hs, sqrt(fmax(s2_r2_min64(y2) - y2, 0.0)));
         R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
         R.c1 = y2 - 1.5; R.ay = am * y * kInvSqrtPi;
         s_rec[threadIdx.x] = R;
@@ -197,29 +198,33 @@ void k_voigt_sum_fp64(const SumArgs a) {
           const u32x4 h3 = more[1];
           const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
           if (ax >= hw2) {
-            // two-node form (|z| >= 700 at every point)
-            const double cw = c1 + 1.0, d2 = fma(2.0, c1, 3.0);
+            // second-order shifted form (faddeeva.h), A K = ay r / D with
+            // r = x^2 + y^2, D = r^2 - 3/2 r + 2 y^2 - 3/2; the eight quotients
+            // by batch inversion as above (D >= 6.9e7 here; a product of
+            // eight stays inside the FP64 range for any r below the one-node
+            // threshold)
+            const double y2 = c1 + 1.5, cc = fma(2.0, c1, 1.5);
             const double x0 = fma(-half, hs, xm);
-            double w[P], D[P];
-  #pragma unroll
+            double rr[P], D[P];
+#pragma unroll
             for (int j = 0; j < P; ++j) {
               const double x = fma((double)j, hs, x0);
-              w[j] = fma(x, x, cw);
-              D[j] = fma(w[j], w[j], d2);
+              rr[j] = fma(x, x, y2);
+              D[j] = fma(rr[j], rr[j] - 1.5, cc);
             }
-            // four reciprocals from one v_rcp_f64 (Montgomery's batch inversion;
-            // D in [2.4e11, ~1e19] here, so a product of four cannot overflow)
-            static_assert(P % 4 == 0, "batches of four points");
-  #pragma unroll
-            for (int g = 0; g < P; g += 4) {
-              const double p01 = D[g] * D[g + 1], p23 = D[g + 2] * D[g + 3];
-              const double r = rcp64_1nr(p01 * p23);
-              const double q01 = r * p23, q23 = r * p01;
-              lay[g] = fma(fma(ay, w[g], ay), q01 * D[g + 1], lay[g]);
-              lay[g + 1] = fma(fma(ay, w[g + 1], ay), q01 * D[g], lay[g + 1]);
-              lay[g + 2] = fma(fma(ay, w[g + 2], ay), q23 * D[g + 3], lay[g + 2]);
-              lay[g + 3] = fma(fma(ay, w[g + 3], ay), q23 * D[g + 2], lay[g + 3]);
-            }
+            const double p01 = D[0] * D[1], p23 = D[2] * D[3], p45 = D[4] * D[5], p67 = D[6] * D[7];
+            const double p03 = p01 * p23, p47 = p45 * p67;
+            const double r = ay * rcp64_1nr(p03 * p47);
+            const double r03 = r * p47, r47 = r * p03;
+            const double r01 = r03 * p23, r23 = r03 * p01, r45 = r47 * p67, r67 = r47 * p45;
+            lay[0] = fma(r01 * D[1], rr[0], lay[0]);
+            lay[1] = fma(r01 * D[0], rr[1], lay[1]);
+            lay[2] = fma(r23 * D[3], rr[2], lay[2]);
+            lay[3] = fma(r23 * D[2], rr[3], lay[3]);
+            lay[4] = fma(r45 * D[5], rr[4], lay[4]);
+            lay[5] = fma(r45 * D[4], rr[5], lay[5]);
+            lay[6] = fma(r67 * D[7], rr[6], lay[6]);
+            lay[7] = fma(r67 * D[6], rr[7], lay[7]);
           } else if (ax >= hw4) {
             // four-node form (|z| >= 35 at every point); A (2y/pi) W = ay (2/sqrt(pi)) W
             const double y2 = c1 + 1.5, fy2 = 4.0 * y2;

@@ -113,6 +113,46 @@ def test_shifted_one_node_fp64_tier():
     assert err[r < 1.01 * r_min[ok]].max() > 2e-12
 
 
+def k_s2(x, y):
+    r = x * x + y * y
+    return y * r / (SQRT_PI * (r * r - 1.5 * r + 2.0 * y * y - 1.5))
+
+
+def test_second_order_shifted_form_error_model():
+    """|err| <= 1.1 (21/4 + 9 y^2)/|z|^6 (+ rounding) for |z| >= 20; the
+    one-node form (no 1/r correction) fails it."""
+    rng = np.random.default_rng(8)
+    n = 400000
+    y = np.exp(rng.uniform(math.log(1e-4), math.log(1000.0), n))
+    r = np.exp(rng.uniform(math.log(20.0), math.log(5000.0), n))
+    ok = r > y * 1.0001
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    k = k_series(x, y)
+    bound = 1.1 * (5.25 + 9.0 * y * y) / r ** 6 + 2e-15
+    assert np.all(np.abs(k_s2(x, y) / k - 1.0) <= bound)
+    assert np.any(np.abs(k_s1(x, y) / k - 1.0) > bound)
+
+
+def test_second_order_fp64_tier():
+    """Where the FP64 line sum uses it (|z|^2 >= cbrt(1.1e11 (9 y^2 + 21/4)),
+    faddeeva.h s2_r2_min64) the second-order form is within 1e-11, and that
+    radius lies below the one-node form's for every y."""
+    rng = np.random.default_rng(13)
+    n = 400000
+    y = np.exp(rng.uniform(math.log(1e-4), math.log(1000.0), n))
+    r_min = np.sqrt(np.cbrt(1.1e11 * (9.0 * y * y + 5.25)))
+    assert np.all(r_min ** 2 < np.sqrt(1.05e11 * (2.0 * y * y + 1.5)))
+    r = r_min * np.exp(rng.uniform(0.0, math.log(20.0), n))
+    r[: n // 10] = r_min[: n // 10]
+    ok = r > y
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    err = np.abs(k_s2(x, y) / k_series(x, y) - 1.0)
+    assert err.max() <= 1e-11
+    assert err[r < 1.01 * r_min[ok]].max() > 1e-12
+
+
 def test_two_node_within_bound_at_25():
     rng = np.random.default_rng(9)
     n = 200000
