@@ -219,6 +219,34 @@ def test_dense_lines_queue_overflow_vs_oracle():
     assert relm.max() < 2e-5
 
 
+def test_extreme_line_widths_vs_oracle():
+    """Lorentz widths from 1e-3x to 100x the usual (y from 1e-4 to 1e4 over
+    10 layers): every far-field tier of both precisions (shifted one-node,
+    the two-node band of broad lines, the FP64 second-order and four-node
+    tiers, the work queue) and each boundary between them falls inside the
+    window for some line."""
+    say("case: 300 lines, widths x1e-3..x100, 10 layers, 700-702 @ 0.001")
+    rng = np.random.default_rng(23)
+    n = 300
+    lines = synth.random_lines(rng, n, 675.0, 727.0, 2)
+    mult = rng.choice([1e-3, 0.1, 1.0, 3.0, 10.0, 100.0], n)
+    lines["g_air"] = lines["g_air"] * mult
+    lines["g_self"] = lines["g_self"] * mult
+    sd = SpectralData(lines=lines, tips={(2, 1): synth.tips_table()}, molar_masses={(2, 1): 43.99})
+    p, t, _ = synth.levels(11)
+    lay = AtmosphericProfile(p, t, {"CO2": np.full(11, 4.15e-4)}).build_layers()
+    nu, od = _oracle_tau(sd, lay, 700.0, 702.0, 0.001, 25.0, 2.0)
+    prob = OpticalDepthProblem(sd, lay, 700.0, 702.0, 0.001, 25.0, 2.0)
+    o64 = prob.run("fp64", mus=(1.0,), want_layers=True)["optical_depth_layers"].cpu().numpy()
+    rel = np.abs(o64 - od) / od
+    say(f"  fp64 max rel tau per layer {rel.max():.2e}")
+    assert rel.max() < TAU_RTOL_FP64
+    om = prob.run("mixed", mus=(1.0,), want_layers=True)["optical_depth_layers"]
+    relm = np.abs(om.double().cpu().numpy() - od) / od
+    say(f"  mixed max rel tau per layer {relm.max():.2e}")
+    assert relm.max() < 2e-5
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_window_without_lines_and_zero_columns(precision):
     """No line reaches the window (the reference returns od = 0, forward.py:245-246),
