@@ -75,6 +75,14 @@ constexpr double kFp64S2Coef = 1.1e11;    // 1.1 / 1e-11
 __device__ __forceinline__ double s2_r2_min64(double y2) {
   return cbrt(kFp64S2Coef * fma(9.0, y2, 5.25));
 }
+// Mixed line sum: the same form where 1.1 (21/4 + 9 y^2)/|z|^6 <= 6.4e-6
+// (|z| >= 9.8 for small y); it reaches below the two-node form's |z| = 25 while
+// y^2 < kF32S2Y2, and there it replaces the two-node band.
+constexpr float kF32S2Coef = 171875.0f;   // 1.1 / 6.4e-6
+constexpr float kF32S2Y2 = 155.0f;        // s2 radius^2 = 625 at y^2 = 157.4
+__device__ __forceinline__ float s2_r2_min32(float y2) {
+  return cbrtf(kF32S2Coef * fmaf(9.0f, y2, 5.25f));
+}
 
 // Gauss-Hermite positive nodes / weights (numpy.polynomial.hermite.hermgauss).
 constexpr double kGh2T = 0.7071067811865475, kGh2W = 0.8862269254527579;

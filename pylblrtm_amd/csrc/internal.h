@@ -59,8 +59,9 @@ constexpr int kPointsMixed = 8;
 //        holds (faddeeva.h) iff |x| at the middle of the run is >= hw1,
 //   hs = step * s, c1 = y^2 - 3/2, ay = A y / sqrt(pi) (one-node form; the
 //        two-node form's y^2 - 1/2 = c1 + 1 and 2 y^2 = 2 c1 + 3 are exact),
-//   hw2 = sqrt(max(25^2 - y^2, 0)) + (kPointsMixed - 1)/2 * hs: the same for
-//        the two-node form (|z| >= 25),
+//   hw2 = the same for the middle tier: the second-order shifted form
+//        (faddeeva.h s2_r2_min32) when y^2 < kF32S2Y2, else the two-node form
+//        (|z| >= 25); the tier is chosen per record from c1,
 //   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D.
 // A skipped (zero-column) record has c = NaN, ay = a = 0.
 struct alignas(16) MixedRec {

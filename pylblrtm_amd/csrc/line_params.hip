@@ -67,7 +67,9 @@ __device__ __forceinline__ void store_mixed(const Records& r, int64_t i, double 
   m.ay = a * y * (float)kInvSqrtPi;
   const float half_run = 0.5f * (float)(kPointsMixed - 1) * m.hs;
   m.hw1 = half_run + sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
-  m.hw2 = half_run + sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
+  // the line sum picks the middle tier by the same test on c1
+  const float r2_mid = (m.c1 < kF32S2Y2 - 1.5f) ? s2_r2_min32(y2) : kF32Gh2R2;
+  m.hw2 = half_run + sqrtf(fmaxf(r2_mid - y2, 0.0f));
   m.y = y;
   m.a = a;
   m.pad0 = m.pad1 = 0.0f;

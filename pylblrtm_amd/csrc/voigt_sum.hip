@@ -438,15 +438,27 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             part[j] = fmaf(ay, __builtin_amdgcn_rcpf(D), part[j]);
           }
         } else if (fabsf(xm) >= __uint_as_float(h1.w)) {
-          // two-node form: K A = (A y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
-          const float cw = c1 + 1.0f, d2 = fmaf(2.0f, c1, 3.0f);
+          if (c1 < kF32S2Y2 - 1.5f) {   // per record: uniform across the wavefront
+            // second-order shifted form: K A = (A y/sqrt(pi)) r / (r^2 - 3/2 r + 2 y^2 - 3/2)
+            const float y2 = c1 + 1.5f, cc = fmaf(2.0f, c1, 1.5f);
 #pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
-            const float w = fmaf(x, x, cw);
-            const float D = fmaf(w, w, d2);
-            const float N = fmaf(ay, w, ay);
-            part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
+            for (int j = 0; j < P; ++j) {
+              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
+              const float r = fmaf(x, x, y2);
+              const float D = fmaf(r, r - 1.5f, cc);
+              part[j] = fmaf(ay * r, __builtin_amdgcn_rcpf(D), part[j]);
+            }
+          } else {
+            // two-node form: K A = (A y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
+            const float cw = c1 + 1.0f, d2 = fmaf(2.0f, c1, 3.0f);
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
+              const float w = fmaf(x, x, cw);
+              const float D = fmaf(w, w, d2);
+              const float N = fmaf(ay, w, ay);
+              part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
+            }
           }
         } else if (live) {
           // append (q, thread) with one LDS atomic per wavefront

@@ -153,6 +153,26 @@ def test_second_order_fp64_tier():
     assert err[r < 1.01 * r_min[ok]].max() > 1e-12
 
 
+def test_second_order_mixed_tier():
+    """Mixed line sum: for y^2 < 155 the middle tier is the second-order form
+    from |z|^2 = cbrt(171875 (9 y^2 + 21/4)) (faddeeva.h s2_r2_min32), inside
+    6.4e-6 and below the two-node form's |z| = 25."""
+    rng = np.random.default_rng(31)
+    n = 300000
+    y = np.exp(rng.uniform(math.log(1e-4), math.log(math.sqrt(155.0)), n)).astype(np.float32)
+    y2 = (y * y).astype(np.float32)
+    r2 = np.cbrt(np.float32(171875.0) * (np.float32(9.0) * y2 + np.float32(5.25))).astype(np.float64)
+    assert np.all(r2 < 625.0)
+    r = np.sqrt(r2) * np.exp(rng.uniform(0.0, math.log(3.0), n))
+    r[: n // 10] = np.sqrt(r2[: n // 10])
+    y = y.astype(np.float64)
+    ok = r > y
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    err = np.abs(k_s2(x, y) / k_series(x, y) - 1.0)
+    assert err.max() <= EPS_FAR
+
+
 def test_two_node_within_bound_at_25():
     rng = np.random.default_rng(9)
     n = 200000
