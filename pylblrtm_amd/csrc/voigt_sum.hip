@@ -14,9 +14,10 @@
 // the kernel writes exp(-tau/mu) for every requested mu.
 //
 // Region split.  Almost every evaluation is far from its line centre, where
-// a short rational form is accurate: in mixed mode the shifted one-node form
-// (4 VALU operations per point) and, closer in for broad lines, the two-node
-// Gauss-Hermite form; in FP64 the two- and four-node forms.  A thread decides
+// a short rational form is accurate: the shifted one-node form (4 VALU
+// operations per point in mixed mode) and, closer in, its second-order
+// refinement or (mixed, broad lines) the two-node Gauss-Hermite form, then in
+// FP64 the four-node form (faddeeva.h gives each form's error).  A thread decides
 // once per record which form holds for all of its P points (a compare of |x|
 // at the middle of its run with a per-record threshold); pairs (record,
 // thread) that need the full region dispatch of faddeeva.h go to an LDS work
@@ -66,8 +67,7 @@ __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x
 // Record of the FP64 line sum as staged in LDS (one per (layer, line)); the
 // queue drain reads y and A from the global records.  hw1, hw2, hw4: |x| at
 // the middle of a thread's run above which all P points are in the shifted
-// one-node, two-node and four-node regions.  The two-node form's y^2 - 1/2
-// and 2 y^2 are c1 + 1 and 2 c1 + 3 (exact).
+// one-node, second-order shifted and four-node regions.
 struct alignas(16) Rec64 {
   double c, s;        // shifted centre, sqrt(ln2)/alpha_D
   double hw1, hs;     // sqrt(max(s1_r2_min64(y^2) - y^2, 0)) + (P-1)/2 hs; step*s
@@ -353,7 +353,7 @@ template <int P, bool kSurrogate>
 __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ MixedRec s_rec[kChunk];
   // Near-centre work queue: (record, thread) pairs whose P points are not all
-  // in the two-node region are appended here instead of being evaluated by
+  // in one of the far regions are appended here instead of being evaluated by
   // a wavefront that is otherwise on the fast path, then drained densely by
   // the whole workgroup at the end of the chunk.  Results come back through
   // s_acc (LDS float atomics) into the owning thread's partial sums.
@@ -420,7 +420,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
       // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
       // keeps each a single ds_read_b128.  Three tiers per (record, thread):
-      // the shifted one-node form, the two-node form, and the work queue.
+      // the shifted one-node form, the middle tier (second-order shifted form
+      // for y^2 < kF32S2Y2, else two-node), and the work queue.
       const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
       const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
       auto record = [&](const int q, const u32x4 h0, const volatile lds_u32x4* const body) {
