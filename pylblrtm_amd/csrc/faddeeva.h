@@ -158,16 +158,18 @@ __device__ inline double K_cr64(double x, double y) {
   const double fr = q - floor(q);
   const bool half = !(fr >= 0.25 && fr <= 0.75);
   const double x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+  // One loop for both lattices (nodes (n + 1/2) h or n h, weights selected per
+  // lane; the integer lattice's n = 0 slot has weight 0), so a wavefront whose
+  // lanes chose differently runs it once.  Terms are added in the same order.
+  const double off = half ? 0.5 : 0.0;
   double s = 0.0;
-  if (half) {
 #pragma unroll
-    for (int n = 12; n >= 0; --n) s += gh_pair64(r, xmy, fxy, (n + 0.5) * h, kCr64E5[n]);
-    s *= 2.0 * y;
-  } else {
-#pragma unroll
-    for (int n = 12; n >= 1; --n) s += gh_pair64(r, xmy, fxy, n * h, kCr64E0[n - 1]);
-    s = fma(2.0 * y, s, y * rcp64_1nr(r));
+  for (int n = 12; n >= 0; --n) {
+    const double w = half ? kCr64E5[n] : (n >= 1 ? kCr64E0[n - 1] : 0.0);
+    s += gh_pair64(r, xmy, fxy, (n + off) * h, w);
   }
+  s *= 2.0 * y;
+  if (!half) s += y * rcp64_1nr(r);
   double K = s * h * kInvPi;
   if (y < kPi / h) {
     const double sg = half ? 1.0 : -1.0;
@@ -226,22 +228,18 @@ __device__ inline float K_cr32(float x, float y) {
   const float fr = q - floorf(q);
   const bool half = !(fr >= 0.25f && fr <= 0.75f);
   const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
+  // One loop for both lattices, as in K_cr64: half-lattice slots n = 4..0,
+  // integer-lattice slots n = 5..1 (the unused slot of each has weight 0).
+  const float off = half ? 0.5f : 0.0f;
   float s = 0.0f;
-  if (half) {
 #pragma unroll
-    for (int n = 4; n >= 0; --n) {
-      const float t = (n + 0.5f) * h;
-      s += gh_pair32(r, xmy, fxy, t * t, kCr32E5[n]);
-    }
-    s *= 2.0f * y;
-  } else {
-#pragma unroll
-    for (int n = 5; n >= 1; --n) {
-      const float t = n * h;
-      s += gh_pair32(r, xmy, fxy, t * t, kCr32E0[n - 1]);
-    }
-    s = fmaf(2.0f * y, s, y * __builtin_amdgcn_rcpf(r));
+  for (int n = 5; n >= 0; --n) {
+    const float w = half ? (n <= 4 ? kCr32E5[n] : 0.0f) : (n >= 1 ? kCr32E0[n - 1] : 0.0f);
+    const float t = (n + off) * h;
+    s += gh_pair32(r, xmy, fxy, t * t, w);
   }
+  s *= 2.0f * y;
+  if (!half) s += y * __builtin_amdgcn_rcpf(r);
   float K = s * h * (float)kInvPi;
   if (y < (float)kPi / h) {
     const float sg = half ? 1.0f : -1.0f;
