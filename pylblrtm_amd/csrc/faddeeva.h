@@ -247,7 +247,7 @@ __device__ inline float K_cr32(float x, float y) {
     const float rho = expf(-a * y);
     const float M = expf(fmaf(y, y - a, -x2));
     float st, ct, s2, c2;
-    sincospif(2.0f * x / h, &st, &ct);
+    sincospif(x * (2.0f / h), &st, &ct);   // 2x/h (1 ulp; no run-time division)
     sincosf(2.0f * x * y, &s2, &c2);
     const float cphi = ct * c2 + st * s2;
     K += 2.0f * M * (sg * cphi + rho * c2) * __builtin_amdgcn_rcpf(1.0f + 2.0f * sg * rho * ct + rho * rho);
@@ -285,7 +285,7 @@ __device__ inline void kgrad_pairs32(float x, float y, const float (&t2s)[N],
     const float t2 = t2s[k], c = cs[k];
     const float w = r - t2;
     const float d = 4.0f * t2 * y2;
-    const float rD = 1.0f / fmaf(w, w, d);
+    const float rD = __builtin_amdgcn_rcpf(fmaf(w, w, d));
     const float F = (w + 2.0f * t2) * rD;
     const float Fp = fmaf(4.0f * t2, t2 - x2, -w * w) * rD * rD;
     sf = fmaf(c, F, sf);
@@ -318,7 +318,7 @@ __device__ inline void kgrad_cr32(float x, float y, float& K, float& Kx, float& 
   const float x2 = x * x, y2 = y * y;
   if (!half) {
     const float c = h * (float)kInvPi;
-    const float rr = 1.0f / (x2 + y2);
+    const float rr = __builtin_amdgcn_rcpf(x2 + y2);
     K = fmaf(c * y, rr, K);
     Kx = fmaf(-2.0f * c * x * y, rr * rr, Kx);
     Ky = fmaf(c * (x2 - y2), rr * rr, Ky);
@@ -329,11 +329,11 @@ __device__ inline void kgrad_cr32(float x, float y, float& K, float& Kx, float& 
     const float rho = expf(-a * y);
     const float M = expf(fmaf(y, y - a, -x2));
     float st, ct, s2, c2;
-    sincospif(2.0f * x / h, &st, &ct);
+    sincospif(x * (2.0f / h), &st, &ct);   // 2x/h (1 ulp; no run-time division)
     sincosf(2.0f * x * y, &s2, &c2);
     const float cphi = ct * c2 + st * s2;   // cos(theta - 2xy)
     const float sphi = st * c2 - ct * s2;   // sin(theta - 2xy)
-    const float rden = 1.0f / (1.0f + 2.0f * sg * rho * ct + rho * rho);
+    const float rden = __builtin_amdgcn_rcpf(1.0f + 2.0f * sg * rho * ct + rho * rho);
     const float g = 2.0f * M * rden;
     const float cr = g * (sg * cphi + rho * c2);
     const float ci = g * (sg * sphi - rho * s2);
