@@ -247,6 +247,31 @@ def test_extreme_line_widths_vs_oracle():
     assert relm.max() < 2e-5
 
 
+@pytest.mark.parametrize("lo,hi,step,ts", [(2400.0, 2406.0, 0.001, 1999),
+                                           (2400.0, 2401.0, 1e-4, 19999)])
+def test_grid_origin_tile_sizes_vs_oracle(lo, hi, step, ts):
+    """Tile size depends on the grid origin (forward.py:175-176): 1999 points
+    at 2400 cm-1 and step 0.001 (tiles just under one 2048-point slice), 19999
+    at step 1e-4 (ten slices per tile, the last one partial)."""
+    say(f"case: 400 lines, {lo}-{hi} @ {step}, 5 layers (tile_samples {ts})")
+    rng = np.random.default_rng(29)
+    lines = synth.random_lines(rng, 400, lo - 27.0, hi + 27.0, 2)
+    sd = SpectralData(lines=lines, tips={(2, 1): synth.tips_table()}, molar_masses={(2, 1): 43.99})
+    p, t, _ = synth.levels(6)
+    lay = AtmosphericProfile(p, t, {"CO2": np.full(6, 4.15e-4)}).build_layers()
+    nu, od = _oracle_tau(sd, lay, lo, hi, step, 25.0, 2.0)
+    prob = OpticalDepthProblem(sd, lay, lo, hi, step, 25.0, 2.0)
+    assert prob.tile_samples == ts and prob.n_nu == len(nu)
+    o64 = prob.run("fp64", mus=(1.0,), want_layers=True)["optical_depth_layers"].cpu().numpy()
+    rel = np.abs(o64 - od) / od
+    say(f"  fp64 max rel tau per layer {rel.max():.2e}")
+    assert rel.max() < TAU_RTOL_FP64
+    om = prob.run("mixed", mus=(1.0,), want_layers=True)["optical_depth_layers"]
+    relm = np.abs(om.double().cpu().numpy() - od) / od
+    say(f"  mixed max rel tau per layer {relm.max():.2e}")
+    assert relm.max() < 2e-5
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_window_without_lines_and_zero_columns(precision):
     """No line reaches the window (the reference returns od = 0, forward.py:245-246),
