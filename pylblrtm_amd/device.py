@@ -58,10 +58,14 @@ def prepare_groups(nu: np.ndarray, sd, layers, cutoff: float, stacklevel: int = 
     iso = np.asarray(lines["iso_id"])
     p_max = float(layers.pressure_atm.max()) if layers.n_layers > 0 else 1.0
     bound = np.abs(delta) * p_max
-    lo = nu[0] - cutoff - bound
-    hi = nu[-1] + cutoff + bound
+    band = (nu0 >= nu[0] - cutoff - bound) & (nu0 <= nu[-1] + cutoff + bound)
+    # (mol_id, iso_id) pairs in sorted order: one int64 key, monotonic in the
+    # pair for any 32-bit ids, so np.unique sorts exactly as sorted(set(...))
+    pair_key = (mol.astype(np.int64) << 32) + (iso.astype(np.int64) + (1 << 31))
+    uniq = np.unique(pair_key)
     groups: List[Group] = []
-    for m, i in sorted(set(zip(mol.tolist(), iso.tolist()))):
+    for kv in uniq.tolist():
+        m, i = kv >> 32, (kv & 0xFFFFFFFF) - (1 << 31)
         gas = MOL_ID_TO_NAME.get(int(m))
         if gas is None:
             warnings.warn(f"mol_id={m} is not in the HITRAN catalogue; lines skipped.",
@@ -73,7 +77,7 @@ def prepare_groups(nu: np.ndarray, sd, layers, cutoff: float, stacklevel: int = 
                 f"atmospheric profile vmr; its lines are skipped.",
                 RuntimeWarning, stacklevel=stacklevel)
             continue
-        sel = (mol == m) & (iso == i) & (nu0 >= lo) & (nu0 <= hi)
+        sel = band & (pair_key == kv)
         if not sel.any():
             continue
         key = (int(m), int(i))
@@ -123,32 +127,38 @@ class OpticalDepthProblem:
                 gases.append(g.gas)
         self.gases = gases
         lines = sd.lines
-        if self.groups:
-            idx = np.concatenate([g.index for g in self.groups])
-            gid = np.concatenate([np.full(g.index.size, j, np.int32) for j, g in enumerate(self.groups)])
-            order = np.argsort(np.asarray(lines["nu0"], np.float64)[idx], kind="stable")
-            idx, gid = idx[order], gid[order]
-        else:
-            idx = np.zeros(0, np.int64)
-            gid = np.zeros(0, np.int32)
-        self.n_lines = int(idx.size)
-        self.line_index = idx
-        self.layers = layers
-        self._host_nu0 = np.asarray(lines["nu0"], np.float64)[idx]
-        self._host_shift = np.asarray(lines["shift"], np.float64)[idx]
-        self._host_group = gid
-        p_max = float(layers.pressure_atm.max()) if self.n_layers > 0 else 1.0
-        shift_sel = np.asarray(lines["shift"], np.float64)[idx]
-        self.shift_bound = float(np.max(np.abs(shift_sel)) * p_max) if idx.size else 0.0
 
         def dev(a, dtype):
             return torch.as_tensor(np.ascontiguousarray(a, dtype=dtype)).to(self.device)
 
         f8 = np.float64
         self._t = {}
-        for key in ("nu0", "Sref", "g_air", "g_self", "Elow", "n_air", "shift"):
-            self._t[key] = dev(np.asarray(lines[key], f8)[idx], f8)
-        self._t["group"] = dev(gid, np.int32)
+        # The surviving lines, ordered by centre (stable: lines with equal
+        # centres keep the group order), gathered on the device.
+        if self.groups:
+            idx = dev(np.concatenate([g.index for g in self.groups]), np.int64)
+            gid = dev(np.concatenate([np.full(g.index.size, j, np.int32)
+                                      for j, g in enumerate(self.groups)]), np.int32)
+            full = {key: dev(lines[key], f8) for key in
+                    ("nu0", "Sref", "g_air", "g_self", "Elow", "n_air", "shift")}
+            order = torch.sort(full["nu0"][idx], stable=True).indices
+            idx, gid = idx[order], gid[order]
+            for key, arr in full.items():
+                self._t[key] = arr[idx].contiguous()
+            del full
+        else:
+            idx = torch.zeros(0, dtype=torch.int64, device=self.device)
+            gid = torch.zeros(0, dtype=torch.int32, device=self.device)
+            for key in ("nu0", "Sref", "g_air", "g_self", "Elow", "n_air", "shift"):
+                self._t[key] = torch.zeros(0, dtype=torch.float64, device=self.device)
+        self._t["group"] = gid.contiguous()
+        self.n_lines = int(idx.numel())
+        self.line_index = idx.cpu().numpy()
+        self.layers = layers
+        p_max = float(layers.pressure_atm.max()) if self.n_layers > 0 else 1.0
+        self.shift_bound = (float(self._t["shift"].abs().max().item()) * p_max
+                            if self.n_lines else 0.0)
+        self._host = None
         offs = np.zeros(len(self.groups) + 1, np.int32)
         for j, g in enumerate(self.groups):
             offs[j + 1] = offs[j] + g.tips[0].size
@@ -189,6 +199,14 @@ class OpticalDepthProblem:
         self.c_grid = _capi.LblGrid(self.nu_min, self.step, self.n_nu, self.tile_samples,
                                     self.cutoff, self.shift_bound)
         self._ws = {}
+
+    # host copies of the sorted line centres, shifts and groups, fetched once
+    # on first use (the evaluation count needs them; the line sums do not)
+    def _host_lines(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        if self._host is None:
+            self._host = (self._t["nu0"].cpu().numpy(), self._t["shift"].cpu().numpy(),
+                          self._t["group"].cpu().numpy())
+        return self._host
 
     # ------------------------------------------------------------ layers --
     def set_layers(self, temperature_k, vmr) -> None:
@@ -269,6 +287,7 @@ class OpticalDepthProblem:
             t0, t1, lo_b, hi_b = t0[sel], t1[sel], lo_b[sel], hi_b[sel]
         npts = t1 - t0
         total = 0
+        h_nu0, h_shift, h_group = self._host_lines()
         ks = range(self.n_layers) if layers is None else layers
         lay = self.layers
         for k in ks:
@@ -276,9 +295,9 @@ class OpticalDepthProblem:
             for j, g in enumerate(self.groups):
                 if float(lay.column_cm2[g.gas][k]) == 0.0:
                     continue
-                m = self._host_group == j
+                m = h_group == j
                 p_self = float(lay.vmr[g.gas][k]) * P
-                c = np.sort(self._host_nu0[m] + self._host_shift[m] * (P - p_self))
+                c = np.sort(h_nu0[m] + h_shift[m] * (P - p_self))
                 cnt = np.searchsorted(c, hi_b, side="right") - np.searchsorted(c, lo_b, side="left")
                 total += int(np.sum(np.maximum(cnt, 0) * npts))
         return total
