@@ -148,7 +148,7 @@ struct VjpArgs {
 struct alignas(16) VjpFar {
   double c;
   float s, hs;
-  float c1, hw1, hw2, ay;    // y^2 - 3/2, mid-run |x| thresholds (one-node, two-node), A y/sqrt(pi)
+  float c1, hw1, hw2, tA;    // y^2 - 3/2, mid-run |x| thresholds (one-node, two-node), tF - tX2
   float tF, tX2, tD, tQ;     // temperature: weights of S1, S4, S2, S5
   float vF, vX, vD, vQ;      // mixing ratio: weights of S1, S3, S2, S5
 };
@@ -255,8 +255,9 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       // thresholds on |x| at the middle of a thread's run of P points
       const float half_run = 0.5f * (float)(P - 1) * f.hs;
       f.hw1 = half_run + sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
-      f.hw2 = half_run + sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f)); f.ay = ay;
+      f.hw2 = half_run + sqrtf(fmaxf(kF32Gh2R2 - y2, 0.0f));
       f.tF = at * ay + cty; f.tX2 = 2.0f * ax_t * ay; f.tD = cty * d2; f.tQ = -2.0f * d2 * cty;
+      f.tA = f.tF - f.tX2;
       f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
       n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
       gas = a.gas_of_group[a.group[base + tid]];
@@ -293,7 +294,6 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
         // x at the middle of the run; point j at xm + (j - (P-1)/2) hs
         const float xm = (float)(nu_m - R.c) * R.s;
         const float hs = R.hs;
-        float acc_t, acc_v;
         if (fabsf(xm) >= R.hw1) {
           const float c1 = R.c1;
           float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
@@ -307,11 +307,11 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
             S2 += q2;
             S3 = fmaf(x, q2, S3);
           }
-          // x^2 r^2 = r - c1 r^2, so sum W x^2 r^2 = S1 - c1 S2; its rounding error
-          // is at most that of S1, the size of the tF S1 term's own
-          const float S4 = fmaf(-c1, S2, S1);
-          acc_t = fmaf(R.tF, S1, -fmaf(R.tX2, S4, R.tD * S2));
-          acc_v = fmaf(R.vF, S1, -fmaf(R.vX, S3, R.vD * S2));
+          // x^2 r^2 = r - c1 r^2, so sum W x^2 r^2 = S1 - c1 S2 and the temperature
+          // share tF S1 - tX2 (S1 - c1 S2) - tD S2 is (tF - tX2) S1 + (tX2 c1 - tD) S2;
+          // both shares go straight into the running sums
+          ft = fmaf(R.tA, S1, fmaf(fmaf(R.tX2, c1, -R.tD), S2, ft));
+          fv = fmaf(R.vF, S1, fmaf(-R.vX, S3, fmaf(-R.vD, S2, fv)));
         } else if (fabsf(xm) >= R.hw2) {
           const float cw = R.c1 + 1.0f, d2 = fmaf(2.0f, R.c1, 3.0f);
           float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f, S4 = 0.0f, S5 = 0.0f;
@@ -331,11 +331,11 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
             S4 = fmaf(x, xqe, S4);
             S5 = fmaf(q, w1, S5);
           }
-          acc_t = fmaf(R.tF, S1, fmaf(R.tX2, S4, fmaf(R.tD, S2, R.tQ * S5)));
-          acc_v = fmaf(R.vF, S1, fmaf(R.vX, S3, fmaf(R.vD, S2, R.vQ * S5)));
+          ft += fmaf(R.tF, S1, fmaf(R.tX2, S4, fmaf(R.tD, S2, R.tQ * S5)));
+          fv += fmaf(R.vF, S1, fmaf(R.vX, S3, fmaf(R.vD, S2, R.vQ * S5)));
         } else {
           const VjpNear& N = s_near[qi];
-          acc_t = acc_v = 0.0f;
+          float acc_t = 0.0f, acc_v = 0.0f;
 #pragma unroll 1
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
@@ -350,9 +350,9 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
             acc_t = fmaf(wr, N.at * AK + Kx * (ax_t * x) + Ky * N.dyt, acc_t);
             acc_v = fmaf(wr, N.av * AK + Kx * N.bxv + Ky * N.dyv, acc_v);
           }
+          ft += acc_t;
+          fv += acc_v;
         }
-        ft += acc_t;
-        fv += acc_v;
         if (((qi - q0) & (kVjpFold - 1)) == kVjpFold - 1) {
           g_t += (double)ft;
           dv += (double)fv;
