@@ -223,6 +223,10 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
     }
     g_t = (double)rad;
   }
+  // (j - (P-1)/2) WR_j: sum_j WR_j x_j r_j^2 = xm S2 + hs sum_j oW_j r_j^2
+  float oW[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) oW[j] = ((float)j - 0.5f * (float)(P - 1)) * WR[j];
   double g_v[8];
 #pragma unroll
   for (int g = 0; g < 8; ++g) g_v[g] = 0.0;
@@ -296,17 +300,17 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
         const float hs = R.hs;
         if (fabsf(xm) >= R.hw1) {
           const float c1 = R.c1;
-          float S1 = 0.0f, S2 = 0.0f, S3 = 0.0f;
+          float S1 = 0.0f, S2 = 0.0f, S3o = 0.0f;
 #pragma unroll
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
             const float r = __builtin_amdgcn_rcpf(fmaf(x, x, c1));
-            const float q = WR[j] * r;
-            const float q2 = q * r;
-            S1 += q;
-            S2 += q2;
-            S3 = fmaf(x, q2, S3);
+            const float r2 = r * r;
+            S1 = fmaf(WR[j], r, S1);
+            S2 = fmaf(WR[j], r2, S2);
+            S3o = fmaf(oW[j], r2, S3o);
           }
+          const float S3 = fmaf(xm, S2, hs * S3o);   // sum W x r^2
           // x^2 r^2 = r - c1 r^2, so sum W x^2 r^2 = S1 - c1 S2 and the temperature
           // share tF S1 - tX2 (S1 - c1 S2) - tD S2 is (tF - tX2) S1 + (tX2 c1 - tD) S2;
           // both shares go straight into the running sums
