@@ -171,10 +171,14 @@ __device__ inline double K_cr64(double x, double y) {
   s *= 2.0 * y;
   if (!half) s += y * rcp64_1nr(r);
   double K = s * h * kInvPi;
-  if (y < kPi / h) {
+  // The residue is at most 4M in size: its numerator is at most 2M (1 + rho)
+  // and, on the lattice chosen above (cos theta > 0 on the half lattice,
+  // <= 0 on the integer one), its denominator is at least 1.  Where 4M is
+  // below 1e-15 of the pole-pair sum it is left out (never when y = 0).
+  const double M = exp(y2 - x2 - 2.0 * kPi * y / h);
+  if (y < kPi / h && 4.0 * M > 1e-15 * K) {
     const double sg = half ? 1.0 : -1.0;
     const double rho = exp(-2.0 * kPi * y / h);
-    const double M = exp(y2 - x2 - 2.0 * kPi * y / h);
     double st, ct, s2, c2;
     sincospi(2.0 * x / h, &st, &ct);
     sincos(2.0 * x * y, &s2, &c2);
@@ -241,11 +245,12 @@ __device__ inline float K_cr32(float x, float y) {
   s *= 2.0f * y;
   if (!half) s += y * __builtin_amdgcn_rcpf(r);
   float K = s * h * (float)kInvPi;
-  if (y < (float)kPi / h) {
+  // |residue| <= 4M, as in K_cr64; left out where below 1e-8 of the pair sum
+  const float a = 2.0f * (float)kPi / h;
+  const float M = expf(fmaf(y, y - a, -x2));
+  if (y < (float)kPi / h && 4.0f * M > 1e-8f * K) {
     const float sg = half ? 1.0f : -1.0f;
-    const float a = 2.0f * (float)kPi / h;
     const float rho = expf(-a * y);
-    const float M = expf(fmaf(y, y - a, -x2));
     float st, ct, s2, c2;
     sincospif(x * (2.0f / h), &st, &ct);   // 2x/h (1 ulp; no run-time division)
     sincosf(2.0f * x * y, &s2, &c2);
