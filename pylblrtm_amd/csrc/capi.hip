@@ -71,7 +71,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-  size_t c, s, y, a, lo, hi, err, tau_part, total;
+  size_t c, s, y, a, lo, hi, wlo, whi, err, tau_part, far, total;
   size_t g[5];   // adjoint coefficient arrays (only with_grad)
   int64_t n_tiles, n_wg;
   int32_t slices_per_tile, layer_groups;
@@ -97,10 +97,19 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->a = off; off = align_up(off + (fp64 ? nrec * 8 : 0));
   L->lo = off; off = align_up(off + (size_t)L->n_tiles * 4);
   L->hi = off; off = align_up(off + (size_t)L->n_tiles * 4);
+  const size_t n_waves = (size_t)L->n_wg * (size_t)(lbl::sum_block(precision) / 64);
+  L->wlo = off; off = align_up(off + n_waves * 4);
+  L->whi = off; off = align_up(off + n_waves * 4);
   L->err = off; off = align_up(off + 4);
   // partial tau per layer group (float sum + compensation, or double): 8 B
   L->tau_part = off;
-  if (L->layer_groups > 1) off = align_up(off + (size_t)L->layer_groups * (size_t)grid->n_nu * 8);
+  // (the mixed sums keep their column totals here even with one group)
+  if (L->layer_groups > 1 || (!fp64 && !with_grad))
+    off = align_up(off + (size_t)std::max<int32_t>(L->layer_groups, 1) * (size_t)grid->n_nu * 8);
+  // mixed modes: far-field sum per (layer, point), FP32
+  L->far = off;
+  if (!fp64 && !with_grad)
+    off = align_up(off + (size_t)std::max<int32_t>(layers->n_layers, 0) * (size_t)grid->n_nu * 4);
   for (int i = 0; i < 5; ++i) {
     L->g[i] = off;
     if (with_grad) off = align_up(off + nrec * 4);
@@ -218,10 +227,32 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   tr.margin = fabs(grid->shift_bound) * (1.0 + 1e-9) + 1e-9;
   HIP_TRY(lbl::launch_tile_ranges(tr, s));
 
+  // near range of every wavefront for the far-field series: records whose
+  // nu0 is within kSeriesRatio * H (+ the shift bound) of its middle point
+  int32_t* wlo = reinterpret_cast<int32_t*>(ws + L.wlo);
+  int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
+  if (precision != LBL_FP64) {
+    lbl::WaveRangeArgs wr{};
+    wr.nu0 = lines->nu0;
+    wr.n_lines = lines->n_lines;
+    wr.nu_min = grid->nu_min;
+    wr.step = grid->step;
+    wr.half_width = lbl::kSeriesRatio * 0.5 * lbl::kWavePoints * grid->step + tr.margin;
+    wr.ts = grid->tile_samples;
+    wr.n_wg = L.n_wg;
+    wr.slices_per_tile = L.slices_per_tile;
+    wr.waves_per_slice = lbl::sum_block(precision) / 64;
+    wr.wlo = wlo;
+    wr.whi = whi;
+    HIP_TRY(lbl::launch_wave_ranges(wr, s));
+  }
+
   lbl::SumArgs sa{};
   sa.rec = rec;
   sa.lo = lo;
   sa.hi = hi;
+  sa.wlo = wlo;
+  sa.whi = whi;
   sa.t_k = layers->t_k;
   sa.n_lines = lines->n_lines;
   sa.n_layers = layers->n_layers;
@@ -236,6 +267,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   if (sa.n_wg > INT32_MAX) return fail(LBL_EINVAL, "grid too large");
   sa.layer_groups = L.layer_groups;
   sa.tau_part = ws + L.tau_part;
+  sa.far_sum = reinterpret_cast<float*>(ws + L.far);
   sa.n_mu = out->n_mu;
   for (int m = 0; m < out->n_mu; ++m) sa.inv_mu[m] = 1.0 / out->mu[m];
   sa.trans = out->transmittance;

@@ -49,6 +49,23 @@ int set_error(int code, const char* fmt, ...);
 // Grid points per thread of the mixed-precision line sums.
 constexpr int kPointsMixed = 8;
 
+// Far-field series of the line sums.  A wavefront's 64 lanes own
+// kWavePoints consecutive grid points, whose middle is nu_w (wave_middle) and
+// half-span H = kWavePoints/2 * step.  A record whose centre lies at least
+// kSeriesRatio * H from nu_w contributes a smooth function of
+// tau = (nu - nu_w) / H, |tau| < 1, to all of them: its one-node form's Taylor
+// series in tau, kSeriesTerms terms, is summed per lane into kSeriesTerms
+// coefficients, reduced across the wavefront and evaluated at each point.
+constexpr int kWavePoints = 64 * kPointsMixed;
+constexpr double kSeriesRatio = 3.0;
+constexpr int kSeriesTerms = 18;
+
+// Middle of the wavefront whose first grid point is i: grid(i) + (kWavePoints-1)/2 step.
+__device__ __forceinline__ double wave_middle(double nu_min, double step, int64_t i) {
+  return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
+                   0.5 * (double)(kWavePoints - 1) * step);
+}
+
 // Per-(layer, line) record of the mixed-precision paths (48 bytes, one
 // 16-byte-aligned struct so the line sum stages it with three 16-byte loads).
 // The first 16 bytes are what the line sum's per-record test reads, the next
@@ -62,13 +79,16 @@ constexpr int kPointsMixed = 8;
 //   hw2 = the same for the middle tier: the second-order shifted form
 //        (faddeeva.h s2_r2_min32) when y^2 < kF32S2Y2, else the two-node form
 //        (|z| >= 25); the tier is chosen per record from c1,
-//   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D.
+//   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D,
+//   ds1 = H + sqrt(max(s1_r2_min(y^2) - y^2, 0)) / s: the one-node form holds
+//        at all kWavePoints points of a wavefront iff |nu_w - c| >= ds1,
+//   sh = s H (H = kWavePoints/2 * step, the far-field series' unit).
 // A skipped (zero-column) record has c = NaN, ay = a = 0.
 struct alignas(16) MixedRec {
   double c;
   float s, hw1;
   float hs, c1, ay, hw2;
-  float y, a, pad0, pad1;
+  float y, a, ds1, sh;
 };
 static_assert(sizeof(MixedRec) == 48, "record layout");
 
@@ -100,10 +120,25 @@ struct TileRangeArgs {
   int32_t* hi;
 };
 
+// Near range of each wavefront of the line sum: [wlo, whi) are the records
+// whose nu0 lies within half_width of the wavefront's middle point, one
+// entry per (workgroup slice, wavefront) in launch order.
+struct WaveRangeArgs {
+  const double* nu0;   // sorted ascending
+  int64_t n_lines;
+  double nu_min, step, half_width;
+  int64_t ts, n_wg;
+  int32_t slices_per_tile, waves_per_slice;
+  int32_t* wlo;
+  int32_t* whi;
+};
+
 struct SumArgs {
   Records rec;
   const int32_t* lo;
   const int32_t* hi;
+  const int32_t* wlo;   // mixed modes: near range of every wavefront
+  const int32_t* whi;
   const double* t_k;
   int64_t n_lines;
   int32_t n_layers;
@@ -131,6 +166,9 @@ struct SumArgs {
   // line sum itself.
   int32_t layer_groups;
   void* tau_part;
+  // mixed modes: far-field sum per (layer, point), written by
+  // k_far_series_mixed and added by the line sum
+  float* far_sum;
 };
 
 // Workgroups a line-sum launch aims for.  Many short workgroups (a layer
@@ -152,6 +190,7 @@ inline int32_t layer_groups_for(int64_t n_wg, int32_t n_layers, int precision) {
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_line_params(const LineParamsArgs& a, int precision, hipStream_t s);
 hipError_t launch_tile_ranges(const TileRangeArgs& a, hipStream_t s);
+hipError_t launch_wave_ranges(const WaveRangeArgs& a, hipStream_t s);
 hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s);
 // Sum of the layer-group partials, T = exp(-tau/mu) (only when layer_groups > 1).
 hipError_t launch_tau_epilogue(const SumArgs& a, int precision, hipStream_t s);

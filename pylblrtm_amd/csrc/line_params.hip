@@ -66,13 +66,16 @@ __device__ __forceinline__ void store_mixed(const Records& r, int64_t i, double 
   m.c1 = y2 - 1.5f;
   m.ay = a * y * (float)kInvSqrtPi;
   const float half_run = 0.5f * (float)(kPointsMixed - 1) * m.hs;
-  m.hw1 = half_run + sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
+  const float x_s1 = sqrtf(fmaxf(s1_r2_min(y2) - y2, 0.0f));
+  m.hw1 = half_run + x_s1;
   // the line sum picks the middle tier by the same test on c1
   const float r2_mid = (m.c1 < kF32S2Y2 - 1.5f) ? s2_r2_min32(y2) : kF32Gh2R2;
   m.hw2 = half_run + sqrtf(fmaxf(r2_mid - y2, 0.0f));
   m.y = y;
   m.a = a;
-  m.pad0 = m.pad1 = 0.0f;
+  const float H = 0.5f * (float)kWavePoints * step;
+  m.ds1 = H + x_s1 / s;
+  m.sh = s * H;
   r.mix[i] = m;
 }
 
@@ -186,6 +189,35 @@ __global__ void k_tile_ranges(const TileRangeArgs a) {
 
 }  // namespace
 
+// One thread per (workgroup slice, wavefront): the records whose nu0 lies in
+// [nu_w - half_width, nu_w + half_width] (lower and upper bound in the
+// sorted nu0), nu_w the wavefront's middle point as the line sum forms it.
+__global__ void k_wave_ranges(const WaveRangeArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.n_wg * a.waves_per_slice) return;
+  const int64_t wg = g / a.waves_per_slice;
+  const int w = (int)(g - wg * a.waves_per_slice);
+  const int64_t tile = wg / a.slices_per_tile;
+  const int64_t slice = wg - tile * a.slices_per_tile;
+  const int64_t first =
+      tile * a.ts + (slice * a.waves_per_slice + w) * (int64_t)kWavePoints;
+  const double nu_w = wave_middle(a.nu_min, a.step, first);
+  const double lo_v = nu_w - a.half_width, hi_v = nu_w + a.half_width;
+  int64_t lo = 0, hi = a.n_lines;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a.nu0[mid] < lo_v) lo = mid + 1; else hi = mid;
+  }
+  const int64_t first_rec = lo;
+  hi = a.n_lines;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a.nu0[mid] <= hi_v) lo = mid + 1; else hi = mid;
+  }
+  a.wlo[g] = (int32_t)first_rec;
+  a.whi[g] = (int32_t)lo;
+}
+
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
@@ -205,6 +237,13 @@ hipError_t launch_line_params(const LineParamsArgs& a, int precision, hipStream_
 hipError_t launch_tile_ranges(const TileRangeArgs& a, hipStream_t s) {
   if (a.n_tiles <= 0) return hipSuccess;
   k_tile_ranges<<<(unsigned)((a.n_tiles + 255) / 256), 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_wave_ranges(const WaveRangeArgs& a, hipStream_t s) {
+  const int64_t n = a.n_wg * a.waves_per_slice;
+  if (n <= 0) return hipSuccess;
+  k_wave_ranges<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a);
   return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
 // The full FP64 dispatch, called (not inlined) from the rare near-centre
 // paths so that its registers do not crowd the far-field loop.
 __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x, y); }
+__device__ __noinline__ float K_mixed_call(float x, float y) { return K_mixed(x, y); }
 
 // Record of the FP64 line sum as staged in LDS (one per (layer, line)); the
 // queue drain reads y and A from the global records.  hw1, hw2, hw4: |x| at
@@ -349,6 +350,198 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
 // exact inclusion test, two records per thread per chunk of kChunk.
 constexpr int kMixedBlock = 256;
 
+// Far-field series of the mixed line sums (see kSeriesRatio): one
+// workgroup per (slice, layer), its four wavefronts independent (no
+// barriers).  Writes far_sum[k][i], the layer-k sum over all records of the
+// tile outside the wavefront's near range, compensated, for every point of
+// the tile; the line sum adds it before the radiation term.
+constexpr int kExCap = 128;   // far-field exception list entries per wavefront
+
+template <int P>
+__global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_far_series_mixed(const SumArgs a) {
+  static_assert(P == kPointsMixed && 64 * P == kWavePoints, "wavefront geometry");
+  constexpr int kWaves = kMixedBlock / 64;
+  // per wavefront: records nearer than their ds1, evaluated point by point
+  // after (or, when the list fills, during) the loop
+  __shared__ int32_t s_ex[kWaves][kExCap];
+  // (layer, slice), layer-major so that an XCD holds neighbouring tiles of a layer
+  const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
+  const int k = (int)(lw / a.n_wg);
+  const int64_t wg = lw - (int64_t)k * a.n_wg;
+  const int64_t tile = wg / a.slices_per_tile;
+  const int slice = (int)(wg - tile * a.slices_per_tile);
+  const int64_t t0 = tile * a.ts;
+  const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  const int64_t s0 = t0 + (int64_t)slice * (kMixedBlock * P);
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
+  if (w_first >= t1) return;   // the whole wavefront: no points
+  const int64_t i0 = w_first + (int64_t)lane * P;
+  const double nu_w = wave_middle(a.nu_min, a.step, w_first);
+  const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
+  const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
+  const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
+  const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  const int64_t kb = (int64_t)k * a.n_lines;
+  const MixedRec* __restrict__ recs = a.rec.mix;
+  float lay[P], layc[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) lay[j] = layc[j] = 0.0f;
+    // Far field: every record of the tile outside [nlo, nhi), one per lane,
+    // straight from global memory (the next one loaded while this one is
+    // summed).  A record inside the tile window with |nu_w - c| >= ds1 adds
+    // its one-node form's Taylor series in tau to the lane's coefficients:
+    //   ay / ((xm + sh tau)^2 + c1) = sum_n Q_n tau^n,
+    //   Q_0 = ay / D0, Q_1 = al Q_0, Q_n = al Q_(n-1) + be Q_(n-2),
+    //   D0 = xm^2 + c1, al = -2 xm sh / D0, be = -sh^2 / D0.
+    // |nu_w - c| >= kSeriesRatio H puts both poles of the form at
+    // |tau| >= 1/0.345 (0.345: y^2 < 3/2, |xm| >= 25 + sh), so the terms
+    // dropped after kSeriesTerms weigh <= 2.6e-7 of the record's value at
+    // any |tau| < 1.  A record that is nearer than ds1 (a broad line whose
+    // one-node region starts farther out) is evaluated at every lane's own
+    // points instead, with the same tiers as the near field.
+    float C[kSeriesTerms];
+#pragma unroll
+    for (int n = 0; n < kSeriesTerms; ++n) C[n] = 0.0f;
+    float S0 = 0.0f, S0c = 0.0f, S1 = 0.0f, S1c = 0.0f;   // compensated Q_0, Q_1 sums
+    const int32_t n_below = nlo - rlo;
+    const int32_t n_far = n_below + (rhi - nhi);
+    const MixedRec* __restrict__ lrec = recs + kb;
+    auto far_index = [&](int32_t v) -> int64_t {
+      return v < n_below ? (int64_t)rlo + v : (int64_t)nhi + (v - n_below);
+    };
+    // the five fields the series needs, prefetched one iteration ahead
+    struct FarRec { double c; float s, c1, ay, ds1, sh; };
+    auto far_load = [&](int64_t i) {
+      const MixedRec* const p = lrec + i;
+      FarRec f;
+      f.c = p->c; f.s = p->s; f.c1 = p->c1; f.ay = p->ay; f.ds1 = p->ds1; f.sh = p->sh;
+      return f;
+    };
+    // records of s_ex[wave][0, n) at every lane's own points (record index
+    // uniform across the wavefront), into the compensated layer sum
+    auto far_exceptions = [&](const int n) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the lanes' list writes
+      for (int e = 0; e < n; ++e) {
+        const int32_t idx = __builtin_amdgcn_readfirstlane(s_ex[wave][e]);
+        const MixedRec R = lrec[idx];
+        const float xm = (float)(nu_m - R.c) * R.s;
+        float v[P];
+        if (fabsf(xm) >= R.hw1) {
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+            v[j] = R.ay * __builtin_amdgcn_rcpf(fmaf(x, x, R.c1));
+          }
+        } else if (fabsf(xm) >= R.hw2) {
+          const bool s2 = R.c1 < kF32S2Y2 - 1.5f;
+          const float y2 = R.c1 + 1.5f, cc = fmaf(2.0f, R.c1, 1.5f);
+          const float cw = R.c1 + 1.0f, d2 = fmaf(2.0f, R.c1, 3.0f);
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+            if (s2) {   // second-order shifted form
+              const float r = fmaf(x, x, y2);
+              v[j] = R.ay * r * __builtin_amdgcn_rcpf(fmaf(r, r - 1.5f, cc));
+            } else {    // two-node form
+              const float w = fmaf(x, x, cw);
+              v[j] = fmaf(R.ay, w, R.ay) * __builtin_amdgcn_rcpf(fmaf(w, w, d2));
+            }
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < P; ++j)
+            v[j] = R.a * K_mixed_call(fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm), R.y);
+        }
+#pragma unroll
+        for (int j = 0; j < P; ++j) kahan_add(lay[j], layc[j], v[j]);
+      }
+    };
+    int n_ex = 0;
+    FarRec nx;
+    nx.c = __builtin_nan("");
+    if (lane < n_far) nx = far_load(far_index(lane));
+    const int32_t n_it = (n_far + 63) >> 6;
+    for (int32_t it = 0; it < n_it; ++it) {
+      const FarRec m = nx;
+      const int32_t vn = (it + 1) * 64 + lane;
+      if (vn < n_far) nx = far_load(far_index(vn));
+      const float d = (float)(nu_w - m.c);
+      // in range, inside the tile window, and far enough for the series (no
+      // short-circuit: one branch)
+      const bool inw = (it * 64 + lane < n_far) & (m.c >= lo_b) & (m.c <= hi_b);
+      const bool ser = inw & (fabsf(d) >= m.ds1);
+      if (ser) {
+        const float xm = d * m.s;
+        const float r = __builtin_amdgcn_rcpf(fmaf(xm, xm, m.c1));
+        const float u = m.sh * r;
+        const float al = -2.0f * xm * u, be = -m.sh * u;
+        float qa = m.ay * r, qb = al * qa;
+        C[0] += qa;
+        C[1] += qb;
+#pragma unroll
+        for (int n = 2; n < kSeriesTerms; ++n) {
+          const float qn = fmaf(al, qb, be * qa);
+          C[n] += qn;
+          qa = qb;
+          qb = qn;
+        }
+      }
+      const bool exc = inw & !ser;
+      const uint64_t ex = __ballot(exc);
+      if (ex) {   // uniform
+        if (n_ex + 64 > kExCap) {
+          far_exceptions(n_ex);
+          n_ex = 0;
+        }
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
+        if (exc) s_ex[wave][n_ex + (int)rank] = (int32_t)far_index(it * 64 + lane);
+        n_ex += (int)__popcll(ex);
+      }
+      if ((it & 15) == 15) {
+        kahan_add(S0, S0c, C[0]);
+        kahan_add(S1, S1c, C[1]);
+        C[0] = C[1] = 0.0f;
+      }
+    }
+    far_exceptions(n_ex);
+    kahan_add(S0, S0c, C[0]);
+    kahan_add(S1, S1c, C[1]);
+    // totals over the wavefront's 64 lanes, in every lane
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      S0 += __shfl_xor(S0, off);
+      S0c += __shfl_xor(S0c, off);
+      S1 += __shfl_xor(S1, off);
+      S1c += __shfl_xor(S1c, off);
+#pragma unroll
+      for (int n = 2; n < kSeriesTerms; ++n) C[n] += __shfl_xor(C[n], off);
+    }
+    const float s1t = S1 - S1c;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      // tau of point lane*P + j of the wavefront (exact in FP32)
+      const float t = ((float)(lane * P + j) - 0.5f * (float)(kWavePoints - 1)) *
+                      (2.0f / (float)kWavePoints);
+      float f = C[kSeriesTerms - 1];
+#pragma unroll
+      for (int n = kSeriesTerms - 2; n >= 2; --n) f = fmaf(f, t, C[n]);
+      f = fmaf(f, t, s1t) * t;
+      kahan_add(lay[j], layc[j], S0);
+      kahan_add(lay[j], layc[j], f - S0c);
+    }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int64_t i = i0 + j;
+    if (i < t1) a.far_sum[(int64_t)k * a.n_nu + i] = lay[j] - layc[j];
+  }
+}
+
+
 template <int P, bool kSurrogate>
 __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ MixedRec s_rec[kChunk];
@@ -388,20 +581,45 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // s_rec as an LDS-qualified pointer, so record reads stay ds_read_b128
   lds_mixed_rec* const s_rec_lds = (lds_mixed_rec*)s_rec;
 
+  // This wavefront's near range [nlo, nhi) (records within kSeriesRatio H
+  // of its middle point, by nu0; k_far_series_mixed has summed all others);
+  // the workgroup stages the union [ulo, uhi) of its wavefronts' ranges.
+  constexpr int kWaves = kMixedBlock / 64;
+  const int wave = (int)threadIdx.x >> 6;
+  int32_t nlo = rlo, nhi = rlo, ulo = rhi, uhi = rlo;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const int32_t l = min(max(a.wlo[wg * kWaves + w], rlo), rhi);
+    const int32_t h = max(min(a.whi[wg * kWaves + w], rhi), l);
+    if (s0 + (int64_t)w * kWavePoints < t1 && h > l) {
+      ulo = min(ulo, l);
+      uhi = max(uhi, h);
+      if (w == wave) { nlo = l; nhi = h; }
+    } else if (w == wave) {
+      nlo = nhi = l;
+    }
+  }
+  if (uhi < ulo) uhi = ulo;
+
   for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0f;
   if (threadIdx.x == 0) s_qn = 0;
 
-  float tau[P], tauc[P];
-#pragma unroll
-  for (int j = 0; j < P; ++j) tau[j] = tauc[j] = 0.0f;
+  // column totals (compensated) live in tau_part[grp], read and rewritten
+  // once per layer, so that no registers hold them through the record loops
+  float2* const tau_acc = static_cast<float2*>(a.tau_part) + (int64_t)grp * a.n_nu;
 
   for (int k = k_begin; k < k_end; ++k) {
-    float lay[P], layc[P], part[P];
+    float lay[P], layc[P];
 #pragma unroll
-    for (int j = 0; j < P; ++j) lay[j] = layc[j] = part[j] = 0.0f;
+    for (int j = 0; j < P; ++j) lay[j] = layc[j] = 0.0f;
     const int64_t kb = (int64_t)k * a.n_lines;
-    for (int32_t base = rlo; base < rhi; base += kChunk) {
-      const int cnt = min(kChunk, rhi - base);
+    // Near field: the union of the wavefronts' near ranges through LDS; each
+    // wavefront takes its own range's records of every chunk.
+    float part[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) part[j] = 0.0f;
+    for (int32_t base = ulo; base < uhi; base += kChunk) {
+      const int cnt = min(kChunk, uhi - base);
       __syncthreads();
 #pragma unroll
       for (int h = 0; h < kChunk / kMixedBlock; ++h) {
@@ -488,21 +706,25 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       // exist, so record 1 is readable even when cnt == 1.  The record
       // address lives in a VGPR (the empty asm makes it one) so every LDS
       // read uses an immediate offset instead of a copy from an SGPR.
-      u32x4 ha = rb[0], hb = rb[3];
-      const volatile lds_u32x4* pr = rb;
-      asm volatile("" : "+v"(pr));
-      for (int q = 0; q < ncnt; q += 2, pr += 6) {
-        record(q, ha, pr + 1);
-        if (q + 2 < ncnt) ha = pr[6];
-        if (q + 1 < ncnt) {
-          record(q + 1, hb, pr + 4);
-          if (q + 3 < ncnt) hb = pr[9];
-        }
-        if (((q + 2) & (kFoldEvery - 1)) == 0) {   // after records 31, 63, ...
+      const int qlo = __builtin_amdgcn_readfirstlane(min(max(nlo - base, 0), ncnt));
+      const int qhi = __builtin_amdgcn_readfirstlane(min(max(nhi - base, 0), ncnt));
+      if (qlo < qhi) {
+        u32x4 ha = rb[3 * qlo], hb = rb[3 * min(qlo + 1, kChunk - 1)];
+        const volatile lds_u32x4* pr = rb + 3 * qlo;
+        asm volatile("" : "+v"(pr));
+        for (int q = qlo; q < qhi; q += 2, pr += 6) {
+          record(q, ha, pr + 1);
+          if (q + 2 < qhi) ha = pr[6];
+          if (q + 1 < qhi) {
+            record(q + 1, hb, pr + 4);
+            if (q + 3 < qhi) hb = pr[9];
+          }
+          if (((q - qlo + 2) & (kFoldEvery - 1)) == 0) {   // after every 32 records
 #pragma unroll
-          for (int j = 0; j < P; ++j) {
-            kahan_add(lay[j], layc[j], part[j]);
-            part[j] = 0.0f;
+            for (int j = 0; j < P; ++j) {
+              kahan_add(lay[j], layc[j], part[j]);
+              part[j] = 0.0f;
+            }
           }
         }
       }
@@ -572,16 +794,21 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     }
     const double T = a.t_k[k];
     bool bad = false;
+    const float* const far_k = a.far_sum + (int64_t)k * a.n_nu;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const int64_t i = i0 + j;
       if (i < t1) {
+        kahan_add(lay[j], layc[j], far_k[i]);   // the far field of this layer
         // R(nu, T) in FP32 (relative error ~1e-7, far inside the mixed budget)
         const float nu = (float)grid_nu(a.nu_min, a.step, i);
         const float R = nu * tanhf(0.5f * (float)kC2 * nu / (float)T);
         const float v = (lay[j] - layc[j]) * R;
         bad |= !isfinite(v);
-        kahan_add(tau[j], tauc[j], v);
+        float2 acc = make_float2(0.0f, 0.0f);
+        if (k > k_begin) acc = tau_acc[i];
+        kahan_add(acc.x, acc.y, v);
+        tau_acc[i] = acc;
         if (a.tau_layers) static_cast<float*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
       }
     }
@@ -591,11 +818,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
     if (i < t1) {
-      if (a.layer_groups > 1) {
-        static_cast<float2*>(a.tau_part)[(int64_t)grp * a.n_nu + i] = make_float2(tau[j], tauc[j]);
-        continue;
-      }
-      const float t = tau[j] - tauc[j];
+      if (k_begin == k_end) tau_acc[i] = make_float2(0.0f, 0.0f);   // no layers in this group
+      if (a.layer_groups > 1) continue;   // k_tau_epilogue32 adds the groups
+      const float2 acc = tau_acc[i];
+      const float t = acc.x - acc.y;
       for (int m = 0; m < a.n_mu; ++m)
         static_cast<float*>(a.trans)[(int64_t)m * a.n_nu + i] = expf(-t * (float)a.inv_mu[m]);
       if (a.tau) static_cast<float*>(a.tau)[i] = t;
@@ -675,6 +901,11 @@ hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0) return hipSuccess;
   const int64_t n_blocks = a.n_wg * a.layer_groups;
   if (a.layer_groups < 1 || n_blocks * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
+  if (precision != LBL_FP64 && a.n_layers > 0) {
+    const int64_t n_far = a.n_wg * a.n_layers;
+    if (n_far * kMixedBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
+    k_far_series_mixed<kPointsMixed><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
+  }
   if (precision == LBL_FP64)
     k_voigt_sum_fp64<kPointsFp64><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
   else if (precision == LBL_SURROGATE)
