@@ -30,7 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Voigt line-gridpoint evals/s at max|dT|<=1e-5 vs FP64; surrogate vs plain FP32"
-FLOP_PER_EVAL = 16            # far-wing rational form, every mode charged the same
+FLOP_PER_EVAL = 16            # per point-by-point evaluation, every mode charged the same
+# FP32 operations per (record, wavefront) pair of the mixed far-field series:
+# x, D0, rcp, the three recurrence factors and the first two terms (12), then
+# 16 terms of mul + fma + add (64)
+FLOP_PER_SERIES_PAIR = 76
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -58,15 +62,18 @@ def time_mode(prob, precision, mus, steps, warmup, capi, torch):
     outs = None
     k_ms = []
     lp_ms = []
+    split = []
     t0 = time.perf_counter()
     for _ in range(steps):
         outs = prob.run(precision, mus=mus)
         a, b = capi.last_timings()
         lp_ms.append(a)
         k_ms.append(b)
+        split.append(capi.last_timings_split())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     capi.enable_timing(False)
+    time_mode.split_ms = tuple(float(v) for v in np.mean(np.asarray(split), axis=0))
     return outs, wall / steps, float(np.mean(k_ms)), float(np.mean(lp_ms))
 
 
@@ -428,16 +435,41 @@ def run(args, pool) -> int:
         "line_params_kernel_ms": lp_ms,
     }
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
-    achieved = E * FLOP_PER_EVAL / (k_ms * 1e-3) / 1e12
-    traffic = None
+    traffic = {}
     tf_path = os.path.join(ROOT, "profiles", "traffic_voigt_sum_five_gas.json")
     if os.path.exists(tf_path):
         with open(tf_path) as f:
-            traffic = json.load(f).get(args.precision)
-    result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                          "frac": achieved / peak, "traffic": traffic,
-                          "kernel": "k_voigt_sum_" + ("fp64" if args.precision == "fp64" else "mixed"),
-                          "flop_per_eval": FLOP_PER_EVAL}
+            traffic = json.load(f)
+    if args.precision == "fp64":
+        achieved = E * FLOP_PER_EVAL / (k_ms * 1e-3) / 1e12
+        result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                              "frac": achieved / peak, "traffic": traffic.get("fp64"),
+                              "kernel": "k_voigt_sum_fp64", "flop_per_eval": FLOP_PER_EVAL}
+    else:
+        # two kernels: the far-field series (units: (record, wavefront) pairs)
+        # and the near-field line sum (units: point-by-point evaluations)
+        far_ms, near_ms = time_mode.split_ms
+        work = prob.count_series_work()
+        kern = {
+            "k_far_series_mixed": {"ms": far_ms, "units": work["far_pairs"],
+                                   "unit": "(record, wavefront) series pairs",
+                                   "flop_per_unit": FLOP_PER_SERIES_PAIR},
+            "k_voigt_sum_" + ("surrogate" if args.precision == "surrogate" else "mixed"):
+                {"ms": near_ms, "units": work["near_evals"], "unit": "point evaluations",
+                 "flop_per_unit": FLOP_PER_EVAL},
+        }
+        for name, d in kern.items():
+            d["achieved_tflops"] = d["units"] * d["flop_per_unit"] / (max(d["ms"], 1e-9) * 1e-3) / 1e12
+            d["frac"] = d["achieved_tflops"] / peak
+            d["traffic"] = traffic.get(name)
+        dom = max(kern, key=lambda n: kern[n]["ms"])
+        d = kern[dom]
+        result["roofline"] = {"bound": "valu", "achieved": d["achieved_tflops"], "peak": peak,
+                              "unit": "TFLOP/s", "frac": d["frac"], "traffic": d["traffic"],
+                              "kernel": dom, "units_per_launch": d["units"],
+                              "flop_per_unit": d["flop_per_unit"], "kernels": kern}
+        log(f"[bench] kernels: " + ", ".join(f"{n} {v['ms']:.2f} ms ({v['units']:.3e} units, "
+                                             f"frac {v['frac']:.3f})" for n, v in kern.items()))
 
     if args.precision == "mixed" and not args.no_surrogate:
         result["surrogate"] = surrogate_report(prob, E, value, mus, max(2, args.steps // 2), _capi,

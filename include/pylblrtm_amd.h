@@ -221,6 +221,10 @@ int lbl_par_read(const char *path, int64_t capacity, int64_t *n_text_lines, int8
  * their durations in ms for the calling thread's last call. */
 void lbl_enable_timing(int on);
 int lbl_last_timings(double *ms_line_params, double *ms_voigt_sum);
+/* The line-sum time split (no counterpart in the reference; for
+ * benchmarks): mixed modes run a far-field series kernel before the
+ * near-field line sum; ms_far_series is 0 in FP64 mode. */
+int lbl_last_timings_split(double *ms_far_series, double *ms_near_sum);
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char *lbl_last_error(void);

@@ -18,7 +18,7 @@ LBL_OK, LBL_EINVAL, LBL_ESTRENGTH, LBL_ENONFINITE, LBL_EHIP = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": 0, "mixed": 1, "surrogate": 2}
 
 EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_last_error",
-            "lbl_enable_timing", "lbl_last_timings", "lbl_vjp_workspace_bytes",
+            "lbl_enable_timing", "lbl_last_timings", "lbl_last_timings_split", "lbl_vjp_workspace_bytes",
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
             "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
             "lbl_optical_depth_surrogate", "lbl_par_read")
@@ -115,6 +115,8 @@ def load() -> ctypes.CDLL:
     lib.lbl_enable_timing.restype = None
     lib.lbl_last_timings.argtypes = [P(ctypes.c_double), P(ctypes.c_double)]
     lib.lbl_last_timings.restype = ctypes.c_int
+    lib.lbl_last_timings_split.argtypes = [P(ctypes.c_double), P(ctypes.c_double)]
+    lib.lbl_last_timings_split.restype = ctypes.c_int
     lib.lbl_par_read.argtypes = [ctypes.c_char_p, ctypes.c_int64, P(ctypes.c_int64), _vp, _vp,
                                  _vp]
     lib.lbl_par_read.restype = ctypes.c_int
@@ -149,4 +151,13 @@ def last_timings():
     calling thread's last lbl_optical_depth call with timing enabled."""
     a, b = ctypes.c_double(0), ctypes.c_double(0)
     load().lbl_last_timings(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def last_timings_split():
+    """(ms of the far-field series kernel, ms of the near-field line sum and
+    epilogue) of the calling thread's last lbl_optical_depth call with timing
+    enabled; together they make up last_timings()[1]."""
+    a, b = ctypes.c_double(0), ctypes.c_double(0)
+    load().lbl_last_timings_split(ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value

@@ -16,9 +16,10 @@ thread_local char g_last_error[512] = "";
 // the caller's stream (one event set per device, created on first use).
 constexpr int kMaxDevices = 64;
 bool g_timing = false;
-hipEvent_t g_events[kMaxDevices][4];
+hipEvent_t g_events[kMaxDevices][5];
 bool g_events_made[kMaxDevices] = {};
 thread_local double g_ms_line_params = 0.0, g_ms_voigt_sum = 0.0;
+thread_local double g_ms_far = 0.0, g_ms_near = 0.0;
 
 hipError_t timing_events(hipEvent_t** ev) {
   int dev = 0;
@@ -26,7 +27,7 @@ hipError_t timing_events(hipEvent_t** ev) {
   if (e != hipSuccess) return e;
   if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
   if (!g_events_made[dev]) {
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
       e = hipEventCreate(&g_events[dev][i]);
       if (e != hipSuccess) return e;
     }
@@ -162,6 +163,12 @@ int lbl_last_timings(double* ms_line_params, double* ms_voigt_sum) {
   return LBL_OK;
 }
 
+int lbl_last_timings_split(double* ms_far_series, double* ms_near_sum) {
+  if (ms_far_series) *ms_far_series = g_ms_far;
+  if (ms_near_sum) *ms_near_sum = g_ms_near;
+  return LBL_OK;
+}
+
 int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
                         const lbl_grid_t* grid, int precision, size_t* bytes) {
   Layout L;
@@ -282,6 +289,8 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     sa.exact_count = reinterpret_cast<unsigned long long*>(sur->n_exact_near);
   }
   if (ev) HIP_TRY(hipEventRecord(ev[2], s));
+  HIP_TRY(lbl::launch_far_series(sa, precision, s));
+  if (ev) HIP_TRY(hipEventRecord(ev[4], s));
   HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));
   HIP_TRY(lbl::launch_tau_epilogue(sa, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[3], s));
@@ -295,6 +304,11 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     HIP_TRY(hipEventElapsedTime(&b, ev[2], ev[3]));
     g_ms_line_params = a;
     g_ms_voigt_sum = b;
+    float f = 0.f, n = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, ev[2], ev[4]));
+    HIP_TRY(hipEventElapsedTime(&n, ev[4], ev[3]));
+    g_ms_far = f;
+    g_ms_near = n;
   }
   if (code != lbl::kErrNone) {
     const int kind2 = (int)(code >> lbl::kErrGroupBits);

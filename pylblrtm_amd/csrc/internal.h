@@ -191,6 +191,8 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_line_params(const LineParamsArgs& a, int precision, hipStream_t s);
 hipError_t launch_tile_ranges(const TileRangeArgs& a, hipStream_t s);
 hipError_t launch_wave_ranges(const WaveRangeArgs& a, hipStream_t s);
+// mixed modes: k_far_series_mixed, before launch_voigt_sum (no-op for FP64)
+hipError_t launch_far_series(const SumArgs& a, int precision, hipStream_t s);
 hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s);
 // Sum of the layer-group partials, T = exp(-tau/mu) (only when layer_groups > 1).
 hipError_t launch_tau_epilogue(const SumArgs& a, int precision, hipStream_t s);

@@ -345,9 +345,9 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
   return ac * mlp::kref(x, y) <= gate;
 }
 
-// Mixed-precision line sum: kMixedBlock threads x P points per workgroup.
+// Mixed-precision line sums: kMixedBlock threads x P points per workgroup.
 // Records arrive precomputed (MixedRec); staging is a copy plus the tile's
-// exact inclusion test, two records per thread per chunk of kChunk.
+// exact inclusion test.
 constexpr int kMixedBlock = 256;
 
 // Far-field series of the mixed line sums (see kSeriesRatio): one
@@ -544,15 +544,18 @@ void k_far_series_mixed(const SumArgs a) {
 
 template <int P, bool kSurrogate>
 __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
-  __shared__ MixedRec s_rec[kChunk];
-  // Near-centre work queue: (record, thread) pairs whose P points are not all
-  // in one of the far regions are appended here instead of being evaluated by
-  // a wavefront that is otherwise on the fast path, then drained densely by
-  // the whole workgroup at the end of the chunk.  Results come back through
-  // s_acc (LDS float atomics) into the owning thread's partial sums.
-  __shared__ uint16_t s_q[kQueueCap];
+  // Every wavefront works alone (no workgroup barriers): it streams its own
+  // near range through its own LDS stage of 64 records, and (record, lane)
+  // pairs whose P points are not all in one of the far-field regions go to
+  // its own queue, drained by the same wavefront after each stage with the
+  // full region dispatch; results come back through s_acc (LDS float
+  // atomics) into the owning lane's partial sums.
+  constexpr int kWaves = kMixedBlock / 64;
+  constexpr int kStage = 64;                        // records per wavefront stage
+  constexpr int kQueueCapW = kQueueCap / kWaves;    // queue entries per wavefront
+  __shared__ MixedRec s_rec[kWaves][kStage];
+  __shared__ uint16_t s_q[kWaves][kQueueCapW];
   __shared__ float s_acc[kMixedBlock * P];
-  __shared__ int s_qn;
 
   // (layer group, tile slice), group-major so that an XCD holds neighbouring
   // tiles of one layer range
@@ -566,9 +569,9 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
   const int64_t s0 = t0 + (int64_t)slice * (kMixedBlock * P);   // first point of the slice
-  // a slice wholly past the end of a short (last) tile has no points: the
-  // whole workgroup leaves before its first barrier
-  if (s0 >= t1) return;
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  // a wavefront wholly past the end of a short (last) tile has no points
+  if (s0 + (int64_t)wave * kWavePoints >= t1) return;
   const int64_t i0 = s0 + (int64_t)threadIdx.x * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
@@ -578,76 +581,55 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
   const bool live = i0 < t1;   // lanes wholly past the tile end never write
   const MixedRec* __restrict__ recs = a.rec.mix;
-  // s_rec as an LDS-qualified pointer, so record reads stay ds_read_b128
-  lds_mixed_rec* const s_rec_lds = (lds_mixed_rec*)s_rec;
-
-  // This wavefront's near range [nlo, nhi) (records within kSeriesRatio H
-  // of its middle point, by nu0; k_far_series_mixed has summed all others);
-  // the workgroup stages the union [ulo, uhi) of its wavefronts' ranges.
-  constexpr int kWaves = kMixedBlock / 64;
-  const int wave = (int)threadIdx.x >> 6;
-  int32_t nlo = rlo, nhi = rlo, ulo = rhi, uhi = rlo;
+  // This wavefront's near range: records within kSeriesRatio H of its middle
+  // point by nu0 (k_far_series_mixed has summed all others of the tile).
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  // this wavefront's stage as an LDS-qualified pointer, so record reads stay ds_read_b128
+  lds_mixed_rec* const s_rec_lds = (lds_mixed_rec*)s_rec[wave];
+  float* const acc = s_acc + threadIdx.x * P;
 #pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    const int32_t l = min(max(a.wlo[wg * kWaves + w], rlo), rhi);
-    const int32_t h = max(min(a.whi[wg * kWaves + w], rhi), l);
-    if (s0 + (int64_t)w * kWavePoints < t1 && h > l) {
-      ulo = min(ulo, l);
-      uhi = max(uhi, h);
-      if (w == wave) { nlo = l; nhi = h; }
-    } else if (w == wave) {
-      nlo = nhi = l;
-    }
-  }
-  if (uhi < ulo) uhi = ulo;
-
-  for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0f;
-  if (threadIdx.x == 0) s_qn = 0;
+  for (int j = 0; j < P; ++j) acc[j] = 0.0f;
 
   // column totals (compensated) live in tau_part[grp], read and rewritten
   // once per layer, so that no registers hold them through the record loops
   float2* const tau_acc = static_cast<float2*>(a.tau_part) + (int64_t)grp * a.n_nu;
 
   for (int k = k_begin; k < k_end; ++k) {
-    float lay[P], layc[P];
+    float lay[P], layc[P], part[P];
 #pragma unroll
-    for (int j = 0; j < P; ++j) lay[j] = layc[j] = 0.0f;
+    for (int j = 0; j < P; ++j) lay[j] = layc[j] = part[j] = 0.0f;
     const int64_t kb = (int64_t)k * a.n_lines;
-    // Near field: the union of the wavefronts' near ranges through LDS; each
-    // wavefront takes its own range's records of every chunk.
-    float part[P];
-#pragma unroll
-    for (int j = 0; j < P; ++j) part[j] = 0.0f;
-    for (int32_t base = ulo; base < uhi; base += kChunk) {
-      const int cnt = min(kChunk, uhi - base);
-      __syncthreads();
-#pragma unroll
-      for (int h = 0; h < kChunk / kMixedBlock; ++h) {
-        const int q = threadIdx.x + h * kMixedBlock;
-        if (q < cnt) {
-          MixedRec m = recs[kb + base + q];
-          if (!(m.c >= lo_b && m.c <= hi_b)) {  // outside the tile window (or skipped):
-            // x = 0 passes the far test and adds ay / (0 + c1) = 0 exactly
-            m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.c1 = 1.0f;
-            m.ay = 0.0f; m.hw1 = 0.0f; m.a = 0.0f;
-          }
-          s_rec[q] = m;
+    for (int32_t base = nlo; base < nhi; base += kStage) {
+      const int cnt = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
+      // stage: lane l copies record base + l (the previous stage's readers
+      // are this wavefront's own, earlier in program order)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      if (lane < cnt) {
+        MixedRec m = recs[kb + base + lane];
+        if (!(m.c >= lo_b && m.c <= hi_b)) {  // outside the tile window (or skipped):
+          // x = 0 passes the far test and adds ay / (0 + c1) = 0 exactly
+          m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.c1 = 1.0f;
+          m.ay = 0.0f; m.hw1 = 0.0f; m.a = 0.0f;
         }
+        s_rec[wave][lane] = m;
       }
-      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      int nq = 0;   // queue entries of this stage (uniform)
       // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
       // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
       // keeps each a single ds_read_b128.  Three tiers per (record, thread):
       // the shifted one-node form, the middle tier (second-order shifted form
       // for y^2 < kF32S2Y2, else two-node), and the work queue.
       const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
-      const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
       auto record = [&](const int q, const u32x4 h0, const volatile lds_u32x4* const body) {
         const double c = __hiloint2double((int)h0.y, (int)h0.x);
         const float xm = (float)(nu_m - c) * __uint_as_float(h0.z);
         const u32x4 h1 = *body;
         const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
         const float ay = __uint_as_float(h1.z);
+        bool queued = false;
         if (fabsf(xm) >= __uint_as_float(h0.w)) {
           // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
 #pragma unroll
@@ -679,91 +661,94 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
               part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
             }
           }
-        } else if (live) {
-          // append (q, thread) with one LDS atomic per wavefront
-          const uint64_t msk = __ballot(1);
+        } else {
+          queued = live;
+        }
+        // append the queued lanes' (q, lane) to this wavefront's queue; the
+        // count nq advances in every lane (outside the divergent tiers)
+        const uint64_t msk = __ballot(queued);
+        if (msk) {   // uniform
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
               (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-          int qb = 0;
-          if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(msk));
-          qb = __builtin_amdgcn_readfirstlane(qb);
-          const int pos = qb + (int)rank;
-          if (pos < kQueueCap) {
-            s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
-          } else {  // queue full: evaluate here
-            const float y = s_rec[q].y, am = s_rec[q].a;
+          const int pos = nq + (int)rank;
+          if (queued) {
+            if (pos < kQueueCapW) {
+              s_q[wave][pos] = (uint16_t)((q << 6) | lane);
+            } else {  // queue full: evaluate here
+              const float y = s_rec[wave][q].y, am = s_rec[wave][q].a;
 #pragma unroll 1
-            for (int j = 0; j < P; ++j) {
-              const float v = am * K_mixed(fmaf((float)j - 0.5f * (float)(P - 1), hs, xm), y);
+              for (int j = 0; j < P; ++j) {
+                const float v = am * K_mixed(fmaf((float)j - 0.5f * (float)(P - 1), hs, xm), y);
 #pragma unroll
-              for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
+                for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
+              }
             }
           }
+          nq += (int)__popcll(msk);
         }
       };
       // Two records per iteration, each header read two records ahead into
-      // its own registers (no copies between iterations); kChunk entries
+      // its own registers (no copies between iterations); kStage entries
       // exist, so record 1 is readable even when cnt == 1.  The record
       // address lives in a VGPR (the empty asm makes it one) so every LDS
       // read uses an immediate offset instead of a copy from an SGPR.
-      const int qlo = __builtin_amdgcn_readfirstlane(min(max(nlo - base, 0), ncnt));
-      const int qhi = __builtin_amdgcn_readfirstlane(min(max(nhi - base, 0), ncnt));
-      if (qlo < qhi) {
-        u32x4 ha = rb[3 * qlo], hb = rb[3 * min(qlo + 1, kChunk - 1)];
-        const volatile lds_u32x4* pr = rb + 3 * qlo;
-        asm volatile("" : "+v"(pr));
-        for (int q = qlo; q < qhi; q += 2, pr += 6) {
-          record(q, ha, pr + 1);
-          if (q + 2 < qhi) ha = pr[6];
-          if (q + 1 < qhi) {
-            record(q + 1, hb, pr + 4);
-            if (q + 3 < qhi) hb = pr[9];
-          }
-          if (((q - qlo + 2) & (kFoldEvery - 1)) == 0) {   // after every 32 records
+      u32x4 ha = rb[0], hb = rb[3];
+      const volatile lds_u32x4* pr = rb;
+      asm volatile("" : "+v"(pr));
+      for (int q = 0; q < cnt; q += 2, pr += 6) {
+        record(q, ha, pr + 1);
+        if (q + 2 < cnt) ha = pr[6];
+        if (q + 1 < cnt) {
+          record(q + 1, hb, pr + 4);
+          if (q + 3 < cnt) hb = pr[9];
+        }
+        if (((q + 2) & (kFoldEvery - 1)) == 0) {   // after records 31, 63
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-              kahan_add(lay[j], layc[j], part[j]);
-              part[j] = 0.0f;
-            }
+          for (int j = 0; j < P; ++j) {
+            kahan_add(lay[j], layc[j], part[j]);
+            part[j] = 0.0f;
           }
         }
       }
-      __syncthreads();
-      const int nq = min(s_qn, kQueueCap);
-      if (nq > 0) {
-        for (int e = threadIdx.x; e < nq; e += kMixedBlock) {
-          const uint32_t ent = s_q[e];
-          const int q = (int)(ent >> 8), t = (int)(ent & 0xFF);
-          const MixedRec& R = s_rec[q];
-          const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P);
+      nq = min(nq, kQueueCapW);
+      if (nq > 0) {   // uniform
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // dense drain by this wavefront: entry = (record, lane); points at
+        // their exact grid values
+        for (int e = lane; e < nq; e += 64) {
+          const uint32_t ent = s_q[wave][e];
+          const int q = (int)(ent >> 6), t = (int)(ent & 63);
+          const MixedRec& R = s_rec[wave][q];
+          const int tt = wave * 64 + t;   // owning thread
+          const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P);
           const float x0 = (float)(nu_t - R.c) * R.s;
           const float ac = R.a * (float)R.c;
           int exact = 0;
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, R.hs, x0);
             if (kSurrogate && surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
-            atomicAdd(&s_acc[t * P + j], R.a * K_mixed(x, R.y));
+            atomicAdd(&s_acc[tt * P + j], R.a * K_mixed(x, R.y));
             ++exact;
           }
           if (kSurrogate && a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
         }
         if constexpr (kSurrogate) {
-          // network pass: each wavefront takes 64 / P queue entries (64
-          // points) per round (lane = entry (l / P), point (l % P)); all lanes
-          // stay active through the MFMA evaluation.
+          // network pass: 64 / P queue entries (64 points) per round (lane =
+          // entry (l / P), point (l % P)); all lanes stay active through the
+          // MFMA evaluation.
           constexpr int kPer = 64 / P;
-          const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-          for (int g = wave; g * kPer < nq; g += kMixedBlock / 64) {
+          for (int g = 0; g * kPer < nq; ++g) {
             const int e = g * kPer + lane / P, j = lane % P;
             bool use = false;
             float x = 0.0f, y = 1.0f, am = 0.0f;
-            int t = 0;
+            int tt = 0;
             if (e < nq) {
-              const uint32_t ent = s_q[e];
-              const int q = (int)(ent >> 8);
-              t = (int)(ent & 0xFF);
-              const MixedRec& R = s_rec[q];
-              const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P);
+              const uint32_t ent = s_q[wave][e];
+              const int q = (int)(ent >> 6);
+              tt = wave * 64 + (int)(ent & 63);
+              const MixedRec& R = s_rec[wave][q];
+              const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P);
               x = fmaf((float)j, R.hs, (float)(nu_t - R.c) * R.s);
               y = R.y;
               am = R.a;
@@ -774,17 +759,17 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             if (a.mlp_count && lane == 0) atomicAdd(a.mlp_count, (unsigned long long)__popcll(users));
             const float u = use ? mlp::input_u(x) : 0.0f;
             const float v = use ? mlp::input_v(__log10f(y)) : 0.0f;
-            const float tt = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
-            if (use) atomicAdd(&s_acc[t * P + j], am * mlp::kref(x, y) * __expf(tt));
+            const float tv = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
+            if (use) atomicAdd(&s_acc[tt * P + j], am * mlp::kref(x, y) * __expf(tv));
           }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          part[j] += s_acc[threadIdx.x * P + j];
-          s_acc[threadIdx.x * P + j] = 0.0f;
+          part[j] += acc[j];
+          acc[j] = 0.0f;
         }
-        if (threadIdx.x == 0) s_qn = 0;
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
@@ -805,10 +790,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         const float R = nu * tanhf(0.5f * (float)kC2 * nu / (float)T);
         const float v = (lay[j] - layc[j]) * R;
         bad |= !isfinite(v);
-        float2 acc = make_float2(0.0f, 0.0f);
-        if (k > k_begin) acc = tau_acc[i];
-        kahan_add(acc.x, acc.y, v);
-        tau_acc[i] = acc;
+        float2 acc2 = make_float2(0.0f, 0.0f);
+        if (k > k_begin) acc2 = tau_acc[i];
+        kahan_add(acc2.x, acc2.y, v);
+        tau_acc[i] = acc2;
         if (a.tau_layers) static_cast<float*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
       }
     }
@@ -820,8 +805,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     if (i < t1) {
       if (k_begin == k_end) tau_acc[i] = make_float2(0.0f, 0.0f);   // no layers in this group
       if (a.layer_groups > 1) continue;   // k_tau_epilogue32 adds the groups
-      const float2 acc = tau_acc[i];
-      const float t = acc.x - acc.y;
+      const float2 acc2 = tau_acc[i];
+      const float t = acc2.x - acc2.y;
       for (int m = 0; m < a.n_mu; ++m)
         static_cast<float*>(a.trans)[(int64_t)m * a.n_nu + i] = expf(-t * (float)a.inv_mu[m]);
       if (a.tau) static_cast<float*>(a.tau)[i] = t;
@@ -897,15 +882,18 @@ int sum_points_per_thread(int precision) {
 
 int sum_block(int precision) { return precision == LBL_FP64 ? kBlock : kMixedBlock; }
 
+hipError_t launch_far_series(const SumArgs& a, int precision, hipStream_t s) {
+  if (a.n_wg <= 0 || a.n_layers <= 0 || precision == LBL_FP64) return hipSuccess;
+  const int64_t n_far = a.n_wg * a.n_layers;
+  if (n_far * kMixedBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
+  k_far_series_mixed<kPointsMixed><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
+  return hipGetLastError();
+}
+
 hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0) return hipSuccess;
   const int64_t n_blocks = a.n_wg * a.layer_groups;
   if (a.layer_groups < 1 || n_blocks * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
-  if (precision != LBL_FP64 && a.n_layers > 0) {
-    const int64_t n_far = a.n_wg * a.n_layers;
-    if (n_far * kMixedBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
-    k_far_series_mixed<kPointsMixed><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
-  }
   if (precision == LBL_FP64)
     k_voigt_sum_fp64<kPointsFp64><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
   else if (precision == LBL_SURROGATE)

@@ -302,6 +302,53 @@ class OpticalDepthProblem:
                 total += int(np.sum(np.maximum(cnt, 0) * npts))
         return total
 
+    def count_series_work(self) -> dict:
+        """Work units of the mixed line sum, split as the kernels do.  Each
+        wavefront owns WAVE_POINTS consecutive points of a slice of
+        SLICE_POINTS points of a tile; its near range is the lines whose nu0
+        lies within SERIES_RATIO * H (+ the shift bound) of its middle point,
+        H = WAVE_POINTS / 2 * step.  near_evals: (layer, near line in the
+        tile window, point) triples, evaluated point by point; far_pairs:
+        (layer, other line in the tile window, wavefront) pairs, each summed
+        as one series.  Groups with a zero column are left out (they add 0)."""
+        wave_pts, slice_pts, ratio = 512, 2048, 3.0
+        t0, t1, lo_b, hi_b = self.tile_edges()
+        h_nu0, h_shift, h_group = self._host_lines()
+        # wavefronts: (tile, slice, wave) with a live first point
+        spt = (self.tile_samples + slice_pts - 1) // slice_pts
+        offs = np.arange(spt * (slice_pts // wave_pts), dtype=np.int64) * wave_pts
+        first = (t0[:, None] + offs[None, :]).ravel()
+        tile_of = np.repeat(np.arange(len(t0)), len(offs))
+        live = first < t1[tile_of]
+        first, tile_of = first[live], tile_of[live]
+        pts = np.minimum(t1[tile_of] - first, wave_pts)
+        nu_w = self.nu[first] + 0.5 * (wave_pts - 1) * self.step
+        half = ratio * 0.5 * wave_pts * self.step + abs(self.shift_bound) * (1 + 1e-9) + 1e-9
+        near_evals = far_pairs = 0
+        lay = self.layers
+        for k in range(self.n_layers):
+            P = float(lay.pressure_atm[k])
+            win = np.zeros(len(t0), np.int64)
+            nu0_on = []
+            for j, g in enumerate(self.groups):
+                if float(lay.column_cm2[g.gas][k]) == 0.0:
+                    continue
+                m = h_group == j
+                p_self = float(lay.vmr[g.gas][k]) * P
+                c = np.sort(h_nu0[m] + h_shift[m] * (P - p_self))
+                win += np.maximum(np.searchsorted(c, hi_b, side="right")
+                                  - np.searchsorted(c, lo_b, side="left"), 0)
+                nu0_on.append(h_nu0[m])
+            if not nu0_on:
+                continue
+            nu0 = np.sort(np.concatenate(nu0_on))
+            near = (np.searchsorted(nu0, nu_w + half, side="right")
+                    - np.searchsorted(nu0, nu_w - half, side="left"))
+            near = np.minimum(near, win[tile_of])
+            near_evals += int(np.sum(near * pts))
+            far_pairs += int(np.sum(win[tile_of] - near))
+        return {"near_evals": near_evals, "far_pairs": far_pairs}
+
     # ------------------------------------------------------------------ run --
     def workspace(self, precision: int):
         torch = _torch()
