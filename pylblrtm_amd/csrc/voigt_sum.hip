@@ -630,7 +630,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
         const float ay = __uint_as_float(h1.z);
         bool queued = false;
-        if (fabsf(xm) >= __uint_as_float(h0.w)) {
+        // One tier per record for the whole wavefront where possible: the
+        // middle tier holds wherever the one-node form does, so a record for
+        // which some lane needs it runs it in every lane (one branch, not two).
+        if (__ballot(fabsf(xm) < __uint_as_float(h0.w)) == 0) {
           // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
 #pragma unroll
           for (int j = 0; j < P; ++j) {
