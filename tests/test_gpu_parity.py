@@ -285,3 +285,63 @@ def test_window_without_lines_and_zero_columns(precision):
                               {"CO2": np.zeros_like(prof.pressure_hpa)})
     nu, T = compute_transmittance(sd, zero, 720.0, 722.0, precision=precision)
     assert nu.size == 2001 and np.all(T == 1.0)
+
+
+def test_many_layers_per_workgroup_vs_oracle():
+    """200 layers on a 1,000,001-point grid: 500 tile slices, so the mixed
+    sum runs 132 layer groups and the FP64 sum 33 (workgroups loop over 1-2
+    and 6-7 layers, carrying their column totals from layer to layer).  FP64
+    against the oracle on two tiles, mixed against FP64 on the whole grid."""
+    say("case: 500 lines, 200 layers, 700-1700 @ 0.001, tiles 0/250")
+    rng = np.random.default_rng(31)
+    lines = synth.random_lines(rng, 500, 680.0, 1720.0, 2)
+    sd = SpectralData(lines=lines, tips={(2, 1): synth.tips_table()}, molar_masses={(2, 1): 43.99})
+    p, t, _ = synth.levels(201)
+    lay = AtmosphericProfile(p, t, {"CO2": np.full(201, 4.15e-4)}).build_layers()
+    tiles = [0, 250]
+    nu, od = _oracle_tau(sd, lay, 700.0, 1700.0, 0.001, 25.0, 2.0, tiles=tiles)
+    m = ~np.isnan(od[0])
+    prob = OpticalDepthProblem(sd, lay, 700.0, 1700.0, 0.001, 25.0, 2.0)
+    assert prob.n_nu == 1_000_001 and prob.n_layers == 200
+    o64 = prob.run("fp64", mus=(1.0,), want_tau=True, want_layers=True)
+    odl = o64["optical_depth_layers"].cpu().numpy()
+    rel = np.abs(odl[:, m] - od[:, m]) / od[:, m]
+    say(f"  fp64 max rel tau per layer {rel.max():.2e}")
+    assert rel.max() < TAU_RTOL_FP64
+    tau64 = o64["optical_depth"].cpu().numpy()
+    np.testing.assert_allclose(tau64[m], od[:, m].sum(0), rtol=TAU_RTOL_FP64)
+    om = prob.run("mixed", mus=(1.0, 0.5), want_tau=True)
+    taum = om["optical_depth"].double().cpu().numpy()
+    relm = np.abs(taum - tau64) / tau64
+    say(f"  mixed max rel column tau (whole grid) {relm.max():.2e}")
+    assert relm.max() < 2e-5
+    for k, mu in enumerate((1.0, 0.5)):
+        Tm = om["transmittance"][k].double().cpu().numpy()
+        assert np.max(np.abs(Tm - np.exp(-tau64 / mu))) <= DT_ATOL_MIXED
+
+
+def test_cutoff_below_near_field_radius_vs_oracle():
+    """Cutoff 0.5 cm-1, below the mixed sum's near-field radius (3 x 256 grid
+    steps = 0.768 cm-1 at step 0.001): near-field records reach the edges of
+    the tile window, whose exact inclusion test must still hold."""
+    say("case: 400 lines, 700-704 @ 0.001, cutoff 0.5, 5 layers")
+    rng = np.random.default_rng(37)
+    lines = synth.random_lines(rng, 400, 699.0, 705.0, 2)
+    sd = SpectralData(lines=lines, tips={(2, 1): synth.tips_table()}, molar_masses={(2, 1): 43.99})
+    p, t, _ = synth.levels(6)
+    lay = AtmosphericProfile(p, t, {"CO2": np.full(6, 4.15e-4)}).build_layers()
+    nu, od = _oracle_tau(sd, lay, 700.0, 704.0, 0.001, 0.5, 2.0)
+    prob = OpticalDepthProblem(sd, lay, 700.0, 704.0, 0.001, 0.5, 2.0)
+    pos = od > 0
+    say(f"  points outside every line's reach: {np.count_nonzero(~pos)} of {od.size}")
+    o64 = prob.run("fp64", mus=(1.0,), want_layers=True)["optical_depth_layers"].cpu().numpy()
+    assert np.all(o64[~pos] == 0.0)
+    rel = np.abs(o64[pos] - od[pos]) / od[pos]
+    say(f"  fp64 max rel tau per layer {rel.max():.2e}")
+    assert rel.max() < TAU_RTOL_FP64
+    om = prob.run("mixed", mus=(1.0,), want_layers=True)["optical_depth_layers"].double().cpu().numpy()
+    assert np.all(om[~pos] == 0.0)
+    big = od > 1e-30   # above FP32's normal range
+    relm = np.abs(om[big] - od[big]) / od[big]
+    say(f"  mixed max rel tau per layer {relm.max():.2e}")
+    assert relm.max() < 2e-5
