@@ -107,10 +107,11 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   // (the mixed sums keep their column totals here even with one group)
   if (L->layer_groups > 1 || (!fp64 && !with_grad))
     off = align_up(off + (size_t)std::max<int32_t>(L->layer_groups, 1) * (size_t)grid->n_nu * 8);
-  // mixed modes: far-field sum per (layer, point), FP32
+  // far-field sum per (layer, point): FP32 (mixed modes) or FP64
   L->far = off;
-  if (!fp64 && !with_grad)
-    off = align_up(off + (size_t)std::max<int32_t>(layers->n_layers, 0) * (size_t)grid->n_nu * 4);
+  if (!with_grad)
+    off = align_up(off + (size_t)std::max<int32_t>(layers->n_layers, 0) * (size_t)grid->n_nu *
+                             (fp64 ? 8 : 4));
   for (int i = 0; i < 5; ++i) {
     L->g[i] = off;
     if (with_grad) off = align_up(off + nrec * 4);
@@ -238,7 +239,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   // nu0 is within kSeriesRatio * H (+ the shift bound) of its middle point
   int32_t* wlo = reinterpret_cast<int32_t*>(ws + L.wlo);
   int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
-  if (precision != LBL_FP64) {
+  {
     lbl::WaveRangeArgs wr{};
     wr.nu0 = lines->nu0;
     wr.n_lines = lines->n_lines;
@@ -274,7 +275,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   if (sa.n_wg > INT32_MAX) return fail(LBL_EINVAL, "grid too large");
   sa.layer_groups = L.layer_groups;
   sa.tau_part = ws + L.tau_part;
-  sa.far_sum = reinterpret_cast<float*>(ws + L.far);
+  sa.far_sum = ws + L.far;
   sa.n_mu = out->n_mu;
   for (int m = 0; m < out->n_mu; ++m) sa.inv_mu[m] = 1.0 / out->mu[m];
   sa.trans = out->transmittance;

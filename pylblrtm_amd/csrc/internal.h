@@ -58,7 +58,8 @@ constexpr int kPointsMixed = 8;
 // coefficients, reduced across the wavefront and evaluated at each point.
 constexpr int kWavePoints = 64 * kPointsMixed;
 constexpr double kSeriesRatio = 3.0;
-constexpr int kSeriesTerms = 18;
+constexpr int kSeriesTerms = 18;     // mixed: one-node form, FP32
+constexpr int kSeriesTerms64 = 28;   // FP64: second-order shifted form
 
 // Middle of the wavefront whose first grid point is i: grid(i) + (kWavePoints-1)/2 step.
 __device__ __forceinline__ double wave_middle(double nu_min, double step, int64_t i) {
@@ -166,9 +167,9 @@ struct SumArgs {
   // line sum itself.
   int32_t layer_groups;
   void* tau_part;
-  // mixed modes: far-field sum per (layer, point), written by
-  // k_far_series_mixed and added by the line sum
-  float* far_sum;
+  // far-field sum per (layer, point), FP32 (mixed modes) or FP64, written
+  // by k_far_series_* and added by the line sum
+  void* far_sum;
 };
 
 // Workgroups a line-sum launch aims for.  Many short workgroups (a layer

@@ -41,6 +41,8 @@ namespace {
 constexpr int kChunk = kBlock;  // records staged per LDS round
 constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
 constexpr int kQueueCap = 8192; // near-centre (record, thread) pairs per chunk (16-bit entries)
+constexpr int kExCap = 128;     // far-field exception list entries per wavefront
+constexpr int kPointsFp64 = 8;  // grid points per thread of the FP64 line sums
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
@@ -91,10 +93,16 @@ constexpr int kQueueCap64 = 3840;
 template <int P>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_fp64(const SumArgs a) {
-  __shared__ Rec64 s_rec[kChunk];
-  __shared__ uint16_t s_q[kQueueCap64];
+  // every wavefront works alone on its own near range (k_far_series_fp64 has
+  // summed the rest of the tile): its own LDS stage of kStage records, its
+  // own queue drained by itself, no workgroup barriers
+  constexpr int kWaves = kBlock / 64;
+  constexpr int kStage = 64;
+  constexpr int kQueueCapW = kQueueCap64 / kWaves;
+  __shared__ Rec64 s_rec[kWaves][kStage];
+  __shared__ uint16_t s_q[kWaves][kQueueCapW];
   __shared__ double s_acc[kBlock * P];
-  __shared__ int s_qn;
+  __shared__ int s_qn[kWaves];
 
   // (layer group, tile slice), group-major so that an XCD holds neighbouring
   // tiles of one layer range
@@ -108,9 +116,9 @@ void k_voigt_sum_fp64(const SumArgs a) {
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
   const int64_t s0 = t0 + (int64_t)slice * (kBlock * P);
-  // a slice wholly past the end of a short (last) tile has no points: the
-  // whole workgroup leaves before its first barrier
-  if (s0 >= t1) return;
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  // a wavefront wholly past the end of a short (last) tile has no points
+  if (s0 + (int64_t)wave * kWavePoints >= t1) return;
   const int64_t i0 = s0 + (int64_t)threadIdx.x * P;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
@@ -122,9 +130,11 @@ void k_voigt_sum_fp64(const SumArgs a) {
   const double* ra = static_cast<const double*>(a.rec.a);
   const double half = 0.5 * (P - 1);
   const double t0sq = kGh4T[0] * kGh4T[0], t1sq = kGh4T[1] * kGh4T[1];
-
-  for (int j = 0; j < P; ++j) s_acc[threadIdx.x * P + j] = 0.0;
-  if (threadIdx.x == 0) s_qn = 0;
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  double* const acc = s_acc + threadIdx.x * P;
+  for (int j = 0; j < P; ++j) acc[j] = 0.0;
+  if (lane == 0) s_qn[wave] = 0;
 
   double tau[P];
 #pragma unroll
@@ -135,11 +145,11 @@ void k_voigt_sum_fp64(const SumArgs a) {
 #pragma unroll
     for (int j = 0; j < P; ++j) lay[j] = 0.0;
     const int64_t kb = (int64_t)k * a.n_lines;
-    for (int32_t base = rlo; base < rhi; base += kChunk) {
-      const int cnt = min(kChunk, rhi - base);
-      __syncthreads();
-      if ((int)threadIdx.x < cnt) {
-        const int64_t r = kb + base + threadIdx.x;
+    for (int32_t base = nlo; base < nhi; base += kStage) {
+      const int cnt = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      if (lane < cnt) {
+        const int64_t r = kb + base + lane;
         double c = a.rec.c[r], sv = rs[r], y = ry[r], am = ra[r];
         if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped): adds 0
           c = 0.0; sv = 0.0; y = 1.0e3; am = 0.0;
@@ -151,14 +161,15 @@ void k_voigt_sum_fp64(const SumArgs a) {
         R.hw2 = fma(half, hs, sqrt(fmax(s2_r2_min64(y2) - y2, 0.0)));
         R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
         R.c1 = y2 - 1.5; R.ay = am * y * kInvSqrtPi;
-        s_rec[threadIdx.x] = R;
+        s_rec[wave][lane] = R;
       }
-      __syncthreads();
-      const int ncnt = __builtin_amdgcn_readfirstlane(cnt);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const int ncnt = cnt;
       // Records are read from LDS as 16-byte pieces: (c, s) and (hw1, hs) two
       // records ahead into their own registers (two records per iteration),
       // (c1, ay) at the record and (hw2, hw4) only off the one-node branch.
-      const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)(lds_rec64*)s_rec;
+      const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)(lds_rec64*)s_rec[wave];
       auto f64 = [](unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); };
       auto record = [&](const int q, const u32x4 h0, const u32x4 h1,
                         const volatile lds_u32x4* const more) {
@@ -248,11 +259,11 @@ void k_voigt_sum_fp64(const SumArgs a) {
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                 (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
             int qb = 0;
-            if (rank == 0) qb = atomicAdd(&s_qn, (int)__popcll(msk));
+            if (rank == 0) qb = atomicAdd(&s_qn[wave], (int)__popcll(msk));
             qb = __builtin_amdgcn_readfirstlane(qb);
             const int pos = qb + (int)rank;
-            if (pos < kQueueCap64) {
-              s_q[pos] = (uint16_t)((q << 8) | (int)threadIdx.x);
+            if (pos < kQueueCapW) {
+              s_q[wave][pos] = (uint16_t)((q << 6) | lane);
             } else {  // queue full: evaluate here
               const int64_t r = kb + base + q;
               const double y = ry[r], am = ra[r];
@@ -278,38 +289,43 @@ void k_voigt_sum_fp64(const SumArgs a) {
           if (q + 3 < ncnt) { hb0 = pr[12]; hb1 = pr[13]; }
         }
       }
-      __syncthreads();
-      const int nq = min(s_qn, kQueueCap64);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const int nq = __builtin_amdgcn_readfirstlane(min(s_qn[wave], kQueueCapW));
       if (nq > 0) {
-        // dense drain: entry e = (record, thread); points at their exact grid values
-        for (int e = threadIdx.x; e < nq; e += kBlock) {
-          const uint32_t ent = s_q[e];
-          const int q = (int)(ent >> 8), t = (int)(ent & 0xFF);
-          const Rec64& R = s_rec[q];
+        // dense drain by this wavefront: entry = (record, lane); points at
+        // their exact grid values
+        for (int e = lane; e < nq; e += 64) {
+          const uint32_t ent = s_q[wave][e];
+          const int q = (int)(ent >> 6), tt = wave * 64 + (int)(ent & 63);
+          const Rec64& R = s_rec[wave][q];
           const int64_t r = kb + base + q;
           const double y = ry[r], am = ra[r];
           for (int j = 0; j < P; ++j) {
-            const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)t * P + j) - R.c) * R.s;
-            atomicAdd(&s_acc[t * P + j], am * K_fp64_call(x, y));
+            const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P + j) - R.c) * R.s;
+            atomicAdd(&s_acc[tt * P + j], am * K_fp64_call(x, y));
           }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          lay[j] += s_acc[threadIdx.x * P + j];
-          s_acc[threadIdx.x * P + j] = 0.0;
+          lay[j] += acc[j];
+          acc[j] = 0.0;
         }
-        if (threadIdx.x == 0) s_qn = 0;
+        if (lane == 0) s_qn[wave] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       }
     }
     const double T = a.t_k[k];
     bool bad = false;
+    const double* const far_k = static_cast<const double*>(a.far_sum) + (int64_t)k * a.n_nu;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const int64_t i = i0 + j;
       if (i < t1) {
         const double nu = grid_nu(a.nu_min, a.step, i);
-        const double v = lay[j] * (nu * tanh(0.5 * kC2 * nu / T));
+        const double v = (lay[j] + far_k[i]) * (nu * tanh(0.5 * kC2 * nu / T));
         bad |= !isfinite(v);
         tau[j] += v;
         if (a.tau_layers) static_cast<double*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
@@ -345,6 +361,150 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
   return ac * mlp::kref(x, y) <= gate;
 }
 
+// FP64 far-field series (FP64 mode; see kSeriesRatio): as k_far_series_mixed
+// with the second-order shifted form, whose FP64 region (error <= 1e-11)
+// starts much nearer than the one-node form's:
+//   A K = ay r / D, r = (xm + sh tau)^2 + y^2 = r0 + a1 tau + a2 tau^2,
+//   D = r^2 - 3/2 r + 2 y^2 - 3/2 = d0 + d1 tau + ... + d4 tau^4,
+// whose Taylor coefficients follow from D Q = ay r:
+//   Q_n = (ay r_n + e1 Q_(n-1) + e2 Q_(n-2) + e3 Q_(n-3) + e4 Q_(n-4)) / ..., e_k = -d_k / d0.
+// The poles lie at |tau| >= 1/0.345 (kSeriesRatio H), so kSeriesTerms64
+// terms leave <= 1e-11 of each record's value.  A record is summed this way
+// iff the form holds at all of the wavefront's points: with x_n = s (|nu_w - c| - H),
+// (x_n^2 + y^2)^3 >= kFp64S2Coef (9 y^2 + 21/4) (the form's threshold, cubed);
+// others are evaluated point by point with the full FP64 dispatch.
+template <int P>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void k_far_series_fp64(const SumArgs a) {
+  static_assert(P == kPointsFp64 && 64 * P == kWavePoints, "wavefront geometry");
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int32_t s_ex[kWaves][kExCap];
+  const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
+  const int k = (int)(lw / a.n_wg);
+  const int64_t wg = lw - (int64_t)k * a.n_wg;
+  const int64_t tile = wg / a.slices_per_tile;
+  const int slice = (int)(wg - tile * a.slices_per_tile);
+  const int64_t t0 = tile * a.ts;
+  const int64_t t1 = min(t0 + a.ts, a.n_nu);
+  const int64_t s0 = t0 + (int64_t)slice * (kBlock * P);
+  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
+  if (w_first >= t1) return;   // the whole wavefront: no points
+  const int64_t i0 = w_first + (int64_t)lane * P;
+  const double nu_w = wave_middle(a.nu_min, a.step, w_first);
+  const double H = 0.5 * kWavePoints * a.step;
+  const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
+  const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
+  const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  const int64_t kb = (int64_t)k * a.n_lines;
+  const double* __restrict__ rc = a.rec.c + kb;
+  const double* __restrict__ rs = static_cast<const double*>(a.rec.s) + kb;
+  const double* __restrict__ ry = static_cast<const double*>(a.rec.y) + kb;
+  const double* __restrict__ ra = static_cast<const double*>(a.rec.a) + kb;
+  double C[kSeriesTerms64];
+#pragma unroll
+  for (int n = 0; n < kSeriesTerms64; ++n) C[n] = 0.0;
+  const int32_t n_below = nlo - rlo;
+  const int32_t n_far = n_below + (rhi - nhi);
+  auto far_index = [&](int32_t v) -> int64_t {
+    return v < n_below ? (int64_t)rlo + v : (int64_t)nhi + (v - n_below);
+  };
+  double lay[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) lay[j] = 0.0;
+  // records of s_ex[wave][0, n) at every lane's own points, full dispatch
+  auto far_exceptions = [&](const int n) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the lanes' list writes
+    for (int e = 0; e < n; ++e) {
+      const int32_t idx = __builtin_amdgcn_readfirstlane(s_ex[wave][e]);
+      const double c = rc[idx], sv = rs[idx], y = ry[idx], am = ra[idx];
+#pragma unroll 1
+      for (int j = 0; j < P; ++j) {
+        const double v = am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - c) * sv, y);
+#pragma unroll
+        for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
+      }
+    }
+  };
+  int n_ex = 0;
+  double nc = __builtin_nan(""), ns = 0.0, ny = 1.0, na = 0.0;
+  if (lane < n_far) {
+    const int64_t i = far_index(lane);
+    nc = rc[i]; ns = rs[i]; ny = ry[i]; na = ra[i];
+  }
+  const int32_t n_it = (n_far + 63) >> 6;
+  for (int32_t it = 0; it < n_it; ++it) {
+    const double c = nc, sv = ns, y = ny, am = na;
+    const int32_t vn = (it + 1) * 64 + lane;
+    if (vn < n_far) {
+      const int64_t i = far_index(vn);
+      nc = rc[i]; ns = rs[i]; ny = ry[i]; na = ra[i];
+    }
+    const bool inw = (it * 64 + lane < n_far) & (c >= lo_b) & (c <= hi_b);
+    const double d = nu_w - c;
+    const double y2 = y * y;
+    const double xn = sv * (fabs(d) - H);
+    const double rn = fma(xn, xn, y2);
+    const bool ser = inw & (rn * rn * rn >= kFp64S2Coef * fma(9.0, y2, 5.25));
+    if (ser) {
+      const double xm = d * sv, sh = sv * H;
+      const double a1 = 2.0 * xm * sh, a2 = sh * sh;
+      const double r0 = fma(xm, xm, y2);
+      const double inv = rcp64_1nr(fma(r0, r0 - 1.5, fma(2.0, y2, -1.5)));
+      const double t = fma(2.0, r0, -1.5);
+      const double e1 = -(t * a1) * inv, e2 = -fma(a1, a1, t * a2) * inv;
+      const double e3 = -(2.0 * a1 * a2) * inv, e4 = -(a2 * a2) * inv;
+      const double g = am * y * kInvSqrtPi * inv;
+      double q0 = g * r0;
+      double q1 = fma(e1, q0, g * a1);
+      double q2 = fma(e1, q1, fma(e2, q0, g * a2));
+      double q3 = fma(e1, q2, fma(e2, q1, e3 * q0));
+      C[0] += q0;
+      C[1] += q1;
+      C[2] += q2;
+      C[3] += q3;
+#pragma unroll
+      for (int n = 4; n < kSeriesTerms64; ++n) {
+        const double qn = fma(e1, q3, fma(e2, q2, fma(e3, q1, e4 * q0)));
+        C[n] += qn;
+        q0 = q1; q1 = q2; q2 = q3; q3 = qn;
+      }
+    }
+    const bool exc = inw & !ser;
+    const uint64_t ex = __ballot(exc);
+    if (ex) {   // uniform
+      if (n_ex + 64 > kExCap) {
+        far_exceptions(n_ex);
+        n_ex = 0;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
+      if (exc) s_ex[wave][n_ex + (int)rank] = (int32_t)far_index(it * 64 + lane);
+      n_ex += (int)__popcll(ex);
+    }
+  }
+  far_exceptions(n_ex);
+  // totals over the wavefront's 64 lanes, in every lane
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int n = 0; n < kSeriesTerms64; ++n) C[n] += __shfl_xor(C[n], off);
+  }
+  double* const out = static_cast<double*>(a.far_sum) + (int64_t)k * a.n_nu;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const double t = ((double)(lane * P + j) - 0.5 * (double)(kWavePoints - 1)) *
+                     (2.0 / (double)kWavePoints);
+    double f = C[kSeriesTerms64 - 1];
+#pragma unroll
+    for (int n = kSeriesTerms64 - 2; n >= 0; --n) f = fma(f, t, C[n]);
+    const int64_t i = i0 + j;
+    if (i < t1) out[i] = lay[j] + f;
+  }
+}
+
 // Mixed-precision line sums: kMixedBlock threads x P points per workgroup.
 // Records arrive precomputed (MixedRec); staging is a copy plus the tile's
 // exact inclusion test.
@@ -355,7 +515,6 @@ constexpr int kMixedBlock = 256;
 // barriers).  Writes far_sum[k][i], the layer-k sum over all records of the
 // tile outside the wavefront's near range, compensated, for every point of
 // the tile; the line sum adds it before the radiation term.
-constexpr int kExCap = 128;   // far-field exception list entries per wavefront
 
 template <int P>
 __global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
@@ -537,7 +696,7 @@ void k_far_series_mixed(const SumArgs a) {
 #pragma unroll
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
-    if (i < t1) a.far_sum[(int64_t)k * a.n_nu + i] = lay[j] - layc[j];
+    if (i < t1) static_cast<float*>(a.far_sum)[(int64_t)k * a.n_nu + i] = lay[j] - layc[j];
   }
 }
 
@@ -782,7 +941,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     }
     const double T = a.t_k[k];
     bool bad = false;
-    const float* const far_k = a.far_sum + (int64_t)k * a.n_nu;
+    const float* const far_k = static_cast<const float*>(a.far_sum) + (int64_t)k * a.n_nu;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const int64_t i = i0 + j;
@@ -875,7 +1034,6 @@ __global__ void k_tau_epilogue32(const SumArgs a) {
   if (a.tau) static_cast<float*>(a.tau)[i] = t;
 }
 
-constexpr int kPointsFp64 = 8;
 
 }  // namespace
 
@@ -886,10 +1044,13 @@ int sum_points_per_thread(int precision) {
 int sum_block(int precision) { return precision == LBL_FP64 ? kBlock : kMixedBlock; }
 
 hipError_t launch_far_series(const SumArgs& a, int precision, hipStream_t s) {
-  if (a.n_wg <= 0 || a.n_layers <= 0 || precision == LBL_FP64) return hipSuccess;
+  if (a.n_wg <= 0 || a.n_layers <= 0) return hipSuccess;
   const int64_t n_far = a.n_wg * a.n_layers;
-  if (n_far * kMixedBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
-  k_far_series_mixed<kPointsMixed><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
+  if (n_far * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
+  if (precision == LBL_FP64)
+    k_far_series_fp64<kPointsFp64><<<(unsigned)n_far, kBlock, 0, s>>>(a);
+  else
+    k_far_series_mixed<kPointsMixed><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
   return hipGetLastError();
 }
 
