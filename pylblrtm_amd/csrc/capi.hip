@@ -72,7 +72,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-  size_t c, s, y, a, lo, hi, wlo, whi, err, tau_part, far, total;
+  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, err, tau_part, far, total;
   size_t g[5];   // adjoint coefficient arrays (only with_grad)
   int64_t n_tiles, n_wg;
   int32_t slices_per_tile, layer_groups;
@@ -101,6 +101,8 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   const size_t n_waves = (size_t)L->n_wg * (size_t)(lbl::sum_block(precision) / 64);
   L->wlo = off; off = align_up(off + n_waves * 4);
   L->whi = off; off = align_up(off + n_waves * 4);
+  L->slo = off; off = align_up(off + (size_t)L->n_wg * 4);
+  L->shi = off; off = align_up(off + (size_t)L->n_wg * 4);
   L->err = off; off = align_up(off + 4);
   // partial tau per layer group (float sum + compensation, or double): 8 B
   L->tau_part = off;
@@ -239,6 +241,8 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   // nu0 is within kSeriesRatio * H (+ the shift bound) of its middle point
   int32_t* wlo = reinterpret_cast<int32_t*>(ws + L.wlo);
   int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
+  int32_t* slo = reinterpret_cast<int32_t*>(ws + L.slo);
+  int32_t* shi = reinterpret_cast<int32_t*>(ws + L.shi);
   {
     lbl::WaveRangeArgs wr{};
     wr.nu0 = lines->nu0;
@@ -246,12 +250,15 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     wr.nu_min = grid->nu_min;
     wr.step = grid->step;
     wr.half_width = lbl::kSeriesRatio * 0.5 * lbl::kWavePoints * grid->step + tr.margin;
+    wr.slice_half_width = lbl::kSeriesRatio * 0.5 * lbl::kSlicePoints * grid->step + tr.margin;
     wr.ts = grid->tile_samples;
     wr.n_wg = L.n_wg;
     wr.slices_per_tile = L.slices_per_tile;
     wr.waves_per_slice = lbl::sum_block(precision) / 64;
     wr.wlo = wlo;
     wr.whi = whi;
+    wr.slo = slo;
+    wr.shi = shi;
     HIP_TRY(lbl::launch_wave_ranges(wr, s));
   }
 
@@ -261,6 +268,8 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   sa.hi = hi;
   sa.wlo = wlo;
   sa.whi = whi;
+  sa.slo = slo;
+  sa.shi = shi;
   sa.t_k = layers->t_k;
   sa.n_lines = lines->n_lines;
   sa.n_layers = layers->n_layers;

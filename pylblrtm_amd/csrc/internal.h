@@ -57,6 +57,10 @@ constexpr int kPointsMixed = 8;
 // series in tau, kSeriesTerms terms, is summed per lane into kSeriesTerms
 // coefficients, reduced across the wavefront and evaluated at each point.
 constexpr int kWavePoints = 64 * kPointsMixed;
+// A second level for the farther records: the series about a slice's middle
+// (kSlicePoints points, half-span Hs = kSlicePoints/2 * step = 4 H), for
+// records at least kSeriesRatio * Hs away, summed once per slice.
+constexpr int kSlicePoints = 4 * kWavePoints;
 constexpr double kSeriesRatio = 3.0;
 constexpr int kSeriesTerms = 18;     // mixed: one-node form, FP32
 constexpr int kSeriesTerms64 = 28;   // FP64: second-order shifted form
@@ -65,6 +69,11 @@ constexpr int kSeriesTerms64 = 28;   // FP64: second-order shifted form
 __device__ __forceinline__ double wave_middle(double nu_min, double step, int64_t i) {
   return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
                    0.5 * (double)(kWavePoints - 1) * step);
+}
+// Middle of the slice whose first grid point is i: grid(i) + (kSlicePoints-1)/2 step.
+__device__ __forceinline__ double slice_middle(double nu_min, double step, int64_t i) {
+  return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
+                   0.5 * (double)(kSlicePoints - 1) * step);
 }
 
 // Per-(layer, line) record of the mixed-precision paths (48 bytes, one
@@ -121,25 +130,31 @@ struct TileRangeArgs {
   int32_t* hi;
 };
 
-// Near range of each wavefront of the line sum: [wlo, whi) are the records
-// whose nu0 lies within half_width of the wavefront's middle point, one
-// entry per (workgroup slice, wavefront) in launch order.
+// Near ranges of the far-field series: [wlo, whi) are the records whose nu0
+// lies within half_width of a wavefront's middle point, one entry per
+// (workgroup slice, wavefront) in launch order; [slo, shi) those within
+// slice_half_width of a slice's middle point (kSlicePoints points), one per
+// slice.
 struct WaveRangeArgs {
   const double* nu0;   // sorted ascending
   int64_t n_lines;
-  double nu_min, step, half_width;
+  double nu_min, step, half_width, slice_half_width;
   int64_t ts, n_wg;
   int32_t slices_per_tile, waves_per_slice;
   int32_t* wlo;
   int32_t* whi;
+  int32_t* slo;
+  int32_t* shi;
 };
 
 struct SumArgs {
   Records rec;
   const int32_t* lo;
   const int32_t* hi;
-  const int32_t* wlo;   // mixed modes: near range of every wavefront
+  const int32_t* wlo;   // near range of every wavefront (far-field series)
   const int32_t* whi;
+  const int32_t* slo;   // near range of every slice
+  const int32_t* shi;
   const double* t_k;
   int64_t n_lines;
   int32_t n_layers;

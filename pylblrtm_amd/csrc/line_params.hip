@@ -216,6 +216,23 @@ __global__ void k_wave_ranges(const WaveRangeArgs a) {
   }
   a.wlo[g] = (int32_t)first_rec;
   a.whi[g] = (int32_t)lo;
+  if (w == 0) {   // the slice's near range
+    const double nu_s = slice_middle(a.nu_min, a.step, tile * a.ts + slice * (int64_t)kSlicePoints);
+    const double lo_s = nu_s - a.slice_half_width, hi_s = nu_s + a.slice_half_width;
+    int64_t l = 0, h = a.n_lines;
+    while (l < h) {
+      const int64_t mid = (l + h) >> 1;
+      if (a.nu0[mid] < lo_s) l = mid + 1; else h = mid;
+    }
+    const int64_t f = l;
+    h = a.n_lines;
+    while (l < h) {
+      const int64_t mid = (l + h) >> 1;
+      if (a.nu0[mid] <= hi_s) l = mid + 1; else h = mid;
+    }
+    a.slo[wg] = (int32_t)f;
+    a.shi[wg] = (int32_t)l;
+  }
 }
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {

@@ -510,20 +510,43 @@ void k_far_series_fp64(const SumArgs a) {
 // exact inclusion test.
 constexpr int kMixedBlock = 256;
 
-// Far-field series of the mixed line sums (see kSeriesRatio): one
-// workgroup per (slice, layer), its four wavefronts independent (no
-// barriers).  Writes far_sum[k][i], the layer-k sum over all records of the
-// tile outside the wavefront's near range, compensated, for every point of
-// the tile; the line sum adds it before the radiation term.
-
+// Far-field series of the mixed line sums (see kSeriesRatio and
+// kSlicePoints): one workgroup per (slice, layer).  Writes far_sum[k][i], the
+// layer-k sum over every record of the tile outside the near range of the
+// wavefront that owns point i, compensated; the line sum adds it before the
+// radiation term.
+//
+// Each record's one-node form is summed as its Taylor series in the
+// normalised offset tau from a middle point nu_m of half-span Hu:
+//   ay / ((xm + sh tau)^2 + c1) = sum_n Q_n tau^n,  xm = s (nu_m - c), sh = s Hu,
+//   Q_0 = ay / D0, Q_1 = al Q_0, Q_n = al Q_(n-1) + be Q_(n-2),
+//   D0 = xm^2 + c1, al = -2 xm sh / D0, be = -sh^2 / D0.
+// |nu_m - c| >= kSeriesRatio Hu puts both poles at |tau| >= 1/0.345 (0.345:
+// y^2 < 3/2, |xm| >= 25 + sh), so the terms dropped after kSeriesTerms weigh
+// <= 2.6e-7 of the record's value at any |tau| < 1.  Two levels: the records
+// outside the slice's near range [slo, shi) about the slice's middle
+// (Hu = Hs, all 256 threads, one record per thread), then those between the
+// slice's and the wavefront's near ranges about the wavefront's middle
+// (Hu = H, one record per lane).  The form must hold at every point the
+// series covers: |nu_m - c| >= ds1 + (Hu - H).  A record that fails this at
+// the slice level goes on to the wavefront level; one that fails it there is
+// evaluated at every lane's own points with the near field's tiers.
 template <int P>
 __global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_far_series_mixed(const SumArgs a) {
-  static_assert(P == kPointsMixed && 64 * P == kWavePoints, "wavefront geometry");
+  static_assert(P == kPointsMixed && 64 * P == kWavePoints && kMixedBlock * P == kSlicePoints,
+                "slice and wavefront geometry");
   constexpr int kWaves = kMixedBlock / 64;
+  constexpr int kSliceExCap = 1024;
+  constexpr int kN = kSeriesTerms;
   // per wavefront: records nearer than their ds1, evaluated point by point
-  // after (or, when the list fills, during) the loop
+  // after (or, when the list fills, during) its loop
   __shared__ int32_t s_ex[kWaves][kExCap];
+  // slice level: records nearer than ds1 + (Hs - H), for the wavefront level
+  __shared__ int32_t s_sx[kSliceExCap];
+  __shared__ int s_sxn;
+  // slice-level coefficients per wavefront (Q_0 and Q_1 with compensation)
+  __shared__ float s_coef[kWaves][kN + 2];
   // (layer, slice), layer-major so that an XCD holds neighbouring tiles of a layer
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
   const int k = (int)(lw / a.n_wg);
@@ -532,134 +555,80 @@ void k_far_series_mixed(const SumArgs a) {
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
-  const int64_t s0 = t0 + (int64_t)slice * (kMixedBlock * P);
+  const int64_t s0 = t0 + (int64_t)slice * kSlicePoints;
+  if (s0 >= t1) return;   // the whole workgroup: no points (before the barriers)
   const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
   const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
-  if (w_first >= t1) return;   // the whole wavefront: no points
+  const bool wave_live = w_first < t1;   // uniform across the wavefront
   const int64_t i0 = w_first + (int64_t)lane * P;
+  const double nu_s = slice_middle(a.nu_min, a.step, s0);
   const double nu_w = wave_middle(a.nu_min, a.step, w_first);
   const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
-  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
-  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  const int32_t slo = min(max(a.slo[wg], rlo), rhi);
+  const int32_t shi = max(min(a.shi[wg], rhi), slo);
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], slo), shi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], shi), nlo);
+  const float dS = 0.5f * (float)(kSlicePoints - kWavePoints) * (float)a.step;   // Hs - H
   const int64_t kb = (int64_t)k * a.n_lines;
-  const MixedRec* __restrict__ recs = a.rec.mix;
-  float lay[P], layc[P];
+  const MixedRec* __restrict__ lrec = a.rec.mix + kb;
+  // the fields the series needs, prefetched one iteration ahead
+  struct FarRec { double c; float s, c1, ay, ds1, sh; };
+  auto far_load = [&](int64_t i) {
+    const MixedRec* const p = lrec + i;
+    FarRec f;
+    f.c = p->c; f.s = p->s; f.c1 = p->c1; f.ay = p->ay; f.ds1 = p->ds1; f.sh = p->sh;
+    return f;
+  };
+  // adds record m's series about a middle point at offset d = nu_m - c (units
+  // of the level: sh scaled by the caller) to C (Q_0, Q_1 into C[0], C[1])
+  auto series = [&](float (&C)[kN], const FarRec& m, float d, float sh) {
+    const float xm = d * m.s;
+    const float r = __builtin_amdgcn_rcpf(fmaf(xm, xm, m.c1));
+    const float u = sh * r;
+    const float al = -2.0f * xm * u, be = -sh * u;
+    float qa = m.ay * r, qb = al * qa;
+    C[0] += qa;
+    C[1] += qb;
 #pragma unroll
-  for (int j = 0; j < P; ++j) lay[j] = layc[j] = 0.0f;
-    // Far field: every record of the tile outside [nlo, nhi), one per lane,
-    // straight from global memory (the next one loaded while this one is
-    // summed).  A record inside the tile window with |nu_w - c| >= ds1 adds
-    // its one-node form's Taylor series in tau to the lane's coefficients:
-    //   ay / ((xm + sh tau)^2 + c1) = sum_n Q_n tau^n,
-    //   Q_0 = ay / D0, Q_1 = al Q_0, Q_n = al Q_(n-1) + be Q_(n-2),
-    //   D0 = xm^2 + c1, al = -2 xm sh / D0, be = -sh^2 / D0.
-    // |nu_w - c| >= kSeriesRatio H puts both poles of the form at
-    // |tau| >= 1/0.345 (0.345: y^2 < 3/2, |xm| >= 25 + sh), so the terms
-    // dropped after kSeriesTerms weigh <= 2.6e-7 of the record's value at
-    // any |tau| < 1.  A record that is nearer than ds1 (a broad line whose
-    // one-node region starts farther out) is evaluated at every lane's own
-    // points instead, with the same tiers as the near field.
-    float C[kSeriesTerms];
+    for (int n = 2; n < kN; ++n) {
+      const float qn = fmaf(al, qb, be * qa);
+      C[n] += qn;
+      qa = qb;
+      qb = qn;
+    }
+  };
+  if (threadIdx.x == 0) s_sxn = 0;
+  __syncthreads();
+
+  // ---- slice level: records outside [slo, shi), one per thread
+  {
+    float C[kN];
 #pragma unroll
-    for (int n = 0; n < kSeriesTerms; ++n) C[n] = 0.0f;
+    for (int n = 0; n < kN; ++n) C[n] = 0.0f;
     float S0 = 0.0f, S0c = 0.0f, S1 = 0.0f, S1c = 0.0f;   // compensated Q_0, Q_1 sums
-    const int32_t n_below = nlo - rlo;
-    const int32_t n_far = n_below + (rhi - nhi);
-    const MixedRec* __restrict__ lrec = recs + kb;
-    auto far_index = [&](int32_t v) -> int64_t {
-      return v < n_below ? (int64_t)rlo + v : (int64_t)nhi + (v - n_below);
+    const int32_t n_below = slo - rlo;
+    const int32_t n_sl = n_below + (rhi - shi);
+    auto sl_index = [&](int32_t v) -> int64_t {
+      return v < n_below ? (int64_t)rlo + v : (int64_t)shi + (v - n_below);
     };
-    // the five fields the series needs, prefetched one iteration ahead
-    struct FarRec { double c; float s, c1, ay, ds1, sh; };
-    auto far_load = [&](int64_t i) {
-      const MixedRec* const p = lrec + i;
-      FarRec f;
-      f.c = p->c; f.s = p->s; f.c1 = p->c1; f.ay = p->ay; f.ds1 = p->ds1; f.sh = p->sh;
-      return f;
-    };
-    // records of s_ex[wave][0, n) at every lane's own points (record index
-    // uniform across the wavefront), into the compensated layer sum
-    auto far_exceptions = [&](const int n) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the lanes' list writes
-      for (int e = 0; e < n; ++e) {
-        const int32_t idx = __builtin_amdgcn_readfirstlane(s_ex[wave][e]);
-        const MixedRec R = lrec[idx];
-        const float xm = (float)(nu_m - R.c) * R.s;
-        float v[P];
-        if (fabsf(xm) >= R.hw1) {
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
-            v[j] = R.ay * __builtin_amdgcn_rcpf(fmaf(x, x, R.c1));
-          }
-        } else if (fabsf(xm) >= R.hw2) {
-          const bool s2 = R.c1 < kF32S2Y2 - 1.5f;
-          const float y2 = R.c1 + 1.5f, cc = fmaf(2.0f, R.c1, 1.5f);
-          const float cw = R.c1 + 1.0f, d2 = fmaf(2.0f, R.c1, 3.0f);
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
-            if (s2) {   // second-order shifted form
-              const float r = fmaf(x, x, y2);
-              v[j] = R.ay * r * __builtin_amdgcn_rcpf(fmaf(r, r - 1.5f, cc));
-            } else {    // two-node form
-              const float w = fmaf(x, x, cw);
-              v[j] = fmaf(R.ay, w, R.ay) * __builtin_amdgcn_rcpf(fmaf(w, w, d2));
-            }
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < P; ++j)
-            v[j] = R.a * K_mixed_call(fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm), R.y);
-        }
-#pragma unroll
-        for (int j = 0; j < P; ++j) kahan_add(lay[j], layc[j], v[j]);
-      }
-    };
-    int n_ex = 0;
     FarRec nx;
     nx.c = __builtin_nan("");
-    if (lane < n_far) nx = far_load(far_index(lane));
-    const int32_t n_it = (n_far + 63) >> 6;
+    if ((int)threadIdx.x < n_sl) nx = far_load(sl_index(threadIdx.x));
+    const int32_t n_it = (n_sl + kMixedBlock - 1) / kMixedBlock;
     for (int32_t it = 0; it < n_it; ++it) {
       const FarRec m = nx;
-      const int32_t vn = (it + 1) * 64 + lane;
-      if (vn < n_far) nx = far_load(far_index(vn));
-      const float d = (float)(nu_w - m.c);
-      // in range, inside the tile window, and far enough for the series (no
-      // short-circuit: one branch)
-      const bool inw = (it * 64 + lane < n_far) & (m.c >= lo_b) & (m.c <= hi_b);
-      const bool ser = inw & (fabsf(d) >= m.ds1);
-      if (ser) {
-        const float xm = d * m.s;
-        const float r = __builtin_amdgcn_rcpf(fmaf(xm, xm, m.c1));
-        const float u = m.sh * r;
-        const float al = -2.0f * xm * u, be = -m.sh * u;
-        float qa = m.ay * r, qb = al * qa;
-        C[0] += qa;
-        C[1] += qb;
-#pragma unroll
-        for (int n = 2; n < kSeriesTerms; ++n) {
-          const float qn = fmaf(al, qb, be * qa);
-          C[n] += qn;
-          qa = qb;
-          qb = qn;
-        }
-      }
-      const bool exc = inw & !ser;
-      const uint64_t ex = __ballot(exc);
-      if (ex) {   // uniform
-        if (n_ex + 64 > kExCap) {
-          far_exceptions(n_ex);
-          n_ex = 0;
-        }
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
-        if (exc) s_ex[wave][n_ex + (int)rank] = (int32_t)far_index(it * 64 + lane);
-        n_ex += (int)__popcll(ex);
+      const int32_t v = it * kMixedBlock + (int)threadIdx.x;
+      if (v + kMixedBlock < n_sl) nx = far_load(sl_index(v + kMixedBlock));
+      const float d = (float)(nu_s - m.c);
+      const bool inw = (v < n_sl) & (m.c >= lo_b) & (m.c <= hi_b);
+      const bool ser = inw & (fabsf(d) >= m.ds1 + dS);
+      if (ser) series(C, m, d, 4.0f * m.sh);
+      if (inw & !ser) {   // rare: on to the wavefront level
+        const int pos = atomicAdd(&s_sxn, 1);
+        if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
       }
       if ((it & 15) == 15) {
         kahan_add(S0, S0c, C[0]);
@@ -667,10 +636,8 @@ void k_far_series_mixed(const SumArgs a) {
         C[0] = C[1] = 0.0f;
       }
     }
-    far_exceptions(n_ex);
     kahan_add(S0, S0c, C[0]);
     kahan_add(S1, S1c, C[1]);
-    // totals over the wavefront's 64 lanes, in every lane
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
       S0 += __shfl_xor(S0, off);
@@ -678,28 +645,173 @@ void k_far_series_mixed(const SumArgs a) {
       S1 += __shfl_xor(S1, off);
       S1c += __shfl_xor(S1c, off);
 #pragma unroll
-      for (int n = 2; n < kSeriesTerms; ++n) C[n] += __shfl_xor(C[n], off);
+      for (int n = 2; n < kN; ++n) C[n] += __shfl_xor(C[n], off);
     }
-    const float s1t = S1 - S1c;
+    if (lane == 0) {
+      s_coef[wave][0] = S0;
+      s_coef[wave][1] = S0c;
+      s_coef[wave][2] = S1;
+      s_coef[wave][3] = S1c;
+#pragma unroll
+      for (int n = 2; n < kN; ++n) s_coef[wave][n + 2] = C[n];
+    }
+  }
+  __syncthreads();
+  if (!wave_live) return;   // the last barrier is behind every wavefront
+
+  float lay[P], layc[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) lay[j] = layc[j] = 0.0f;
+  {
+    // the slice series at this lane's points
+    float T[kN + 2];
+#pragma unroll
+    for (int n = 0; n < kN + 2; ++n) {
+      float t = 0.0f;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) t += s_coef[w][n];
+      T[n] = t;
+    }
+    const float s1t = T[2] - T[3];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      // tau of point lane*P + j of the wavefront (exact in FP32)
-      const float t = ((float)(lane * P + j) - 0.5f * (float)(kWavePoints - 1)) *
-                      (2.0f / (float)kWavePoints);
-      float f = C[kSeriesTerms - 1];
+      // tau of point wave*kWavePoints + lane*P + j of the slice (exact in FP32)
+      const float t = ((float)(wave * kWavePoints + lane * P + j) - 0.5f * (float)(kSlicePoints - 1)) *
+                      (2.0f / (float)kSlicePoints);
+      float f = T[kN + 1];
 #pragma unroll
-      for (int n = kSeriesTerms - 2; n >= 2; --n) f = fmaf(f, t, C[n]);
+      for (int n = kN - 2; n >= 2; --n) f = fmaf(f, t, T[n + 2]);
       f = fmaf(f, t, s1t) * t;
-      kahan_add(lay[j], layc[j], S0);
-      kahan_add(lay[j], layc[j], f - S0c);
+      kahan_add(lay[j], layc[j], T[0]);
+      kahan_add(lay[j], layc[j], f - T[1]);
     }
+  }
+
+  // ---- wavefront level: [slo, nlo) and [nhi, shi), then the slice level's
+  // exceptions; if their list overflowed, the slice-level range again with
+  // every record that passed there skipped
+  float C[kN];
+#pragma unroll
+  for (int n = 0; n < kN; ++n) C[n] = 0.0f;
+  float S0 = 0.0f, S0c = 0.0f, S1 = 0.0f, S1c = 0.0f;
+  const int n_sx = __builtin_amdgcn_readfirstlane(s_sxn);
+  const bool rescan = n_sx > kSliceExCap;
+  const int32_t nA = nlo - slo, nB = shi - nhi, nC = min(n_sx, kSliceExCap);
+  const int32_t nD = rescan ? (slo - rlo) + (rhi - shi) : 0;
+  const int32_t n_far = nA + nB + nC + nD;
+  auto far_index = [&](int32_t v) -> int64_t {
+    if (v < nA) return (int64_t)slo + v;
+    v -= nA;
+    if (v < nB) return (int64_t)nhi + v;
+    v -= nB;
+    if (v < nC) return (int64_t)s_sx[v];
+    v -= nC;
+    return v < slo - rlo ? (int64_t)rlo + v : (int64_t)shi + (v - (slo - rlo));
+  };
+  // records of s_ex[wave][0, n) at every lane's own points (record index
+  // uniform across the wavefront), into the compensated layer sum
+  auto far_exceptions = [&](const int n) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the lanes' list writes
+    for (int e = 0; e < n; ++e) {
+      const int32_t idx = __builtin_amdgcn_readfirstlane(s_ex[wave][e]);
+      const MixedRec R = lrec[idx];
+      const float xm = (float)(nu_m - R.c) * R.s;
+      float v[P];
+      if (fabsf(xm) >= R.hw1) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+          v[j] = R.ay * __builtin_amdgcn_rcpf(fmaf(x, x, R.c1));
+        }
+      } else if (fabsf(xm) >= R.hw2) {
+        const bool s2 = R.c1 < kF32S2Y2 - 1.5f;
+        const float y2 = R.c1 + 1.5f, cc = fmaf(2.0f, R.c1, 1.5f);
+        const float cw = R.c1 + 1.0f, d2 = fmaf(2.0f, R.c1, 3.0f);
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+          if (s2) {   // second-order shifted form
+            const float r = fmaf(x, x, y2);
+            v[j] = R.ay * r * __builtin_amdgcn_rcpf(fmaf(r, r - 1.5f, cc));
+          } else {    // two-node form
+            const float w = fmaf(x, x, cw);
+            v[j] = fmaf(R.ay, w, R.ay) * __builtin_amdgcn_rcpf(fmaf(w, w, d2));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < P; ++j)
+          v[j] = R.a * K_mixed_call(fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm), R.y);
+      }
+#pragma unroll
+      for (int j = 0; j < P; ++j) kahan_add(lay[j], layc[j], v[j]);
+    }
+  };
+  int n_ex = 0;
+  FarRec nx;
+  nx.c = __builtin_nan("");
+  if (lane < n_far) nx = far_load(far_index(lane));
+  const int32_t n_it = (n_far + 63) >> 6;
+  for (int32_t it = 0; it < n_it; ++it) {
+    const FarRec m = nx;
+    const int32_t v = it * 64 + lane;
+    if (v + 64 < n_far) nx = far_load(far_index(v + 64));
+    const float d = (float)(nu_w - m.c);
+    bool inw = (v < n_far) & (m.c >= lo_b) & (m.c <= hi_b);
+    // the rescan skips what the slice level summed
+    if (v >= nA + nB + nC) inw &= !(fabsf((float)(nu_s - m.c)) >= m.ds1 + dS);
+    const bool ser = inw & (fabsf(d) >= m.ds1);
+    if (ser) series(C, m, d, m.sh);
+    const bool exc = inw & !ser;
+    const uint64_t ex = __ballot(exc);
+    if (ex) {   // uniform
+      if (n_ex + 64 > kExCap) {
+        far_exceptions(n_ex);
+        n_ex = 0;
+      }
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
+      if (exc) s_ex[wave][n_ex + (int)rank] = (int32_t)far_index(v);
+      n_ex += (int)__popcll(ex);
+    }
+    if ((it & 15) == 15) {
+      kahan_add(S0, S0c, C[0]);
+      kahan_add(S1, S1c, C[1]);
+      C[0] = C[1] = 0.0f;
+    }
+  }
+  far_exceptions(n_ex);
+  kahan_add(S0, S0c, C[0]);
+  kahan_add(S1, S1c, C[1]);
+  // totals over the wavefront's 64 lanes, in every lane
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    S0 += __shfl_xor(S0, off);
+    S0c += __shfl_xor(S0c, off);
+    S1 += __shfl_xor(S1, off);
+    S1c += __shfl_xor(S1c, off);
+#pragma unroll
+    for (int n = 2; n < kN; ++n) C[n] += __shfl_xor(C[n], off);
+  }
+  const float s1t = S1 - S1c;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    // tau of point lane*P + j of the wavefront (exact in FP32)
+    const float t = ((float)(lane * P + j) - 0.5f * (float)(kWavePoints - 1)) *
+                    (2.0f / (float)kWavePoints);
+    float f = C[kN - 1];
+#pragma unroll
+    for (int n = kN - 2; n >= 2; --n) f = fmaf(f, t, C[n]);
+    f = fmaf(f, t, s1t) * t;
+    kahan_add(lay[j], layc[j], S0);
+    kahan_add(lay[j], layc[j], f - S0c);
+  }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
     if (i < t1) static_cast<float*>(a.far_sum)[(int64_t)k * a.n_nu + i] = lay[j] - layc[j];
   }
 }
-
 
 template <int P, bool kSurrogate>
 __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
@@ -789,9 +901,11 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
         const float ay = __uint_as_float(h1.z);
         bool queued = false;
-        // One tier per record for the whole wavefront where possible: the
-        // middle tier holds wherever the one-node form does, so a record for
-        // which some lane needs it runs it in every lane (one branch, not two).
+        // One tier per record for the whole wavefront: the middle tier holds
+        // wherever the one-node form does, so a record for which some lane
+        // needs it runs it in every lane.  Lanes that need the queue instead
+        // run it too, at a safe x (the run's middle at |x| = hw2) with weight
+        // 0, so that no branch splits the wavefront.
         if (__ballot(fabsf(xm) < __uint_as_float(h0.w)) == 0) {
           // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
 #pragma unroll
@@ -800,31 +914,37 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             const float D = fmaf(x, x, c1);
             part[j] = fmaf(ay, __builtin_amdgcn_rcpf(D), part[j]);
           }
-        } else if (fabsf(xm) >= __uint_as_float(h1.w)) {
-          if (c1 < kF32S2Y2 - 1.5f) {   // per record: uniform across the wavefront
+        } else {
+          const float hw2 = __uint_as_float(h1.w);
+          const bool mid = fabsf(xm) >= hw2;
+          queued = !mid && live;
+          const float xe = mid ? xm : hw2, aye = mid ? ay : 0.0f;
+          // c1 is the record's (the same in every lane): compare its bits
+          // (as signed integers, which order floats of either sign against a
+          // positive bound) in a scalar register
+          if (__builtin_amdgcn_readfirstlane(__float_as_int(c1)) <
+              __float_as_int(kF32S2Y2 - 1.5f)) {
             // second-order shifted form: K A = (A y/sqrt(pi)) r / (r^2 - 3/2 r + 2 y^2 - 3/2)
             const float y2 = c1 + 1.5f, cc = fmaf(2.0f, c1, 1.5f);
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
+              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xe);
               const float r = fmaf(x, x, y2);
               const float D = fmaf(r, r - 1.5f, cc);
-              part[j] = fmaf(ay * r, __builtin_amdgcn_rcpf(D), part[j]);
+              part[j] = fmaf(aye * r, __builtin_amdgcn_rcpf(D), part[j]);
             }
           } else {
             // two-node form: K A = (A y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
             const float cw = c1 + 1.0f, d2 = fmaf(2.0f, c1, 3.0f);
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
+              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xe);
               const float w = fmaf(x, x, cw);
               const float D = fmaf(w, w, d2);
-              const float N = fmaf(ay, w, ay);
+              const float N = fmaf(aye, w, aye);
               part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
             }
           }
-        } else {
-          queued = live;
         }
         // append the queued lanes' (q, lane) to this wavefront's queue; the
         // count nq advances in every lane (outside the divergent tiers)
