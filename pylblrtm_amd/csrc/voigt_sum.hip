@@ -361,24 +361,33 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
   return ac * mlp::kref(x, y) <= gate;
 }
 
-// FP64 far-field series (FP64 mode; see kSeriesRatio): as k_far_series_mixed
-// with the second-order shifted form, whose FP64 region (error <= 1e-11)
-// starts much nearer than the one-node form's:
+// FP64 far-field series (FP64 mode): as k_far_series_mixed, two levels, with
+// the second-order shifted form, whose FP64 region (error <= 1e-11) starts
+// much nearer than the one-node form's:
 //   A K = ay r / D, r = (xm + sh tau)^2 + y^2 = r0 + a1 tau + a2 tau^2,
 //   D = r^2 - 3/2 r + 2 y^2 - 3/2 = d0 + d1 tau + ... + d4 tau^4,
 // whose Taylor coefficients follow from D Q = ay r:
-//   Q_n = (ay r_n + e1 Q_(n-1) + e2 Q_(n-2) + e3 Q_(n-3) + e4 Q_(n-4)) / ..., e_k = -d_k / d0.
-// The poles lie at |tau| >= 1/0.345 (kSeriesRatio H), so kSeriesTerms64
-// terms leave <= 1e-11 of each record's value.  A record is summed this way
-// iff the form holds at all of the wavefront's points: with x_n = s (|nu_w - c| - H),
-// (x_n^2 + y^2)^3 >= kFp64S2Coef (9 y^2 + 21/4) (the form's threshold, cubed);
-// others are evaluated point by point with the full FP64 dispatch.
+//   Q_n = (ay r_n + e1 Q_(n-1) + ... + e4 Q_(n-4)) / d0 ... with e_k = -d_k / d0.
+// The poles lie at |tau| >= 1/0.345 (kSeriesRatio half-spans), so
+// kSeriesTerms64 terms leave <= 1e-11 of each record's value.  A record is
+// summed about a middle point of half-span Hu iff the form holds at all the
+// points covered: with x_n = s (|nu_m - c| - Hu),
+// (x_n^2 + y^2)^3 >= kFp64S2Coef (9 y^2 + 21/4) (the form's threshold,
+// cubed); a record that fails at the slice level goes on to the wavefront
+// level, and one that fails there is evaluated point by point with the full
+// FP64 dispatch.
 template <int P>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_far_series_fp64(const SumArgs a) {
-  static_assert(P == kPointsFp64 && 64 * P == kWavePoints, "wavefront geometry");
+  static_assert(P == kPointsFp64 && 64 * P == kWavePoints && kBlock * P == kSlicePoints,
+                "slice and wavefront geometry");
   constexpr int kWaves = kBlock / 64;
+  constexpr int kSliceExCap = 1024;
+  constexpr int kN = kSeriesTerms64;
   __shared__ int32_t s_ex[kWaves][kExCap];
+  __shared__ int32_t s_sx[kSliceExCap];
+  __shared__ int s_sxn;
+  __shared__ double s_coef[kWaves][kN];
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
   const int k = (int)(lw / a.n_wg);
   const int64_t wg = lw - (int64_t)k * a.n_wg;
@@ -386,45 +395,203 @@ void k_far_series_fp64(const SumArgs a) {
   const int slice = (int)(wg - tile * a.slices_per_tile);
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
-  const int64_t s0 = t0 + (int64_t)slice * (kBlock * P);
+  const int64_t s0 = t0 + (int64_t)slice * kSlicePoints;
+  if (s0 >= t1) return;   // the whole workgroup: no points (before the barriers)
   const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
   const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
-  if (w_first >= t1) return;   // the whole wavefront: no points
+  const bool wave_live = w_first < t1;
   const int64_t i0 = w_first + (int64_t)lane * P;
+  const double nu_s = slice_middle(a.nu_min, a.step, s0);
   const double nu_w = wave_middle(a.nu_min, a.step, w_first);
-  const double H = 0.5 * kWavePoints * a.step;
+  const double H = 0.5 * kWavePoints * a.step, Hs = 0.5 * kSlicePoints * a.step;
   const double lo_b = __dsub_rn(grid_nu(a.nu_min, a.step, t0), a.cutoff);
   const double hi_b = __dadd_rn(grid_nu(a.nu_min, a.step, t1 - 1), a.cutoff);
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
-  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
-  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  const int32_t slo = min(max(a.slo[wg], rlo), rhi);
+  const int32_t shi = max(min(a.shi[wg], rhi), slo);
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], slo), shi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wave], shi), nlo);
   const int64_t kb = (int64_t)k * a.n_lines;
   const double* __restrict__ rc = a.rec.c + kb;
   const double* __restrict__ rs = static_cast<const double*>(a.rec.s) + kb;
   const double* __restrict__ ry = static_cast<const double*>(a.rec.y) + kb;
   const double* __restrict__ ra = static_cast<const double*>(a.rec.a) + kb;
-  double C[kSeriesTerms64];
-#pragma unroll
-  for (int n = 0; n < kSeriesTerms64; ++n) C[n] = 0.0;
-  const int32_t n_below = nlo - rlo;
-  const int32_t n_far = n_below + (rhi - nhi);
-  auto far_index = [&](int32_t v) -> int64_t {
-    return v < n_below ? (int64_t)rlo + v : (int64_t)nhi + (v - n_below);
+  // the form holds at every point within Hu of the middle point d away
+  auto holds = [&](double d, double sv, double y2, double Hu) {
+    const double xn = sv * (fabs(d) - Hu);
+    const double rn = fma(xn, xn, y2);
+    return rn * rn * rn >= kFp64S2Coef * fma(9.0, y2, 5.25);
   };
-  double lay[P];
+  // adds the Taylor coefficients of (n0 + n1 tau + n2 tau^2) / (d0 + ... + d4 tau^4)
+  // to C: D Q = N term by term
+  auto rational = [&](double (&C)[kN], double n0, double n1, double n2, double d0, double d1,
+                      double d2, double d3, double d4) {
+    const double inv = rcp64_1nr(d0);
+    const double e1 = -d1 * inv, e2 = -d2 * inv, e3 = -d3 * inv, e4 = -d4 * inv;
+    double q0 = n0 * inv;
+    double q1 = fma(e1, q0, n1 * inv);
+    double q2 = fma(e1, q1, fma(e2, q0, n2 * inv));
+    double q3 = fma(e1, q2, fma(e2, q1, e3 * q0));
+    C[0] += q0;
+    C[1] += q1;
+    C[2] += q2;
+    C[3] += q3;
 #pragma unroll
-  for (int j = 0; j < P; ++j) lay[j] = 0.0;
+    for (int n = 4; n < kN; ++n) {
+      const double qn = fma(e1, q3, fma(e2, q2, fma(e3, q1, e4 * q0)));
+      C[n] += qn;
+      q0 = q1; q1 = q2; q2 = q3; q3 = qn;
+    }
+  };
+  // second-order shifted form about a middle point d = nu_m - c away:
+  // ay r / (r^2 - 3/2 r + 2 y^2 - 3/2), r = X + y^2, X = x^2 = X0 + a1 tau + a2 tau^2
+  auto series = [&](double (&C)[kN], double d, double sv, double y, double am, double Hu) {
+    const double y2 = y * y;
+    const double xm = d * sv, sh = sv * Hu;
+    const double a1 = 2.0 * xm * sh, a2 = sh * sh;
+    const double r0 = fma(xm, xm, y2);
+    const double t = fma(2.0, r0, -1.5);
+    const double g = am * y * kInvSqrtPi;
+    rational(C, g * r0, g * a1, g * a2, fma(r0, r0 - 1.5, fma(2.0, y2, -1.5)), t * a1,
+             fma(a1, a1, t * a2), 2.0 * a1 * a2, a2 * a2);
+  };
+  // four-node form (|z| >= 35, faddeeva.h) as two pairs of poles, each
+  // w_k (X + b) / (X^2 + (2 y^2 - 2 t_k^2) X + b^2), b = y^2 + t_k^2
+  auto series_gh4 = [&](double (&C)[kN], double d, double sv, double y, double am, double Hu) {
+    const double y2 = y * y;
+    const double xm = d * sv, sh = sv * Hu;
+    const double a1 = 2.0 * xm * sh, a2 = sh * sh;
+    const double X0 = xm * xm;
+    const double ay2 = am * y * kInvSqrtPi * (2.0 * kInvSqrtPi);
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const double t2 = kGh4T[p] * kGh4T[p], w = ay2 * kGh4W[p];
+      const double b = y2 + t2, c = 2.0 * (y2 - t2);
+      const double t = fma(2.0, X0, c);
+      rational(C, w * (X0 + b), w * a1, w * a2, fma(X0, X0 + c, b * b), t * a1,
+               fma(a1, a1, t * a2), 2.0 * a1 * a2, a2 * a2);
+    }
+  };
+  // the four-node form holds at every point within Hu of the middle point
+  auto holds_gh4 = [&](double d, double sv, double y2, double Hu) {
+    const double xn = sv * (fabs(d) - Hu);
+    return fma(xn, xn, y2) >= kFp64Gh4R2;
+  };
+  if (threadIdx.x == 0) s_sxn = 0;
+  __syncthreads();
+
+  // ---- slice level: records outside [slo, shi), one per thread
+  {
+    double C[kN];
+#pragma unroll
+    for (int n = 0; n < kN; ++n) C[n] = 0.0;
+    const int32_t n_below = slo - rlo;
+    const int32_t n_sl = n_below + (rhi - shi);
+    auto sl_index = [&](int32_t v) -> int64_t {
+      return v < n_below ? (int64_t)rlo + v : (int64_t)shi + (v - n_below);
+    };
+    double nc = __builtin_nan(""), ns = 0.0, ny = 1.0, na = 0.0;
+    if ((int)threadIdx.x < n_sl) {
+      const int64_t i = sl_index(threadIdx.x);
+      nc = rc[i]; ns = rs[i]; ny = ry[i]; na = ra[i];
+    }
+    const int32_t n_it = (n_sl + kBlock - 1) / kBlock;
+    for (int32_t it = 0; it < n_it; ++it) {
+      const double c = nc, sv = ns, y = ny, am = na;
+      const int32_t v = it * kBlock + (int)threadIdx.x;
+      if (v + kBlock < n_sl) {
+        const int64_t i = sl_index(v + kBlock);
+        nc = rc[i]; ns = rs[i]; ny = ry[i]; na = ra[i];
+      }
+      const double d = nu_s - c;
+      const bool inw = (v < n_sl) & (c >= lo_b) & (c <= hi_b);
+      const bool ser = inw && holds(d, sv, y * y, Hs);
+      if (ser) series(C, d, sv, y, am, Hs);
+      if (inw & !ser) {   // on to the wavefront level
+        const int pos = atomicAdd(&s_sxn, 1);
+        if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int n = 0; n < kN; ++n) C[n] += __shfl_xor(C[n], off);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int n = 0; n < kN; ++n) s_coef[wave][n] = C[n];
+    }
+  }
+  __syncthreads();
+  if (!wave_live) return;   // the last barrier is behind every wavefront
+
+  // the slice series goes to far_sum at once (no registers held through the
+  // wavefront loop); the wavefront level adds to it
+  double* const out = static_cast<double*>(a.far_sum) + (int64_t)k * a.n_nu;
+  {
+    double T[kN];
+#pragma unroll
+    for (int n = 0; n < kN; ++n) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) t += s_coef[w][n];
+      T[n] = t;
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const double t = ((double)(wave * kWavePoints + lane * P + j) - 0.5 * (double)(kSlicePoints - 1)) *
+                       (2.0 / (double)kSlicePoints);
+      double f = T[kN - 1];
+#pragma unroll
+      for (int n = kN - 2; n >= 0; --n) f = fma(f, t, T[n]);
+      if (i0 + j < t1) out[i0 + j] = f;
+    }
+  }
+
+  // ---- wavefront level
+  double C[kN];
+#pragma unroll
+  for (int n = 0; n < kN; ++n) C[n] = 0.0;
+  const int n_sx = __builtin_amdgcn_readfirstlane(s_sxn);
+  const bool rescan = n_sx > kSliceExCap;
+  const int32_t nA = nlo - slo, nB = shi - nhi, nC = min(n_sx, kSliceExCap);
+  const int32_t nD = rescan ? (slo - rlo) + (rhi - shi) : 0;
+  const int32_t n_far = nA + nB + nC + nD;
+  auto far_index = [&](int32_t v) -> int64_t {
+    if (v < nA) return (int64_t)slo + v;
+    v -= nA;
+    if (v < nB) return (int64_t)nhi + v;
+    v -= nB;
+    if (v < nC) return (int64_t)s_sx[v];
+    v -= nC;
+    return v < slo - rlo ? (int64_t)rlo + v : (int64_t)shi + (v - (slo - rlo));
+  };
   // records of s_ex[wave][0, n) at every lane's own points, full dispatch
   auto far_exceptions = [&](const int n) {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the lanes' list writes
     for (int e = 0; e < n; ++e) {
       const int32_t idx = __builtin_amdgcn_readfirstlane(s_ex[wave][e]);
       const double c = rc[idx], sv = rs[idx], y = ry[idx], am = ra[idx];
+      // four-node form where |z| >= 35 (faddeeva.h), else the full dispatch
+      const double y2 = y * y, fy2 = 4.0 * y2;
+      const double w0 = am * y * kInvSqrtPi * (2.0 * kInvSqrtPi * kGh4W[0]);
+      const double w1 = am * y * kInvSqrtPi * (2.0 * kInvSqrtPi * kGh4W[1]);
+      const double t0sq = kGh4T[0] * kGh4T[0], t1sq = kGh4T[1] * kGh4T[1];
 #pragma unroll 1
       for (int j = 0; j < P; ++j) {
-        const double v = am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - c) * sv, y);
-#pragma unroll
-        for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
+        const double x = (grid_nu(a.nu_min, a.step, i0 + j) - c) * sv;
+        const double x2 = x * x;
+        double v;
+        if (x2 + y2 >= kFp64Gh4R2) {
+          const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+          const double u0 = xmy - t0sq, u1 = xmy - t1sq;
+          const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
+          const double N = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
+          v = N * rcp64_1nr(D0 * D1);
+        } else {
+          v = am * K_fp64_call(x, y);
+        }
+        if (i0 + j < t1) out[i0 + j] += v;   // this lane's own point
       }
     }
   };
@@ -437,41 +604,20 @@ void k_far_series_fp64(const SumArgs a) {
   const int32_t n_it = (n_far + 63) >> 6;
   for (int32_t it = 0; it < n_it; ++it) {
     const double c = nc, sv = ns, y = ny, am = na;
-    const int32_t vn = (it + 1) * 64 + lane;
-    if (vn < n_far) {
-      const int64_t i = far_index(vn);
+    const int32_t v = it * 64 + lane;
+    if (v + 64 < n_far) {
+      const int64_t i = far_index(v + 64);
       nc = rc[i]; ns = rs[i]; ny = ry[i]; na = ra[i];
     }
-    const bool inw = (it * 64 + lane < n_far) & (c >= lo_b) & (c <= hi_b);
     const double d = nu_w - c;
     const double y2 = y * y;
-    const double xn = sv * (fabs(d) - H);
-    const double rn = fma(xn, xn, y2);
-    const bool ser = inw & (rn * rn * rn >= kFp64S2Coef * fma(9.0, y2, 5.25));
-    if (ser) {
-      const double xm = d * sv, sh = sv * H;
-      const double a1 = 2.0 * xm * sh, a2 = sh * sh;
-      const double r0 = fma(xm, xm, y2);
-      const double inv = rcp64_1nr(fma(r0, r0 - 1.5, fma(2.0, y2, -1.5)));
-      const double t = fma(2.0, r0, -1.5);
-      const double e1 = -(t * a1) * inv, e2 = -fma(a1, a1, t * a2) * inv;
-      const double e3 = -(2.0 * a1 * a2) * inv, e4 = -(a2 * a2) * inv;
-      const double g = am * y * kInvSqrtPi * inv;
-      double q0 = g * r0;
-      double q1 = fma(e1, q0, g * a1);
-      double q2 = fma(e1, q1, fma(e2, q0, g * a2));
-      double q3 = fma(e1, q2, fma(e2, q1, e3 * q0));
-      C[0] += q0;
-      C[1] += q1;
-      C[2] += q2;
-      C[3] += q3;
-#pragma unroll
-      for (int n = 4; n < kSeriesTerms64; ++n) {
-        const double qn = fma(e1, q3, fma(e2, q2, fma(e3, q1, e4 * q0)));
-        C[n] += qn;
-        q0 = q1; q1 = q2; q2 = q3; q3 = qn;
-      }
-    }
+    bool inw = (v < n_far) & (c >= lo_b) & (c <= hi_b);
+    // the rescan skips what the slice level summed
+    if (v >= nA + nB + nC) inw = inw && !holds(nu_s - c, sv, y2, Hs);
+    // the four-node form here: it holds much nearer than the second-order
+    // form at FP64 accuracy, so broad lines need no point-by-point pass
+    const bool ser = inw && holds_gh4(d, sv, y2, H);
+    if (ser) series_gh4(C, d, sv, y, am, H);
     const bool exc = inw & !ser;
     const uint64_t ex = __ballot(exc);
     if (ex) {   // uniform
@@ -481,27 +627,25 @@ void k_far_series_fp64(const SumArgs a) {
       }
       const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
           (uint32_t)(ex >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ex, 0u));
-      if (exc) s_ex[wave][n_ex + (int)rank] = (int32_t)far_index(it * 64 + lane);
+      if (exc) s_ex[wave][n_ex + (int)rank] = (int32_t)far_index(v);
       n_ex += (int)__popcll(ex);
     }
   }
   far_exceptions(n_ex);
-  // totals over the wavefront's 64 lanes, in every lane
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
 #pragma unroll
-    for (int n = 0; n < kSeriesTerms64; ++n) C[n] += __shfl_xor(C[n], off);
+    for (int n = 0; n < kN; ++n) C[n] += __shfl_xor(C[n], off);
   }
-  double* const out = static_cast<double*>(a.far_sum) + (int64_t)k * a.n_nu;
 #pragma unroll
   for (int j = 0; j < P; ++j) {
     const double t = ((double)(lane * P + j) - 0.5 * (double)(kWavePoints - 1)) *
                      (2.0 / (double)kWavePoints);
-    double f = C[kSeriesTerms64 - 1];
+    double f = C[kN - 1];
 #pragma unroll
-    for (int n = kSeriesTerms64 - 2; n >= 0; --n) f = fma(f, t, C[n]);
+    for (int n = kN - 2; n >= 0; --n) f = fma(f, t, C[n]);
     const int64_t i = i0 + j;
-    if (i < t1) out[i] = lay[j] + f;
+    if (i < t1) out[i] += f;
   }
 }
 

@@ -303,27 +303,36 @@ class OpticalDepthProblem:
         return total
 
     def count_series_work(self) -> dict:
-        """Work units of the mixed line sum, split as the kernels do.  Each
-        wavefront owns WAVE_POINTS consecutive points of a slice of
-        SLICE_POINTS points of a tile; its near range is the lines whose nu0
-        lies within SERIES_RATIO * H (+ the shift bound) of its middle point,
-        H = WAVE_POINTS / 2 * step.  near_evals: (layer, near line in the
-        tile window, point) triples, evaluated point by point; far_pairs:
-        (layer, other line in the tile window, wavefront) pairs, each summed
-        as one series.  Groups with a zero column are left out (they add 0)."""
+        """Work units of the line sums, split as the kernels do.  A slice of
+        SLICE_POINTS points is four wavefronts of WAVE_POINTS; the near range
+        of a wavefront (a slice) is the lines whose nu0 lies within
+        SERIES_RATIO times its half-span (+ the shift bound) of its middle
+        point.  near_evals: (layer, line in the tile window and the
+        wavefront's near range, point) triples, evaluated point by point;
+        far_pairs: series summed, one per (layer, line, slice) outside the
+        slice's near range and one per (layer, line, wavefront) between the
+        two near ranges.  Groups with a zero column are left out (they add 0)."""
         wave_pts, slice_pts, ratio = 512, 2048, 3.0
         t0, t1, lo_b, hi_b = self.tile_edges()
         h_nu0, h_shift, h_group = self._host_lines()
-        # wavefronts: (tile, slice, wave) with a live first point
+        margin = abs(self.shift_bound) * (1 + 1e-9) + 1e-9
         spt = (self.tile_samples + slice_pts - 1) // slice_pts
-        offs = np.arange(spt * (slice_pts // wave_pts), dtype=np.int64) * wave_pts
-        first = (t0[:, None] + offs[None, :]).ravel()
-        tile_of = np.repeat(np.arange(len(t0)), len(offs))
-        live = first < t1[tile_of]
-        first, tile_of = first[live], tile_of[live]
-        pts = np.minimum(t1[tile_of] - first, wave_pts)
-        nu_w = self.nu[first] + 0.5 * (wave_pts - 1) * self.step
-        half = ratio * 0.5 * wave_pts * self.step + abs(self.shift_bound) * (1 + 1e-9) + 1e-9
+
+        def units(pts_per_unit):
+            offs = np.arange(spt * (slice_pts // pts_per_unit), dtype=np.int64) * pts_per_unit
+            first = (t0[:, None] + offs[None, :]).ravel()
+            tile_of = np.repeat(np.arange(len(t0)), len(offs))
+            live = first < t1[tile_of]
+            first, tile_of = first[live], tile_of[live]
+            mid = self.nu[first] + 0.5 * (pts_per_unit - 1) * self.step
+            half = ratio * 0.5 * pts_per_unit * self.step + margin
+            return first, tile_of, mid, half
+
+        w_first, w_tile, w_mid, w_half = units(wave_pts)
+        s_first, s_tile, s_mid, s_half = units(slice_pts)
+        w_pts = np.minimum(t1[w_tile] - w_first, wave_pts)
+        # slice of each wavefront
+        w_slice = np.searchsorted(s_first, w_first, side="right") - 1
         near_evals = far_pairs = 0
         lay = self.layers
         for k in range(self.n_layers):
@@ -342,11 +351,12 @@ class OpticalDepthProblem:
             if not nu0_on:
                 continue
             nu0 = np.sort(np.concatenate(nu0_on))
-            near = (np.searchsorted(nu0, nu_w + half, side="right")
-                    - np.searchsorted(nu0, nu_w - half, side="left"))
-            near = np.minimum(near, win[tile_of])
-            near_evals += int(np.sum(near * pts))
-            far_pairs += int(np.sum(win[tile_of] - near))
+            s_near = np.minimum(np.searchsorted(nu0, s_mid + s_half, side="right")
+                                - np.searchsorted(nu0, s_mid - s_half, side="left"), win[s_tile])
+            w_near = np.minimum(np.searchsorted(nu0, w_mid + w_half, side="right")
+                                - np.searchsorted(nu0, w_mid - w_half, side="left"), s_near[w_slice])
+            near_evals += int(np.sum(w_near * w_pts))
+            far_pairs += int(np.sum(win[s_tile] - s_near)) + int(np.sum(s_near[w_slice] - w_near))
         return {"near_evals": near_evals, "far_pairs": far_pairs}
 
     # ------------------------------------------------------------------ run --
