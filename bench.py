@@ -31,10 +31,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Voigt line-gridpoint evals/s at max|dT|<=1e-5 vs FP64; surrogate vs plain FP32"
 FLOP_PER_EVAL = 16            # per point-by-point evaluation, every mode charged the same
-# FP32 operations per (record, wavefront) pair of the mixed far-field series:
-# x, D0, rcp, the three recurrence factors and the first two terms (12), then
-# 16 terms of mul + fma + add (64)
-FLOP_PER_SERIES_PAIR = 76
+# Operations per (record, slice) or (record, wavefront) pair of the far-field
+# series, counted from the kernels' source (an fma counts 2):
+#   mixed, both levels (one-node form, 18 FP32 terms): x, D0, rcp, the
+#     recurrence factors and the first two terms (12), then 16 terms of
+#     mul + fma + add (64) = 76;
+#   FP64 slice level (second-order shifted form, 28 terms): the numerator and
+#     denominator polynomials (27), rcp + Newton step, four ratios and the
+#     first four terms (27), then 24 terms of a four-term recurrence + add (192)
+#     = 246;
+#   FP64 wavefront level (four-node form as two pole pairs, one 28-term
+#     recurrence each): 10 shared + 2 x (22 + 219) = 492.
+FLOP_PER_PAIR = {"mixed": {"slice": 76, "wave": 76}, "fp64": {"slice": 246, "wave": 492}}
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -75,6 +83,37 @@ def time_mode(prob, precision, mus, steps, warmup, capi, torch):
     capi.enable_timing(False)
     time_mode.split_ms = tuple(float(v) for v in np.mean(np.asarray(split), axis=0))
     return outs, wall / steps, float(np.mean(k_ms)), float(np.mean(lp_ms))
+
+
+def kernel_rooflines(precision, split_ms, work, traffic):
+    """Per-kernel achieved rate against the vector peak of the kernel's
+    arithmetic: the far-field series kernel (its units: series pairs, slice
+    and wavefront levels priced separately) and the near-field line sum (its
+    units: point-by-point evaluations, FLOP_PER_EVAL each).  `traffic` maps a
+    kernel name to its calibrated HBM bytes per launch."""
+    far_ms, near_ms = split_ms
+    base = "fp64" if precision == "fp64" else "mixed"
+    fp = FLOP_PER_PAIR[base]
+    peak = FP64_PEAK_TFLOPS if base == "fp64" else FP32_PEAK_TFLOPS
+    near_name = {"fp64": "k_voigt_sum_fp64", "mixed": "k_voigt_sum_mixed",
+                 "surrogate": "k_voigt_sum_surrogate"}[precision]
+    far_flop = work["slice_pairs"] * fp["slice"] + work["wave_pairs"] * fp["wave"]
+    kern = {
+        f"k_far_series_{base}": {
+            "ms": far_ms, "units": work["far_pairs"], "unit": "(record, slice or wavefront) series pairs",
+            "slice_pairs": work["slice_pairs"], "wave_pairs": work["wave_pairs"],
+            "flop_per_unit": far_flop / max(work["far_pairs"], 1),
+            "flop_per_pair": dict(fp)},
+        near_name: {"ms": near_ms, "units": work["near_evals"], "unit": "point evaluations",
+                    "flop_per_unit": FLOP_PER_EVAL},
+    }
+    for name, d in kern.items():
+        d["achieved_tflops"] = d["units"] * d["flop_per_unit"] / (max(d["ms"], 1e-9) * 1e-3) / 1e12
+        d["peak"] = peak
+        d["frac"] = d["achieved_tflops"] / peak
+        t = traffic.get(name)
+        d["traffic"] = t if isinstance(t, (int, float)) else None
+    return kern
 
 
 def cpu_baseline(sd, lay, prob, tile, layer_lo, layer_hi, gpu_layers):
@@ -434,42 +473,22 @@ def run(args, pool) -> int:
         "line_sum_kernel_ms": k_ms,
         "line_params_kernel_ms": lp_ms,
     }
-    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     traffic = {}
     tf_path = os.path.join(ROOT, "profiles", "traffic_voigt_sum_five_gas.json")
     if os.path.exists(tf_path):
         with open(tf_path) as f:
             traffic = json.load(f)
-    if args.precision == "fp64":
-        achieved = E * FLOP_PER_EVAL / (k_ms * 1e-3) / 1e12
-        result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                              "frac": achieved / peak, "traffic": traffic.get("fp64"),
-                              "kernel": "k_voigt_sum_fp64", "flop_per_eval": FLOP_PER_EVAL}
-    else:
-        # two kernels: the far-field series (units: (record, wavefront) pairs)
-        # and the near-field line sum (units: point-by-point evaluations)
-        far_ms, near_ms = time_mode.split_ms
-        work = prob.count_series_work()
-        kern = {
-            "k_far_series_mixed": {"ms": far_ms, "units": work["far_pairs"],
-                                   "unit": "(record, wavefront) series pairs",
-                                   "flop_per_unit": FLOP_PER_SERIES_PAIR},
-            "k_voigt_sum_" + ("surrogate" if args.precision == "surrogate" else "mixed"):
-                {"ms": near_ms, "units": work["near_evals"], "unit": "point evaluations",
-                 "flop_per_unit": FLOP_PER_EVAL},
-        }
-        for name, d in kern.items():
-            d["achieved_tflops"] = d["units"] * d["flop_per_unit"] / (max(d["ms"], 1e-9) * 1e-3) / 1e12
-            d["frac"] = d["achieved_tflops"] / peak
-            d["traffic"] = traffic.get(name)
-        dom = max(kern, key=lambda n: kern[n]["ms"])
-        d = kern[dom]
-        result["roofline"] = {"bound": "valu", "achieved": d["achieved_tflops"], "peak": peak,
-                              "unit": "TFLOP/s", "frac": d["frac"], "traffic": d["traffic"],
-                              "kernel": dom, "units_per_launch": d["units"],
-                              "flop_per_unit": d["flop_per_unit"], "kernels": kern}
-        log(f"[bench] kernels: " + ", ".join(f"{n} {v['ms']:.2f} ms ({v['units']:.3e} units, "
-                                             f"frac {v['frac']:.3f})" for n, v in kern.items()))
+    work = prob.count_series_work()
+    result["series_work"] = work
+    kern = kernel_rooflines(args.precision, time_mode.split_ms, work, traffic)
+    dom = max(kern, key=lambda n: kern[n]["ms"])
+    d = kern[dom]
+    result["roofline"] = {"bound": "valu", "achieved": d["achieved_tflops"], "peak": d["peak"],
+                          "unit": "TFLOP/s", "frac": d["frac"], "traffic": d["traffic"],
+                          "kernel": dom, "units_per_launch": d["units"],
+                          "flop_per_unit": d["flop_per_unit"], "kernels": kern}
+    log("[bench] kernels: " + ", ".join(f"{n} {v['ms']:.2f} ms ({v['units']:.3e} units, "
+                                        f"frac {v['frac']:.3f})" for n, v in kern.items()))
 
     if args.precision == "mixed" and not args.no_surrogate:
         result["surrogate"] = surrogate_report(prob, E, value, mus, max(2, args.steps // 2), _capi,
@@ -482,8 +501,7 @@ def run(args, pool) -> int:
             log(f"[bench] fp64: {sps64 * 1e3:.1f} ms/step, kernel {k64:.1f} ms, {E / sps64:.3e} evals/s")
             result["fp64"] = {"evals_per_s": E / sps64, "ms_per_step": sps64 * 1e3,
                               "line_sum_kernel_ms": k64,
-                              "roofline_frac_fp64_peak": E * FLOP_PER_EVAL / (k64 * 1e-3) / 1e12
-                              / FP64_PEAK_TFLOPS}
+                              "kernels": kernel_rooflines("fp64", time_mode.split_ms, work, traffic)}
             dT = {}
             for m, mu in enumerate(mus):
                 d = (out["transmittance"][m].double() - o64["transmittance"][m].double()).abs().max()

@@ -309,9 +309,10 @@ class OpticalDepthProblem:
         SERIES_RATIO times its half-span (+ the shift bound) of its middle
         point.  near_evals: (layer, line in the tile window and the
         wavefront's near range, point) triples, evaluated point by point;
-        far_pairs: series summed, one per (layer, line, slice) outside the
-        slice's near range and one per (layer, line, wavefront) between the
-        two near ranges.  Groups with a zero column are left out (they add 0)."""
+        slice_pairs: series summed once per (layer, line, slice) outside the
+        slice's near range; wave_pairs: once per (layer, line, wavefront)
+        between the two near ranges; far_pairs: their total.  Groups with a
+        zero column are left out (they add 0)."""
         wave_pts, slice_pts, ratio = 512, 2048, 3.0
         t0, t1, lo_b, hi_b = self.tile_edges()
         h_nu0, h_shift, h_group = self._host_lines()
@@ -333,7 +334,7 @@ class OpticalDepthProblem:
         w_pts = np.minimum(t1[w_tile] - w_first, wave_pts)
         # slice of each wavefront
         w_slice = np.searchsorted(s_first, w_first, side="right") - 1
-        near_evals = far_pairs = 0
+        near_evals = slice_pairs = wave_pairs = 0
         lay = self.layers
         for k in range(self.n_layers):
             P = float(lay.pressure_atm[k])
@@ -356,8 +357,10 @@ class OpticalDepthProblem:
             w_near = np.minimum(np.searchsorted(nu0, w_mid + w_half, side="right")
                                 - np.searchsorted(nu0, w_mid - w_half, side="left"), s_near[w_slice])
             near_evals += int(np.sum(w_near * w_pts))
-            far_pairs += int(np.sum(win[s_tile] - s_near)) + int(np.sum(s_near[w_slice] - w_near))
-        return {"near_evals": near_evals, "far_pairs": far_pairs}
+            slice_pairs += int(np.sum(win[s_tile] - s_near))
+            wave_pairs += int(np.sum(s_near[w_slice] - w_near))
+        return {"near_evals": near_evals, "slice_pairs": slice_pairs, "wave_pairs": wave_pairs,
+                "far_pairs": slice_pairs + wave_pairs}
 
     # ------------------------------------------------------------------ run --
     def workspace(self, precision: int):
