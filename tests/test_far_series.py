@@ -1,0 +1,186 @@
+"""Truncation of the far-field Taylor series of the line sums
+(csrc/voigt_sum.hip k_far_series_mixed / k_far_series_fp64), checked on the
+CPU against the closed forms they expand.
+
+A record whose centre c lies far from the middle point nu_m of a run of grid
+points (half-span Hu) contributes, at nu = nu_m + Hu tau (|tau| < 1), a far
+form f(x), x = s (nu - c) = xm + sh tau, with xm = s (nu_m - c), sh = s Hu.
+The kernels sum f as a Taylor series in tau whose coefficients come from the
+form's recurrences; the error claims are
+  * mixed, shifted one-node form ay / (x^2 + c1), 18 terms: <= 2.6e-7 of f;
+  * FP64, second-order form ay r / (r^2 - 3/2 r + 2 y^2 - 3/2), r = x^2 + y^2,
+    28 terms: <= 1e-11 of f;
+  * FP64, four-node form as two pole pairs w (X + b) / (X^2 + cX + b^2),
+    X = x^2, 28 terms each: <= 1e-11 of the record's value,
+wherever the kernels use them: outside the near range (|nu_m - c| >= 3 Hu)
+and where the form holds at every covered point.  The recurrences below are
+written from the power-series identity D Q = N (D, N polynomials in tau); the
+references are the forms evaluated directly in float64, so a wrong
+coefficient, sign or term count in a recurrence fails the bound.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+RATIO = 3.0              # kSeriesRatio
+TERMS32 = 18             # kSeriesTerms
+TERMS64 = 28             # kSeriesTerms64
+S1_COEF = 164062.5       # kF32S1Coef
+S2_COEF64 = 1.1e11       # kFp64S2Coef
+GH4_R2 = 1225.0          # kFp64Gh4R2
+GH4_T = (0.5246476232752904, 1.6506801238857847)
+GH4_W = (0.8049140900055127, 0.08131283544724519)
+INV_SQRT_PI = 1.0 / math.sqrt(math.pi)
+TAU = np.linspace(-1.0, 1.0, 257)
+
+
+def series_rational(num, den, terms):
+    """Taylor coefficients of N(tau) / D(tau) from D Q = N, term by term."""
+    q = []
+    for n in range(terms):
+        v = num[n] if n < len(num) else 0.0
+        for k in range(1, min(n, len(den) - 1) + 1):
+            v -= den[k] * q[n - k]
+        q.append(v / den[0])
+    return np.array(q)
+
+
+def series_one_node(xm, sh, c1, ay, terms=TERMS32):
+    """The mixed kernel's recurrence: Q_0 = ay/D0, Q_1 = al Q_0,
+    Q_n = al Q_(n-1) + be Q_(n-2), al = -2 xm sh / D0, be = -sh^2 / D0."""
+    d0 = xm * xm + c1
+    al, be = -2.0 * xm * sh / d0, -sh * sh / d0
+    q = [ay / d0]
+    q.append(al * q[0])
+    for _ in range(2, terms):
+        q.append(al * q[-1] + be * q[-2])
+    return np.array(q)
+
+
+def poly(q, t):
+    return np.polynomial.polynomial.polyval(t, q)
+
+
+def s1_r2_min(y2):
+    return max(625.0, math.sqrt(S1_COEF * (2.0 * y2 + 1.5)))
+
+
+def far_points(sh, x_form):
+    """|xm| values at the edge of where a series is used: outside the near
+    range (|xm| >= 3 sh) and with the form holding at every covered point
+    (|xm| - sh >= x_form), plus a few farther ones."""
+    edge = max(RATIO * sh, sh + x_form)
+    return [edge, 1.2 * edge, 2.0 * edge, 10.0 * edge]
+
+
+@pytest.mark.parametrize("y", [1e-4, 0.05, 0.5, 1.0, 1.2247, 1.3, 3.0, 10.0, 40.0, 150.0])
+def test_one_node_series_18_terms(y):
+    y2 = y * y
+    c1 = y2 - 1.5
+    x_s1 = math.sqrt(max(s1_r2_min(y2) - y2, 0.0))
+    worst = 0.0
+    for sh in np.geomspace(0.01, 3000.0, 41):
+        for xm0 in far_points(sh, x_s1):
+            for xm in (xm0, -xm0):
+                q = series_one_node(xm, sh, c1, 1.0)
+                x = xm + sh * TAU
+                f = 1.0 / (x * x + c1)
+                worst = max(worst, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+    assert worst <= 2.6e-7, worst
+
+
+def test_one_node_recurrence_is_the_rational_series():
+    # the three-term recurrence against the general D Q = N expansion
+    for xm, sh, c1 in [(40.0, 12.5, -1.4), (700.0, 200.0, 0.3), (-90.0, 3.0, 50.0)]:
+        d = [xm * xm + c1, 2.0 * xm * sh, sh * sh]
+        a = series_one_node(xm, sh, c1, 2.5)
+        b = series_rational([2.5], d, TERMS32)
+        assert np.allclose(a, b, rtol=1e-12, atol=0.0)
+
+
+def test_one_node_series_fails_inside_the_near_range():
+    # a record nearer than 3 Hu (with sh at the 0.345 corner) is left out for a
+    # reason: 18 terms there miss the bound by orders of magnitude
+    sh, c1 = 12.5, 1e-8 - 1.5
+    xm = 1.6 * sh
+    q = series_one_node(xm, sh, c1, 1.0)
+    x = xm + sh * TAU
+    assert np.max(np.abs(poly(q, TAU) * (x * x + c1) - 1.0)) > 1e-3
+
+
+def second_order_coeffs(xm, sh, y, ay):
+    """The FP64 kernel's numerator / denominator polynomials in tau."""
+    y2 = y * y
+    a1, a2 = 2.0 * xm * sh, sh * sh
+    r0 = xm * xm + y2
+    t = 2.0 * r0 - 1.5
+    g = ay
+    num = [g * r0, g * a1, g * a2]
+    den = [r0 * (r0 - 1.5) + 2.0 * y2 - 1.5, t * a1, a1 * a1 + t * a2, 2.0 * a1 * a2, a2 * a2]
+    return num, den
+
+
+@pytest.mark.parametrize("y", [1e-4, 0.1, 1.0, 3.0, 10.0, 50.0, 200.0, 1e3, 1e4])
+def test_second_order_series_28_terms(y):
+    y2 = y * y
+    r_min = (S2_COEF64 * (9.0 * y2 + 5.25)) ** (1.0 / 3.0)
+    x_s2 = math.sqrt(max(r_min - y2, 0.0))
+    worst = 0.0
+    for sh in np.geomspace(0.01, 3000.0, 41):
+        for xm0 in far_points(sh, x_s2):
+            for xm in (xm0, -xm0):
+                num, den = second_order_coeffs(xm, sh, y, 1.0)
+                q = series_rational(num, den, TERMS64)
+                x = xm + sh * TAU
+                r = x * x + y2
+                f = r / (r * r - 1.5 * r + 2.0 * y2 - 1.5)
+                worst = max(worst, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+    assert worst <= 1e-11, worst
+
+
+def gh4_pair_coeffs(xm, sh, y, p):
+    """One pole pair of the four-node form, w (X + b) / (X^2 + cX + b^2),
+    b = y^2 + t^2, c = 2 (y^2 - t^2), X = x^2, as in the FP64 kernel."""
+    y2 = y * y
+    a1, a2 = 2.0 * xm * sh, sh * sh
+    X0 = xm * xm
+    t2 = GH4_T[p] ** 2
+    w = 2.0 * INV_SQRT_PI * INV_SQRT_PI * y * GH4_W[p]
+    b, c = y2 + t2, 2.0 * (y2 - t2)
+    t = 2.0 * X0 + c
+    num = [w * (X0 + b), w * a1, w * a2]
+    den = [X0 * (X0 + c) + b * b, t * a1, a1 * a1 + t * a2, 2.0 * a1 * a2, a2 * a2]
+    return num, den
+
+
+def gh4_direct(x, y):
+    # the four-node Gauss-Hermite form (faddeeva.h K_gh4_64), float64
+    x2, y2 = x * x, y * y
+    k = 0.0
+    for t, w in zip(GH4_T, GH4_W):
+        t2 = t * t
+        u = x2 - y2 - t2
+        k = k + w * (x2 + y2 + t2) / (u * u + 4.0 * x2 * y2)
+    return 2.0 * INV_SQRT_PI * INV_SQRT_PI * y * k
+
+
+@pytest.mark.parametrize("y", [1e-4, 0.1, 1.0, 5.0, 20.0, 34.0, 35.0, 60.0, 400.0])
+def test_four_node_pole_pair_series_28_terms(y):
+    x_4 = math.sqrt(max(GH4_R2 - y * y, 0.0))
+    worst = 0.0
+    for sh in np.geomspace(0.01, 3000.0, 41):
+        for xm0 in far_points(sh, x_4):
+            for xm in (xm0, -xm0):
+                q = sum(series_rational(*gh4_pair_coeffs(xm, sh, y, p), TERMS64) for p in (0, 1))
+                f = gh4_direct(xm + sh * TAU, y)
+                worst = max(worst, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+    assert worst <= 1e-11, worst
+
+
+def test_four_node_pole_pairs_reproduce_the_form():
+    # tau = 0 term of the two pole pairs = the four-node form at xm
+    for xm, y in [(40.0, 0.01), (36.0, 3.0), (200.0, 30.0)]:
+        q0 = sum(series_rational(*gh4_pair_coeffs(xm, 1.0, y, p), 1)[0] for p in (0, 1))
+        assert q0 == pytest.approx(gh4_direct(np.array([xm]), y)[0], rel=1e-14)
