@@ -304,6 +304,32 @@ def surrogate_report(prob, E, value_plain, mus, steps, capi, torch):
     return rep
 
 
+def drop_in_bench(args, torch):
+    """Wall time of the drop-in call compute_transmittance on the five-gas
+    workload: host SpectralData and profile in, host T out, so the upload of
+    the line list, the host prelude (groups, band mask, sort) and the copy of
+    T back are inside the time (one call = one full spectrum)."""
+    from pylblrtm_amd import compute_transmittance, synth
+    sd, prof = synth.five_gas(lines_per_gas=args.lines_per_gas)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        compute_transmittance(sd, prof, 500.0, 2500.0, step_cm1=args.step, precision=prec)
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(args.drop_in):
+            t0 = time.perf_counter()
+            nu, T = compute_transmittance(sd, prof, 500.0, 2500.0, step_cm1=args.step,
+                                          precision=prec)
+            t.append(time.perf_counter() - t0)
+        res[prec] = {"ms_per_call": 1e3 * float(np.mean(t)), "ms_min": 1e3 * float(np.min(t)),
+                     "calls": args.drop_in, "points": int(len(nu))}
+        log(f"[bench] compute_transmittance {prec}: {res[prec]['ms_per_call']:.1f} ms per call")
+    print(json.dumps({"drop_in_compute_transmittance": res,
+                      "workload": "five gases, 500,000 lines, 50 layers, 500-2500 cm-1 at "
+                                  f"{args.step:g} cm-1, mu 1.0, host arrays in and out"}), flush=True)
+    return 0
+
+
 def adjoint_bench(args, dev, torch):
     """Batched gradients: seeded perturbations of the five-gas layers (T_k +
     N(0, 2 K), vmr x lognormal(0, 0.1)) over 1000-1100 cm^-1 at 1e-4 cm^-1
@@ -406,6 +432,8 @@ def main() -> int:
     ap.add_argument("--step", type=float, default=0.001, help="grid step [cm-1]")
     ap.add_argument("--adjoint", type=int, default=0, metavar="N_PROFILES",
                     help="run the batched-gradient workload instead (1000-1100 cm-1 at 1e-4)")
+    ap.add_argument("--drop-in", type=int, default=0, metavar="CALLS",
+                    help="time compute_transmittance (host arrays in, T out) on the five-gas workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -443,6 +471,8 @@ def run(args, pool) -> int:
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
     if args.adjoint:
         return adjoint_bench(args, dev, torch)
+    if args.drop_in:
+        return drop_in_bench(args, torch)
     t = time.perf_counter()
     sd, prof = synth.five_gas(lines_per_gas=args.lines_per_gas)
     lay = prof.build_layers()
