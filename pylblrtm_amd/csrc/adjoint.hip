@@ -36,6 +36,11 @@ namespace {
 constexpr int kVjpPoints = 8;
 constexpr int kVjpChunk = kBlock;
 constexpr int kVjpFold = 32;
+// Far-field series of the adjoint: terms of the one-node form's r = 1/(x^2 + c1)
+// and r^2 (whose double pole converges more slowly): outside the near range
+// 22 terms leave <= 1.1e-9 of r, 1.8e-7 of r^2 and 1.7e-8 of x r^2.
+constexpr int kAdjTerms = 22;
+constexpr int kAdjSliceExCap = 1024;
 
 struct GradArgs {
   lbl_lines_t lines;
@@ -118,6 +123,10 @@ struct VjpArgs {
   const int32_t* gas_of_group;
   const int32_t* lo;
   const int32_t* hi;
+  const int32_t* wlo;          // near range of each (slice, wavefront), k_wave_ranges
+  const int32_t* whi;
+  const int32_t* slo;          // near range of each slice
+  const int32_t* shi;
   const double* t_k;
   int64_t n_lines;
   int32_t n_layers, n_gases;
@@ -166,17 +175,46 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One workgroup per (tile slice, layer): P consecutive points per thread,
-// the layer's records of the tile staged through LDS in chunks.  Each chunk
-// is counting-sorted by gas (ballot ranks, so the order is deterministic) so
-// the per-gas gradient is summed in FP32 over runs of one gas and folded into
-// FP64 every kVjpFold records.
+// One workgroup per (tile slice, layer): P consecutive points per thread.
+//
+// Far field.  As in the forward's series (k_far_series_mixed), a record far
+// from a slice or wavefront middle point contributes smooth functions of the
+// unit's normalised offset tau; here they are the one-node form's
+// r = 1/(x^2 + c1), r^2 and x r^2, x = xm + sh tau, whose sums against WR over
+// the unit's points are the point sums S1, S2, S3 of the one-node branch
+// below.  With the unit's moments M_n = sum_j WR_j tau_j^n (computed once per
+// workgroup),
+//   S1 = sum_n a_n M_n,  S2 = sum_n b_n M_n,  S3 = xm S2 + sh sum_n b_n M_(n+1),
+// where a_n are r's Taylor coefficients (a_0 = 1/D0, a_1 = al a_0,
+// a_n = al a_(n-1) + be a_(n-2); D0 = xm^2 + c1, al = -2 xm sh / D0,
+// be = -sh^2 / D0) and b_n = -d a_n / d c1 those of r^2:
+// b_n = a_0 a_n + al b_(n-1) + be b_(n-2).  The split of records is the
+// forward's: slice level for records outside the slice's near range where
+// the one-node form holds at every point of the slice, wavefront level for
+// the rest outside the wavefront's near range where it holds at every point
+// of the wavefront, point by point for everything else.
+//
+// Near field.  The records of the slice's near range (and the slice level's
+// exceptions) are staged through LDS in chunks, each chunk counting-sorted
+// by gas (ballot ranks, so the order is deterministic) so the per-gas
+// gradient is summed in FP32 over runs of one gas and folded into FP64
+// every kVjpFold records; each wavefront skips the records its series took.
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
+  static_assert(P == kPointsMixed && kBlock * P == kSlicePoints && 64 * P == kWavePoints,
+                "slice and wavefront geometry of the forward's series");
   constexpr int kWaves = kBlock / 64;
+  constexpr int kNT = kAdjTerms;
   __shared__ VjpFar s_far[kVjpChunk];
   __shared__ VjpNear s_near[kVjpChunk];
   __shared__ int32_t s_wcnt[kWaves][8];
+  __shared__ int32_t s_idx[kVjpChunk];       // line index of each staged record
+  __shared__ float s_ds1[kVjpChunk];         // its one-node distance ds1
+  __shared__ float s_ms[kWaves][kNT + 1];    // slice moments, per wavefront's share
+  __shared__ float s_mw[kWaves][kNT + 1];    // wavefront moments
+  __shared__ float s_msum[kNT + 1];          // slice moments
+  __shared__ int32_t s_sx[kAdjSliceExCap];   // slice-level exceptions
+  __shared__ int s_sxn;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
   const int k = (int)(lw / a.n_wg);
@@ -232,14 +270,168 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
   for (int g = 0; g < 8; ++g) g_v[g] = 0.0;
 
   const int64_t kb = (int64_t)k * a.n_lines;
-  for (int32_t base = rlo; base < rhi; base += kVjpChunk) {
-    const int cnt = min(kVjpChunk, rhi - base);
-    __syncthreads();   // previous chunk consumed (s_far, s_near, s_wcnt)
+  // ---- far field: moments, then the slice and wavefront series
+  const int64_t s0 = t0 + (int64_t)slice * kSlicePoints;
+  const int64_t w_first = s0 + (int64_t)wv * kWavePoints;
+  const double nu_s = slice_middle(a.nu_min, a.step, s0);
+  const double nu_w = wave_middle(a.nu_min, a.step, w_first);
+  const float dS = 0.5f * (float)(kSlicePoints - kWavePoints) * stepf;   // Hs - H
+  const int32_t slo = min(max(a.slo[wg], rlo), rhi);
+  const int32_t shi = max(min(a.shi[wg], rhi), slo);
+  const int32_t nlo = min(max(a.wlo[wg * kWaves + wv], slo), shi);
+  const int32_t nhi = max(min(a.whi[wg * kWaves + wv], shi), nlo);
+  {
+    // moments of WR about the slice's and this wavefront's middle (WR = 0
+    // past the tile end), one set at a time to keep registers free
+#pragma unroll
+    for (int lvl = 0; lvl < 2; ++lvl) {
+      float mo[kNT + 1];
+#pragma unroll
+      for (int n = 0; n <= kNT; ++n) mo[n] = 0.0f;
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        const float t = lvl == 0
+            ? ((float)(wv * kWavePoints + lane * P + j) - 0.5f * (float)(kSlicePoints - 1)) *
+                  (2.0f / (float)kSlicePoints)
+            : ((float)(lane * P + j) - 0.5f * (float)(kWavePoints - 1)) * (2.0f / (float)kWavePoints);
+        float pw = WR[j];
+#pragma unroll
+        for (int n = 0; n <= kNT; ++n) {
+          mo[n] += pw;
+          pw *= t;
+        }
+      }
+#pragma unroll
+      for (int n = 0; n <= kNT; ++n) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) mo[n] += __shfl_xor(mo[n], off);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int n = 0; n <= kNT; ++n) (lvl == 0 ? s_ms : s_mw)[wv][n] = mo[n];
+      }
+    }
+    if (tid == 0) s_sxn = 0;
+  }
+  __syncthreads();
+  if (tid <= kNT) {
+    float t = 0.0f;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += s_ms[w][tid];
+    s_msum[tid] = t;
+  }
+  __syncthreads();
+  // S1 = sum W r, S2 = sum W r^2, S3 = sum W x r^2 of one record from the
+  // moments M (read from LDS as they are needed: registers are scarce here)
+  auto series = [&](const float* M, float xm, float sh, float c1, float& S1, float& S2,
+                    float& S3) {
+    const float a0 = __builtin_amdgcn_rcpf(fmaf(xm, xm, c1));
+    const float u = sh * a0;
+    const float al = -2.0f * xm * u, be = -sh * u;
+    float ap = a0, an = al * a0;
+    float bp = a0 * a0, bn = 2.0f * al * bp;
+    float s1 = fmaf(an, M[1], a0 * M[0]);
+    float s2 = fmaf(bn, M[1], bp * M[0]);
+    float s3 = fmaf(bn, M[2], bp * M[1]);
+#pragma unroll
+    for (int n = 2; n < kNT; ++n) {
+      const float a_n = fmaf(al, an, be * ap);
+      const float b_n = fmaf(a0, a_n, fmaf(al, bn, be * bp));
+      s1 = fmaf(a_n, M[n], s1);
+      s2 = fmaf(b_n, M[n], s2);
+      s3 = fmaf(b_n, M[n + 1], s3);
+      ap = an; an = a_n;
+      bp = bn; bn = b_n;
+    }
+    S1 = s1;
+    S2 = s2;
+    S3 = fmaf(xm, s2, sh * s3);
+  };
+  // adds record li's series (middle point at offset d, half-span sh/s) to the gradients
+  auto far_record = [&](const float* M, const MixedRec& m, int64_t li, float d, float sh) {
+    const int64_t r = kb + li;
+    float S1, S2, S3;
+    series(M, d * m.s, sh, m.c1, S1, S2, S3);
+    const float y = m.y, am = m.a;
+    const float aq = am * (float)kInvSqrtPi, ay = m.ay;
+    const float cty = a.dyt[r] * aq, cvy = a.dyv[r] * aq;
+    const float d2 = 2.0f * y * y;
+    const float tX2 = 2.0f * ax_t * ay;
+    const float tA = fmaf(a.at[r], ay, cty) - tX2;
+    const float ft = fmaf(tA, S1, fmaf(tX2, m.c1, -cty * d2) * S2);
+    const float vF = fmaf(a.av[r], ay, cvy);
+    const float fv = fmaf(vF, S1, fmaf(-2.0f * a.bxv[r] * ay, S3, -cvy * d2 * S2));
+    g_t += (double)ft;
+    int gas = a.gas_of_group[a.group[li]];
+#pragma unroll
+    for (int gg = 0; gg < 8; ++gg)
+      if (gg == gas) g_v[gg] += (double)fv;
+  };
+  // the one-node form holds at every point of the slice / of this wavefront
+  auto slice_ok = [&](const MixedRec& m) { return fabsf((float)(nu_s - m.c)) >= m.ds1 + dS; };
+  auto wave_ok = [&](const MixedRec& m) { return fabsf((float)(nu_w - m.c)) >= m.ds1; };
+  const MixedRec* __restrict__ lrec = recs + kb;
+  {
+    // slice level: records of the tile outside [slo, shi), one per thread
+    const float* const M = s_msum;
+    const int32_t n_below = slo - rlo, n_sl = n_below + (rhi - shi);
+    for (int32_t v = tid; v < n_sl; v += kBlock) {
+      const int64_t li = v < n_below ? (int64_t)rlo + v : (int64_t)shi + (v - n_below);
+      const MixedRec m = lrec[li];
+      if (!(m.c >= lo_b && m.c <= hi_b)) continue;   // outside the tile window (or skipped)
+      if (slice_ok(m)) {
+        far_record(M, m, li, (float)(nu_s - m.c), 4.0f * m.sh);
+      } else {
+        const int pos = atomicAdd(&s_sxn, 1);
+        if (pos < kAdjSliceExCap) s_sx[pos] = (int32_t)li;
+      }
+    }
+  }
+  __syncthreads();   // s_sx, s_sxn complete
+  const int n_sx = s_sxn;
+  const bool rescan = n_sx > kAdjSliceExCap;
+  {
+    // wavefront level: [slo, nlo), [nhi, shi) and the slice level's
+    // exceptions (all of the tile outside [slo, shi) that failed there, if
+    // the list overflowed), one per lane
+    const float* const M = s_mw[wv];
+    const int32_t nA = nlo - slo, nB = shi - nhi;
+    const int32_t nC = rescan ? (slo - rlo) + (rhi - shi) : n_sx;
+    for (int32_t v = lane; v < nA + nB + nC; v += 64) {
+      int64_t li;
+      if (v < nA) li = (int64_t)slo + v;
+      else if (v < nA + nB) li = (int64_t)nhi + (v - nA);
+      else if (!rescan) li = s_sx[v - nA - nB];
+      else {
+        const int32_t u = v - nA - nB;
+        li = u < slo - rlo ? (int64_t)rlo + u : (int64_t)shi + (u - (slo - rlo));
+      }
+      const MixedRec m = lrec[li];
+      if (!(m.c >= lo_b && m.c <= hi_b)) continue;
+      if (rescan && v >= nA + nB && slice_ok(m)) continue;   // the slice level summed it
+      if (wave_ok(m)) far_record(M, m, li, (float)(nu_w - m.c), m.sh);
+    }
+  }
+
+  // ---- near field: the slice's near range and the slice level's exceptions
+  // (the whole tile if their list overflowed), point by point, each record
+  // skipped by the wavefronts whose series took it
+  const int32_t n_stage = rescan ? rhi - rlo : (shi - slo) + n_sx;
+  auto stage_index = [&](int32_t v) -> int64_t {
+    if (rescan) return (int64_t)rlo + v;
+    return v < shi - slo ? (int64_t)slo + v : (int64_t)s_sx[v - (shi - slo)];
+  };
+  for (int32_t base = 0; base < n_stage; base += kVjpChunk) {
+    const int cnt = min(kVjpChunk, n_stage - base);
+    __syncthreads();   // previous chunk consumed (s_far, s_near, s_wcnt, s_idx, s_ds1)
     VjpFar f;
     VjpNear n;
     int gas = -1;
+    int32_t li = -1;
+    float ds1 = 0.0f;
     if (tid < cnt) {
-      const int64_t r = kb + base + tid;
+      li = (int32_t)stage_index(base + tid);
+      const int64_t r = kb + li;
       const MixedRec& mr = recs[r];
       double c = mr.c;
       float s = mr.s, y = mr.y, am = mr.a;
@@ -264,7 +456,8 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       f.tA = f.tF - f.tX2;
       f.vF = av * ay + cvy; f.vX = 2.0f * bxv * ay; f.vD = cvy * d2; f.vQ = -2.0f * d2 * cvy;
       n.y = y; n.a = am; n.at = at; n.dyt = dyt; n.av = av; n.bxv = bxv; n.dyv = dyv;
-      gas = a.gas_of_group[a.group[base + tid]];
+      ds1 = mr.ds1;
+      gas = a.gas_of_group[a.group[li]];
       if (gas >= n_gas) gas = -1;   // cannot happen for validated inputs
       n.gas = gas;
     }
@@ -283,6 +476,8 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       for (int w = 0; w < wv; ++w) off += s_wcnt[w][gas];
       s_far[off] = f;
       s_near[off] = n;
+      s_idx[off] = li;
+      s_ds1[off] = ds1;
     }
     __syncthreads();
     int q0 = 0;
@@ -295,6 +490,16 @@ __global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
       double dv = 0.0;
       for (int qi = q0; qi < q1; ++qi) {
         const VjpFar& R = s_far[qi];
+        {
+          // taken by a series (uniform across the wavefront): the same tests
+          // as the far field's, on the same values
+          const int32_t li = s_idx[qi];
+          const float ds1 = s_ds1[qi];
+          const bool by_slice = (li < slo || li >= shi) &&
+                                fabsf((float)(nu_s - R.c)) >= ds1 + dS;
+          const bool by_wave = (li < nlo || li >= nhi) && fabsf((float)(nu_w - R.c)) >= ds1;
+          if (by_slice || by_wave) continue;
+        }
         // x at the middle of the run; point j at xm + (j - (P-1)/2) hs
         const float xm = (float)(nu_m - R.c) * R.s;
         const float hs = R.hs;
@@ -401,7 +606,9 @@ hipError_t launch_line_grad(const lbl_lines_t& lines, const lbl_groups_t& groups
 hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, const float* av,
                       const float* bxv, const float* dyv, const int32_t* group,
                       const int32_t* gas_of_group, const int32_t* lo, const int32_t* hi,
-                      const double* t_k, int64_t n_lines, int n_layers, int n_gases,
+                      const int32_t* wlo, const int32_t* whi, const int32_t* slo,
+                      const int32_t* shi, const double* t_k, int64_t n_lines, int n_layers,
+                      int n_gases,
                       const lbl_grid_t& grid, int64_t n_tiles, int n_mu, const double* mu,
                       const float* trans, const float* grad_trans, const float* od_layers,
                       double* grad_t, double* grad_vmr, hipStream_t s) {
@@ -422,6 +629,10 @@ hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, con
   a.gas_of_group = gas_of_group;
   a.lo = lo;
   a.hi = hi;
+  a.wlo = wlo;
+  a.whi = whi;
+  a.slo = slo;
+  a.shi = shi;
   a.t_k = t_k;
   a.n_lines = n_lines;
   a.n_layers = n_layers;

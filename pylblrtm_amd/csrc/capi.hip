@@ -181,6 +181,30 @@ int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
   return LBL_OK;
 }
 
+// Near ranges of the far-field series: per (slice, wavefront) and per slice,
+// the lines whose nu0 lies within kSeriesRatio half-spans (+ the shift bound)
+// of the unit's middle point.
+static hipError_t launch_near_ranges(const lbl_lines_t* lines, const lbl_grid_t* grid,
+                                     double margin, const Layout& L, int precision, int32_t* wlo,
+                                     int32_t* whi, int32_t* slo, int32_t* shi, hipStream_t s) {
+  lbl::WaveRangeArgs wr{};
+  wr.nu0 = lines->nu0;
+  wr.n_lines = lines->n_lines;
+  wr.nu_min = grid->nu_min;
+  wr.step = grid->step;
+  wr.half_width = lbl::kSeriesRatio * 0.5 * lbl::kWavePoints * grid->step + margin;
+  wr.slice_half_width = lbl::kSeriesRatio * 0.5 * lbl::kSlicePoints * grid->step + margin;
+  wr.ts = grid->tile_samples;
+  wr.n_wg = L.n_wg;
+  wr.slices_per_tile = L.slices_per_tile;
+  wr.waves_per_slice = lbl::sum_block(precision) / 64;
+  wr.wlo = wlo;
+  wr.whi = whi;
+  wr.slo = slo;
+  wr.shi = shi;
+  return lbl::launch_wave_ranges(wr, s);
+}
+
 static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* groups,
                               const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
                               const lbl_outputs_t* out, const lbl_surrogate_t* sur,
@@ -243,24 +267,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
   int32_t* slo = reinterpret_cast<int32_t*>(ws + L.slo);
   int32_t* shi = reinterpret_cast<int32_t*>(ws + L.shi);
-  {
-    lbl::WaveRangeArgs wr{};
-    wr.nu0 = lines->nu0;
-    wr.n_lines = lines->n_lines;
-    wr.nu_min = grid->nu_min;
-    wr.step = grid->step;
-    wr.half_width = lbl::kSeriesRatio * 0.5 * lbl::kWavePoints * grid->step + tr.margin;
-    wr.slice_half_width = lbl::kSeriesRatio * 0.5 * lbl::kSlicePoints * grid->step + tr.margin;
-    wr.ts = grid->tile_samples;
-    wr.n_wg = L.n_wg;
-    wr.slices_per_tile = L.slices_per_tile;
-    wr.waves_per_slice = lbl::sum_block(precision) / 64;
-    wr.wlo = wlo;
-    wr.whi = whi;
-    wr.slo = slo;
-    wr.shi = shi;
-    HIP_TRY(lbl::launch_wave_ranges(wr, s));
-  }
+  HIP_TRY(launch_near_ranges(lines, grid, tr.margin, L, precision, wlo, whi, slo, shi, s));
 
   lbl::SumArgs sa{};
   sa.rec = rec;
@@ -406,9 +413,16 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   tr.hi = hi;
   tr.margin = fabs(grid->shift_bound) * (1.0 + 1e-9) + 1e-9;
   HIP_TRY(lbl::launch_tile_ranges(tr, s));
+  // near ranges of the far-field series, as for the forward's mixed line sum
+  int32_t* wlo = reinterpret_cast<int32_t*>(ws + L.wlo);
+  int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
+  int32_t* slo = reinterpret_cast<int32_t*>(ws + L.slo);
+  int32_t* shi = reinterpret_cast<int32_t*>(ws + L.shi);
+  HIP_TRY(launch_near_ranges(lines, grid, tr.margin, L, LBL_MIXED, wlo, whi, slo, shi, s));
   HIP_TRY(lbl::launch_line_grad(*lines, *groups, *layers, gc[0], gc[1], gc[2], gc[3], gc[4], s));
   HIP_TRY(lbl::launch_vjp(rec, gc[0], gc[1], gc[2], gc[3], gc[4], lines->group, groups->gas, lo,
-                          hi, layers->t_k, lines->n_lines, layers->n_layers, layers->n_gases,
+                          hi, wlo, whi, slo, shi, layers->t_k, lines->n_lines, layers->n_layers,
+                          layers->n_gases,
                           *grid, L.n_tiles, n_mu, mu, transmittance, grad_transmittance,
                           optical_depth_layers, grad_t, grad_vmr, s));
   uint32_t code = lbl::kErrNone;

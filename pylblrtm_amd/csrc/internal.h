@@ -221,7 +221,9 @@ hipError_t launch_line_grad(const lbl_lines_t& lines, const lbl_groups_t& groups
 hipError_t launch_vjp(const Records& rec, const float* at, const float* dyt, const float* av,
                       const float* bxv, const float* dyv, const int32_t* group,
                       const int32_t* gas_of_group, const int32_t* lo, const int32_t* hi,
-                      const double* t_k, int64_t n_lines, int n_layers, int n_gases,
+                      const int32_t* wlo, const int32_t* whi, const int32_t* slo,
+                      const int32_t* shi, const double* t_k, int64_t n_lines, int n_layers,
+                      int n_gases,
                       const lbl_grid_t& grid, int64_t n_tiles, int n_mu, const double* mu,
                       const float* trans, const float* grad_trans, const float* od_layers,
                       double* grad_t, double* grad_vmr, hipStream_t s);
