@@ -554,7 +554,9 @@ void k_far_series_fp64(const SumArgs a) {
   for (int n = 0; n < kN; ++n) C[n] = 0.0;
   const int n_sx = __builtin_amdgcn_readfirstlane(s_sxn);
   const bool rescan = n_sx > kSliceExCap;
-  const int32_t nA = nlo - slo, nB = shi - nhi, nC = min(n_sx, kSliceExCap);
+  // after an overflow the rescan (nD) covers every listed record too: the
+  // list is not read then, or its records would be summed twice
+  const int32_t nA = nlo - slo, nB = shi - nhi, nC = rescan ? 0 : n_sx;
   const int32_t nD = rescan ? (slo - rlo) + (rhi - shi) : 0;
   const int32_t n_far = nA + nB + nC + nD;
   auto far_index = [&](int32_t v) -> int64_t {
@@ -840,7 +842,9 @@ void k_far_series_mixed(const SumArgs a) {
   float S0 = 0.0f, S0c = 0.0f, S1 = 0.0f, S1c = 0.0f;
   const int n_sx = __builtin_amdgcn_readfirstlane(s_sxn);
   const bool rescan = n_sx > kSliceExCap;
-  const int32_t nA = nlo - slo, nB = shi - nhi, nC = min(n_sx, kSliceExCap);
+  // after an overflow the rescan (nD) covers every listed record too: the
+  // list is not read then, or its records would be summed twice
+  const int32_t nA = nlo - slo, nB = shi - nhi, nC = rescan ? 0 : n_sx;
   const int32_t nD = rescan ? (slo - rlo) + (rhi - shi) : 0;
   const int32_t n_far = nA + nB + nC + nD;
   auto far_index = [&](int32_t v) -> int64_t {

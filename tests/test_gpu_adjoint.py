@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import adjoint_oracle as A
-from pylblrtm_amd import SpectralData
+from pylblrtm_amd import SpectralData, synth
 from pylblrtm_amd.atmosphere import _Layers
 from pylblrtm_amd.device import OpticalDepthProblem
 from test_adjoint_oracle import translucent_case
@@ -230,3 +230,96 @@ def test_batched_profiles_match_one_by_one():
                                    rtol=1e-6, atol=1e-7)
         assert _rel(tt.grad[b].cpu().numpy(), t1.grad.cpu().numpy()) < 1e-6, b
         assert _rel(vv.grad[b].cpu().numpy(), v1.grad.cpu().numpy()) < 1e-6, b
+
+
+def _vjp_vs_oracle(prob, case, mus, seed, tile_width):
+    import torch
+    nu, lines, tips, mm, p, t, vmr, air = case
+    fw = prob.run("mixed", mus=mus, want_layers=True)
+    T = fw["transmittance"]
+    G = np.random.default_rng(seed).normal(size=tuple(T.shape))
+    gt, gv = prob.vjp(T, fw["optical_depth_layers"],
+                      torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
+    rt, rv = A.vjp(nu, lines, tips, mm, p, t, vmr, air, prob.cutoff, tile_width, mus, G)
+    ref_v = np.stack([rv[g] for g in prob.gases])
+    return _rel(gt.cpu().numpy(), rt), _rel(gv.cpu().numpy(), ref_v)
+
+
+def test_vjp_several_wavefronts_per_slice():
+    # 2 cm-1 tiles at 0.0005: 4001 points, two slices of four live wavefronts,
+    # so the slice-level series, the wavefront-level series of each wavefront
+    # and the near field's per-wavefront skips all take part
+    say("case: vjp vs oracle adjoint, 700-702 cm-1 at 0.0005, 2 cm-1 tiles")
+    nu, lines, tips, mm, p, t, vmr, air = translucent_case()
+    nu = A.O.grid(700.0, 702.0, 0.0005)
+    sd = SpectralData(lines=lines, tips=tips, molar_masses=mm)
+    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()})
+    prob = OpticalDepthProblem(sd, lay, 700.0, 702.0, 0.0005, 25.0, 2.0)
+    et, ev = _vjp_vs_oracle(prob, (nu, lines, tips, mm, p, t, vmr, air), (1.0, 0.5), 21, 2.0)
+    say(f"  rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
+    assert et < GRAD_RTOL
+    assert ev < GRAD_RTOL
+
+
+def dense_broad_case(n=5000):
+    """5,000 broad H2O-like lines (g_air, g_self 0.4-0.7 cm-1/atm) within
+    2.5 cm-1 of a 0.6 cm-1 window at 1e-4: in the surface layer about 1,500
+    records per slice lie outside the slice's near range yet too near for the
+    one-node form to hold over the whole slice, more than the 1,024-entry list
+    of the series kernels holds, so their rescan path runs (checked below)."""
+    rng = np.random.default_rng(17)
+    lines = synth.random_lines(rng, n, 2497.5, 2502.5, 1)
+    lines["g_air"] = rng.uniform(0.4, 0.7, n)
+    lines["g_self"] = rng.uniform(0.4, 0.7, n)
+    lines["Sref"] = 10.0 ** rng.uniform(-27.5, -25.5, n)
+    tips = {(1, 1): synth.tips_table(174.6)}
+    mm = {(1, 1): 18.01}
+    p = np.array([10.0, 100.0, 300.0, 600.0, 1000.0])
+    lay_p = np.sqrt(p[:-1] * p[1:]) / 1013.25
+    lay_t = np.array([220.3, 235.7, 258.4, 281.1])
+    air = (6.02214086e23 / (9.80665 * 0.0289644)) * (np.diff(p) * 100.0) / 1e4
+    vmr = {"H2O": np.array([5e-6, 2e-4, 2e-3, 8e-3])}
+    nu = A.O.grid(2499.7, 2500.3, 1e-4)
+    return nu, lines, tips, mm, lay_p, lay_t, vmr, air
+
+
+def _slice_exceptions(case, k):
+    """Records of layer k outside each slice's near range (3 half-spans) for
+    which the one-node form fails somewhere in the slice (float64 version of
+    the kernels' test)."""
+    nu, lines, tips, mm, p, t, vmr, air = case
+    idx = np.arange(len(lines["nu0"]))
+    c, _, lor, dop = A.O.line_params(lines, idx, t[k], p[k], vmr["H2O"][k], 1.0, 1.0, 18.01)
+    s = np.sqrt(np.log(2.0)) / dop
+    y2 = (np.sqrt(np.log(2.0)) * lor / dop) ** 2
+    x_s1 = np.sqrt(np.maximum(np.maximum(625.0, np.sqrt(164062.5 * (2.0 * y2 + 1.5))) - y2, 0.0))
+    step = nu[1] - nu[0]
+    hs = 1024 * step
+    out = []
+    for i0 in range(0, len(nu), 2048):
+        mid = nu[0] + (i0 + 1023.5) * step
+        far = np.abs(lines["nu0"] - mid) > 3.0 * hs
+        fails = np.abs(mid - c) < x_s1 / s + hs
+        out.append(int(np.sum(far & fails)))
+    return out
+
+
+def test_slice_exception_overflow_forward_and_vjp():
+    say("case: dense broad lines, slice-level exception list overflows (forward and vjp)")
+    case = dense_broad_case()
+    nu, lines, tips, mm, p, t, vmr, air = case
+    assert min(_slice_exceptions(case, 3)) > 1024   # the rescan path is taken
+    sd = SpectralData(lines=lines, tips=tips, molar_masses=mm)
+    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()})
+    prob = OpticalDepthProblem(sd, lay, 2499.7, 2500.3, 1e-4, 25.0, 2.0)
+    ref = A.forward(nu, lines, tips, mm, p, t, vmr, air, 25.0, 2.0, (1.0,))[0]
+    T64 = prob.run("fp64", mus=(1.0,))["transmittance"][0].cpu().numpy()
+    T32 = prob.run("mixed", mus=(1.0,))["transmittance"][0].double().cpu().numpy()
+    d64, d32 = float(np.max(np.abs(T64 - ref))), float(np.max(np.abs(T32 - ref)))
+    say(f"  forward max |dT|: fp64 {d64:.2e}, mixed {d32:.2e}")
+    assert d64 < 1e-9
+    assert d32 < 1e-5
+    et, ev = _vjp_vs_oracle(prob, case, (1.0,), 23, 2.0)
+    say(f"  vjp rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
+    assert et < GRAD_RTOL
+    assert ev < GRAD_RTOL
