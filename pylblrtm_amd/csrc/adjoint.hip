@@ -200,7 +200,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // gradient is summed in FP32 over runs of one gas and folded into FP64
 // every kVjpFold records; each wavefront skips the records its series took.
 template <int P>
-__global__ __launch_bounds__(kBlock) void k_vjp_mixed(const VjpArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_vjp_mixed(const VjpArgs a) {
   static_assert(P == kPointsMixed && kBlock * P == kSlicePoints && 64 * P == kWavePoints,
                 "slice and wavefront geometry of the forward's series");
   constexpr int kWaves = kBlock / 64;

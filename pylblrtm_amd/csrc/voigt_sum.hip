@@ -388,6 +388,9 @@ void k_far_series_fp64(const SumArgs a) {
   __shared__ int32_t s_sx[kSliceExCap];
   __shared__ int s_sxn;
   __shared__ double s_coef[kWaves][kN];
+  // the slice series plus the wavefront level's exceptions at each thread's
+  // own points (written and read by that thread only)
+  __shared__ double s_exv[kBlock * P];
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
   const int k = (int)(lw / a.n_wg);
   const int64_t wg = lw - (int64_t)k * a.n_wg;
@@ -478,6 +481,7 @@ void k_far_series_fp64(const SumArgs a) {
     return fma(xn, xn, y2) >= kFp64Gh4R2;
   };
   if (threadIdx.x == 0) s_sxn = 0;
+  double* const exv = s_exv + threadIdx.x * P;
   __syncthreads();
 
   // ---- slice level: records outside [slo, shi), one per thread
@@ -525,8 +529,9 @@ void k_far_series_fp64(const SumArgs a) {
   __syncthreads();
   if (!wave_live) return;   // the last barrier is behind every wavefront
 
-  // the slice series goes to far_sum at once (no registers held through the
-  // wavefront loop); the wavefront level adds to it
+  // the slice series goes to this thread's LDS slots at once (no registers
+  // held through the wavefront loop); the wavefront level's exceptions add to
+  // it there, and far_sum is written once at the end
   double* const out = static_cast<double*>(a.far_sum) + (int64_t)k * a.n_nu;
   {
     double T[kN];
@@ -544,7 +549,7 @@ void k_far_series_fp64(const SumArgs a) {
       double f = T[kN - 1];
 #pragma unroll
       for (int n = kN - 2; n >= 0; --n) f = fma(f, t, T[n]);
-      if (i0 + j < t1) out[i0 + j] = f;
+      exv[j] = f;
     }
   }
 
@@ -593,7 +598,7 @@ void k_far_series_fp64(const SumArgs a) {
         } else {
           v = am * K_fp64_call(x, y);
         }
-        if (i0 + j < t1) out[i0 + j] += v;   // this lane's own point
+        exv[j] += v;   // this lane's own point
       }
     }
   };
@@ -647,7 +652,7 @@ void k_far_series_fp64(const SumArgs a) {
 #pragma unroll
     for (int n = kN - 2; n >= 0; --n) f = fma(f, t, C[n]);
     const int64_t i = i0 + j;
-    if (i < t1) out[i] += f;
+    if (i < t1) out[i] = f + exv[j];
   }
 }
 
