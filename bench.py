@@ -176,7 +176,7 @@ def _cpu_task(task):
 
 
 def _cpu_ready(_):
-    return "sd" in _CPU
+    return "sd" in _CPU or "co2" in _CPU
 
 
 def cpu_pool(args):
@@ -302,6 +302,75 @@ def surrogate_report(prob, E, value_plain, mus, steps, capi, torch):
     log(f"[bench] surrogate: {sps * 1e3:.1f} ms/step ({rep['ratio_vs_plain_fp32']:.3f}x plain FP32); "
         f"in-domain network/exact = {rep['in_domain_ratio_network_vs_exact']:.3f}")
     return rep
+
+
+def _co2_init() -> None:
+    from threadpoolctl import threadpool_limits
+    from oracle import lbl_oracle as O
+    from pylblrtm_amd import synth
+    threadpool_limits(1)
+    sd, prof = synth.co2_window()
+    _CPU["co2"] = (sd, prof.build_layers(), O.grid(700.0, 750.0, 0.001))
+
+
+def _co2_task(tile):
+    from oracle import lbl_oracle as O
+    sd, lay, nu = _CPU["co2"]
+    stats = {}
+    t = time.perf_counter()
+    od = O.optical_depth(nu, sd.lines, sd.tips, sd.molar_masses, lay.pressure_atm,
+                         lay.temperature_k, lay.vmr, lay.column_cm2, 25.0, 2.0, tiles=[tile],
+                         stats=stats)
+    ts = O.tile_samples(nu, 2.0)
+    a, b = tile * ts, min(tile * ts + ts, nu.size)
+    return tile, time.perf_counter() - t, int(stats["evals"]), od[:, a:b]
+
+
+def co2_bench(args, pool):
+    """The reference's own CPU-sized case (one CO2-like isotopologue, 5,000
+    lines, 20 layers, 700-750 cm^-1 at 0.001, mu 1): GPU steps in both
+    precisions, and the FP64 oracle over the whole grid, one tile per task on
+    the host's cores (the sum of the task times is the one-core time)."""
+    import torch
+    from pylblrtm_amd import _capi, synth
+    from pylblrtm_amd.device import OpticalDepthProblem
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    sd, prof = synth.co2_window()
+    lay = prof.build_layers()
+    prob = OpticalDepthProblem(sd, lay, 700.0, 750.0, 0.001, 25.0, 2.0, device=dev)
+    E = prob.count_evaluations()
+    res = {"workload": "one CO2-like isotopologue, 5,000 lines, 20 layers, 700-750 cm-1 at 0.001 "
+                       "(50,001 points), 25 cm-1 cutoff, 2 cm-1 tiles, mu 1.0",
+           "evals_per_step": E}
+    od_gpu = {}
+    for prec in ("mixed", "fp64"):
+        _, sps, k_ms, _ = time_mode(prob, prec, (1.0,), args.co2_window, 2, _capi, torch)
+        o = prob.run(prec, mus=(1.0,), want_layers=True)
+        od_gpu[prec] = o["optical_depth_layers"].double().cpu().numpy()
+        res[prec] = {"ms_per_step": sps * 1e3, "line_sum_kernels_ms": k_ms, "evals_per_s": E / sps}
+        log(f"[bench] co2 window {prec}: {sps * 1e3:.3f} ms/step, {E / sps:.3e} evals/s")
+    n_tiles = len(prob.tile_edges()[0])
+    pool[0].map(_cpu_ready, range(pool[1]), chunksize=1)
+    t = time.perf_counter()
+    out = pool[0].map(_co2_task, range(n_tiles), chunksize=1)
+    wall = time.perf_counter() - t
+    od = np.concatenate([o[3] for o in sorted(out, key=lambda o: o[0])], axis=1)
+    evals = sum(o[2] for o in out)
+    one_core = sum(o[1] for o in out)
+    T_ref = np.exp(-od.sum(0))
+    res["cpu_oracle"] = {"evals": evals, "wall_s": wall, "cores": pool[1],
+                         "evals_per_s": evals / wall, "one_core_s": one_core,
+                         "one_core_evals_per_s": evals / one_core}
+    par = {}
+    for prec, g in od_gpu.items():
+        par[f"max_rel_od_{prec}"] = float(np.max(np.abs(g - od) / np.maximum(od, 1e-300)))
+        par[f"max_abs_dT_{prec}"] = float(np.max(np.abs(np.exp(-g.sum(0)) - T_ref)))
+    res["parity_vs_oracle_whole_grid"] = par
+    log(f"[bench] co2 window oracle: {evals:.3e} evaluations, {wall:.1f} s on {pool[1]} cores "
+        f"({one_core:.1f} s of one core); parity {par}")
+    print(json.dumps({"co2_window": res}), flush=True)
+    return 0
 
 
 def drop_in_bench(args, torch):
@@ -434,14 +503,26 @@ def main() -> int:
                     help="run the batched-gradient workload instead (1000-1100 cm-1 at 1e-4)")
     ap.add_argument("--drop-in", type=int, default=0, metavar="CALLS",
                     help="time compute_transmittance (host arrays in, T out) on the five-gas workload")
+    ap.add_argument("--co2-window", type=int, default=0, metavar="STEPS",
+                    help="the single-gas CO2 case (5,000 lines, 20 layers, 700-750 cm-1 at 0.001): "
+                         "GPU timing and the CPU oracle over the whole grid")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the path shards by tile range, see DESIGN.md)", file=sys.stderr)
+              f"(the path would shard by tile range; no multi-GPU run is built)", file=sys.stderr)
         return 2
 
+    if args.co2_window:
+        import multiprocessing as mp
+        n = max(1, min(16, os.cpu_count() or 1))
+        pool = (mp.get_context("fork").Pool(n, initializer=_co2_init), n)   # before the GPU
+        try:
+            return co2_bench(args, pool)
+        finally:
+            pool[0].terminate()
+            pool[0].join()
     pool = cpu_pool(args)   # forked before anything touches the GPU
     try:
         return run(args, pool)
@@ -516,7 +597,10 @@ def run(args, pool) -> int:
     result["roofline"] = {"bound": "valu", "achieved": d["achieved_tflops"], "peak": d["peak"],
                           "unit": "TFLOP/s", "frac": d["frac"], "traffic": d["traffic"],
                           "kernel": dom, "units_per_launch": d["units"],
-                          "flop_per_unit": d["flop_per_unit"], "kernels": kern}
+                          "flop_per_unit": d["flop_per_unit"], "kernels": kern,
+                          # the reference's evaluation count at 16 FLOP each over the two
+                          # line-sum kernels: above the peak, since the series does less work
+                          "reference_equivalent_tflops": E * FLOP_PER_EVAL / (k_ms * 1e-3) / 1e12}
     log("[bench] kernels: " + ", ".join(f"{n} {v['ms']:.2f} ms ({v['units']:.3e} units, "
                                         f"frac {v['frac']:.3f})" for n, v in kern.items()))
 
