@@ -179,7 +179,11 @@ void k_voigt_sum_fp64(const SumArgs a) {
         const double ax = fabs(xm);
         const u32x4 h2 = more[0];
         const double c1 = f64(h2.x, h2.y), ay = f64(h2.z, h2.w);
-        if (ax >= hw1) {
+        // One tier per record for the whole wavefront, the lowest any lane
+        // needs (each form holds wherever the ones above it do): no lane pays
+        // for two tiers.  Lanes that need the queue run the four-node form at
+        // a safe x with weight 0 and queue the pair.
+        if (__ballot(ax < hw1) == 0) {
           // shifted one-node form, A K = ay / D, D = x^2 + y^2 - 3/2; points step
           // from the first one by integer multiples of hs (inline constants of
           // v_fma_f64); the eight quotients share one v_rcp_f64 (Montgomery's
@@ -206,72 +210,87 @@ void k_voigt_sum_fp64(const SumArgs a) {
           lay[5] = fma(r45, D[4], lay[5]);
           lay[6] = fma(r67, D[7], lay[6]);
           lay[7] = fma(r67, D[6], lay[7]);
-        } else {
-          const u32x4 h3 = more[1];
-          const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
-          if (ax >= hw2) {
-            // second-order shifted form (faddeeva.h), A K = ay r / D with
-            // r = x^2 + y^2, D = r^2 - 3/2 r + 2 y^2 - 3/2; the eight quotients
-            // by batch inversion as above (D >= 6.9e7 here; a product of
-            // eight stays inside the FP64 range for any r below the one-node
-            // threshold)
-            const double y2 = c1 + 1.5, cc = fma(2.0, c1, 1.5);
-            const double x0 = fma(-half, hs, xm);
-            double rr[P], D[P];
+          return;
+        }
+        const u32x4 h3 = more[1];
+        const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
+        if (__ballot(ax < hw2) == 0) {
+          // second-order shifted form (faddeeva.h), A K = ay r / D with
+          // r = x^2 + y^2, D = r^2 - 3/2 r + 2 y^2 - 3/2 >= 6.9e7; the quotients
+          // by batch inversion in two batches of four (a product of four D
+          // ~ |z|^16 stays inside the FP64 range for |z| < 1e19, also in lanes
+          // far past the one-node threshold)
+          const double y2 = c1 + 1.5, cc = fma(2.0, c1, 1.5);
+          const double x0 = fma(-half, hs, xm);
+          double rr[P], D[P];
 #pragma unroll
-            for (int j = 0; j < P; ++j) {
-              const double x = fma((double)j, hs, x0);
-              rr[j] = fma(x, x, y2);
-              D[j] = fma(rr[j], rr[j] - 1.5, cc);
-            }
-            const double p01 = D[0] * D[1], p23 = D[2] * D[3], p45 = D[4] * D[5], p67 = D[6] * D[7];
-            const double p03 = p01 * p23, p47 = p45 * p67;
-            const double r = ay * rcp64_1nr(p03 * p47);
-            const double r03 = r * p47, r47 = r * p03;
-            const double r01 = r03 * p23, r23 = r03 * p01, r45 = r47 * p67, r67 = r47 * p45;
-            lay[0] = fma(r01 * D[1], rr[0], lay[0]);
-            lay[1] = fma(r01 * D[0], rr[1], lay[1]);
-            lay[2] = fma(r23 * D[3], rr[2], lay[2]);
-            lay[3] = fma(r23 * D[2], rr[3], lay[3]);
-            lay[4] = fma(r45 * D[5], rr[4], lay[4]);
-            lay[5] = fma(r45 * D[4], rr[5], lay[5]);
-            lay[6] = fma(r67 * D[7], rr[6], lay[6]);
-            lay[7] = fma(r67 * D[6], rr[7], lay[7]);
-          } else if (ax >= hw4) {
-            // four-node form (|z| >= 35 at every point); A (2y/pi) W = ay (2/sqrt(pi)) W
-            const double y2 = c1 + 1.5, fy2 = 4.0 * y2;
-            const double w0 = ay * (2.0 * kInvSqrtPi * kGh4W[0]);
-            const double w1 = ay * (2.0 * kInvSqrtPi * kGh4W[1]);
-            const double x0 = fma(-half, hs, xm);
-  #pragma unroll
-            for (int j = 0; j < P; ++j) {
-              const double x = fma((double)j, hs, x0);
+          for (int j = 0; j < P; ++j) {
+            const double x = fma((double)j, hs, x0);
+            rr[j] = fma(x, x, y2);
+            D[j] = fma(rr[j], rr[j] - 1.5, cc);
+          }
+#pragma unroll
+          for (int h = 0; h < P; h += 4) {
+            const double pa = D[h] * D[h + 1], pb = D[h + 2] * D[h + 3];
+            const double r = ay * rcp64_1nr(pa * pb);
+            const double ra_ = r * pb, rb_ = r * pa;
+            lay[h] = fma(ra_ * D[h + 1], rr[h], lay[h]);
+            lay[h + 1] = fma(ra_ * D[h], rr[h + 1], lay[h + 1]);
+            lay[h + 2] = fma(rb_ * D[h + 3], rr[h + 2], lay[h + 2]);
+            lay[h + 3] = fma(rb_ * D[h + 2], rr[h + 3], lay[h + 3]);
+          }
+          return;
+        }
+        // four-node form (|z| >= 35 at every point of the run; at the safe
+        // middle x = hw4 for lanes whose run reaches nearer, with weight 0);
+        // A (2y/pi) W = ay (2/sqrt(pi)) W; the two points of a pair share one
+        // v_rcp_f64 (a product of two D0 D1 ~ |z|^16)
+        const bool near = ax < hw4;
+        {
+          const double y2 = c1 + 1.5, fy2 = 4.0 * y2;
+          const double ayw = near ? 0.0 : ay;
+          const double w0 = ayw * (2.0 * kInvSqrtPi * kGh4W[0]);
+          const double w1 = ayw * (2.0 * kInvSqrtPi * kGh4W[1]);
+          const double x0 = fma(-half, hs, near ? hw4 : xm);
+#pragma unroll
+          for (int j = 0; j < P; j += 2) {
+            double Np[2], Dp[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const double x = fma((double)(j + u), hs, x0);
               const double x2 = x * x;
               const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
               const double u0 = xmy - t0sq, u1 = xmy - t1sq;
               const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
-              const double N = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
-              lay[j] = fma(N, rcp64_1nr(D0 * D1), lay[j]);
+              Np[u] = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
+              Dp[u] = D0 * D1;
             }
-          } else if (live) {
-            // append (q, thread) with one LDS atomic per wavefront
-            const uint64_t msk = __ballot(1);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-            int qb = 0;
-            if (rank == 0) qb = atomicAdd(&s_qn[wave], (int)__popcll(msk));
-            qb = __builtin_amdgcn_readfirstlane(qb);
-            const int pos = qb + (int)rank;
+            const double rp = rcp64_1nr(Dp[0] * Dp[1]);
+            lay[j] = fma(Np[0] * Dp[1], rp, lay[j]);
+            lay[j + 1] = fma(Np[1] * Dp[0], rp, lay[j + 1]);
+          }
+        }
+        const bool queued = near && live;
+        const uint64_t msk = __ballot(queued);
+        if (msk) {   // uniform
+          // append (q, lane) of the queued lanes with one LDS atomic per wavefront
+          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+          int qb = 0;
+          if (lane == 0) qb = atomicAdd(&s_qn[wave], (int)__popcll(msk));
+          qb = __builtin_amdgcn_readfirstlane(qb);
+          const int pos = qb + (int)rank;
+          if (queued) {
             if (pos < kQueueCapW) {
               s_q[wave][pos] = (uint16_t)((q << 6) | lane);
             } else {  // queue full: evaluate here
               const int64_t r = kb + base + q;
               const double y = ry[r], am = ra[r];
-  #pragma unroll 1
+#pragma unroll 1
               for (int j = 0; j < P; ++j) {
                 const double v =
                     am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - rc) * rsv, y);
-  #pragma unroll
+#pragma unroll
                 for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
               }
             }
