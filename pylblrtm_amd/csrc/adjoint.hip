@@ -38,7 +38,8 @@ constexpr int kVjpChunk = kBlock;
 constexpr int kVjpFold = 32;
 // Far-field series of the adjoint: terms of the one-node form's r = 1/(x^2 + c1)
 // and r^2 (whose double pole converges more slowly): outside the near range
-// 22 terms leave <= 1.1e-9 of r, 1.8e-7 of r^2 and 1.7e-8 of x r^2.
+// 22 terms leave <= 1.1e-9 of r and 1.8e-7 of r^2 (and of x r^2, formed as x
+// times the truncated r^2 series).
 constexpr int kAdjTerms = 22;
 constexpr int kAdjSliceExCap = 1024;
 

@@ -184,3 +184,55 @@ def test_four_node_pole_pairs_reproduce_the_form():
     for xm, y in [(40.0, 0.01), (36.0, 3.0), (200.0, 30.0)]:
         q0 = sum(series_rational(*gh4_pair_coeffs(xm, 1.0, y, p), 1)[0] for p in (0, 1))
         assert q0 == pytest.approx(gh4_direct(np.array([xm]), y)[0], rel=1e-14)
+
+
+# ---- the adjoint's series (csrc/adjoint.hip k_vjp_mixed): r = 1/(x^2 + c1)
+# and r^2 from b_n = -d a_n / d c1 = a_0 a_n + al b_(n-1) + be b_(n-2), and
+# x r^2 = (xm + sh tau) r^2, 22 terms; claims: <= 1.1e-9 of r and 1.8e-7 of
+# r^2 outside the near range where the one-node form holds, and x r^2 as the
+# kernel forms it (x times the truncated r^2) within the same 1.8e-7.
+TERMS_ADJ = 22
+
+
+def adjoint_coeffs(xm, sh, c1, terms=TERMS_ADJ):
+    d0 = xm * xm + c1
+    a0 = 1.0 / d0
+    al, be = -2.0 * xm * sh * a0, -sh * sh * a0
+    a = [a0, al * a0]
+    b = [a0 * a0, 2.0 * al * a0 * a0]
+    for n in range(2, terms):
+        a.append(al * a[-1] + be * a[-2])
+        b.append(a0 * a[-1] + al * b[-1] + be * b[-2])
+    return np.array(a), np.array(b)
+
+
+def test_adjoint_r2_recurrence_is_the_square():
+    # b_n against the Cauchy square of a_n (independent of the derivative identity)
+    for xm, sh, c1 in [(40.0, 12.5, -1.4), (700.0, 200.0, 0.3), (-90.0, 3.0, 50.0)]:
+        a, b = adjoint_coeffs(xm, sh, c1)
+        sq = np.array([sum(a[k] * a[n - k] for k in range(n + 1)) for n in range(TERMS_ADJ)])
+        assert np.allclose(b, sq, rtol=1e-12, atol=0.0)
+
+
+@pytest.mark.parametrize("y", [1e-4, 0.5, 1.2247, 1.3, 3.0, 10.0, 40.0, 150.0])
+def test_adjoint_series_22_terms(y):
+    y2 = y * y
+    c1 = y2 - 1.5
+    x_s1 = math.sqrt(max(s1_r2_min(y2) - y2, 0.0))
+    w1 = w2 = w3 = 0.0
+    for sh in np.geomspace(0.01, 3000.0, 41):
+        for xm0 in far_points(sh, x_s1):
+            for xm in (xm0, -xm0):
+                a, b = adjoint_coeffs(xm, sh, c1)
+                x = xm + sh * TAU
+                r = 1.0 / (x * x + c1)
+                # S3's form: xm S2 + sh sum_n b_n M_(n+1), i.e. x r^2 truncated like the kernel
+                bx = xm * b
+                bx[1:] += sh * b[:-1]
+                w1 = max(w1, float(np.max(np.abs(poly(a, TAU) / r - 1.0))))
+                w2 = max(w2, float(np.max(np.abs(poly(b, TAU) / (r * r) - 1.0))))
+                xr2 = poly(bx, TAU) + sh * b[-1] * TAU ** TERMS_ADJ
+                w3 = max(w3, float(np.max(np.abs(xr2 / (x * r * r) - 1.0))))
+    assert w1 <= 1.1e-9, w1
+    assert w2 <= 1.8e-7, w2
+    assert w3 <= 1.8e-7, w3
