@@ -1096,14 +1096,16 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           // positive bound) in a scalar register
           if (__builtin_amdgcn_readfirstlane(__float_as_int(c1)) <
               __float_as_int(kF32S2Y2 - 1.5f)) {
-            // second-order shifted form: K A = (A y/sqrt(pi)) r / (r^2 - 3/2 r + 2 y^2 - 3/2)
-            const float y2 = c1 + 1.5f, cc = fmaf(2.0f, c1, 1.5f);
+            // second-order shifted form: K A = (A y/sqrt(pi)) r / (r^2 - 3/2 r + 2 y^2 - 3/2),
+            // written about u = r - 3/4 = x^2 + y^2 - 3/4: the denominator is
+            // u^2 + 2 y^2 - 33/16 and the numerator ay u + 3/4 ay (6 VALU per point)
+            const float um = c1 + 0.75f, k2 = fmaf(2.0f, c1, 0.9375f), ay34 = 0.75f * aye;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
               const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xe);
-              const float r = fmaf(x, x, y2);
-              const float D = fmaf(r, r - 1.5f, cc);
-              part[j] = fmaf(aye * r, __builtin_amdgcn_rcpf(D), part[j]);
+              const float u = fmaf(x, x, um);
+              const float D = fmaf(u, u, k2);
+              part[j] = fmaf(fmaf(aye, u, ay34), __builtin_amdgcn_rcpf(D), part[j]);
             }
           } else {
             // two-node form: K A = (A y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
