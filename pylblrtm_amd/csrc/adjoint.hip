@@ -42,6 +42,7 @@ constexpr int kVjpFold = 32;
 // times the truncated r^2 series).
 constexpr int kAdjTerms = 22;
 constexpr int kAdjSliceExCap = 1024;
+constexpr int kAdjQueueCap = 256;   // near-centre (record, lane) pairs per wavefront per gas run
 
 struct GradArgs {
   lbl_lines_t lines;
@@ -216,6 +217,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
   __shared__ float s_msum[kNT + 1];          // slice moments
   __shared__ int32_t s_sx[kAdjSliceExCap];   // slice-level exceptions
   __shared__ int s_sxn;
+  __shared__ uint16_t s_aq[kWaves][kAdjQueueCap];   // near-centre queue: (staged record, lane)
+  __shared__ float s_at[kBlock], s_av[kBlock];       // queue results per owning thread
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
   const int k = (int)(lw / a.n_wg);
@@ -414,6 +417,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     }
   }
 
+  s_at[tid] = 0.0f;
+  s_av[tid] = 0.0f;
+  const bool live = i0 < t1;   // threads wholly past the tile end have WR = 0
   // ---- near field: the slice's near range and the slice level's exceptions
   // (the whole tile if their list overflowed), point by point, each record
   // skipped by the wavefronts whose series took it
@@ -421,6 +427,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
   auto stage_index = [&](int32_t v) -> int64_t {
     if (rescan) return (int64_t)rlo + v;
     return v < shi - slo ? (int64_t)slo + v : (int64_t)s_sx[v - (shi - slo)];
+  };
+  // the near-centre sums of staged record qi at the points of lane o of this
+  // wavefront (x exactly as that lane forms it), given that lane's WR
+  auto near_pair = [&](int qi, int o, const float (&wr_o)[P], float& acc_t, float& acc_v) {
+    const VjpFar& R = s_far[qi];
+    const VjpNear& N = s_near[qi];
+    const int64_t i0o = t0 + (int64_t)slice * (kBlock * P) + (int64_t)(wv * 64 + o) * P;
+    const double nu_mo = __dadd_rn(grid_nu(a.nu_min, a.step, i0o), 0.5 * (P - 1) * a.step);
+    const float xm = (float)(nu_mo - R.c) * R.s;
+    acc_t = 0.0f;
+    acc_v = 0.0f;
+#pragma unroll 1
+    for (int j = 0; j < P; ++j) {
+      const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+      float K, Kx, Ky;
+      kgrad_mixed(x, N.y, K, Kx, Ky);
+      const float AK = N.a * K;
+      Kx *= N.a;
+      Ky *= N.a;
+      float wr = 0.0f;
+#pragma unroll
+      for (int jj = 0; jj < P; ++jj) wr = (jj == j) ? wr_o[jj] : wr;
+      acc_t = fmaf(wr, N.at * AK + Kx * (ax_t * x) + Ky * N.dyt, acc_t);
+      acc_v = fmaf(wr, N.av * AK + Kx * N.bxv + Ky * N.dyv, acc_v);
+    }
   };
   for (int32_t base = 0; base < n_stage; base += kVjpChunk) {
     const int cnt = min(kVjpChunk, n_stage - base);
@@ -489,6 +520,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       const int q1 = q0 + __builtin_amdgcn_readfirstlane(len);
       float ft = 0.0f, fv = 0.0f;
       double dv = 0.0;
+      int naq = 0;   // queue entries of this run (uniform across the wavefront)
       for (int qi = q0; qi < q1; ++qi) {
         const VjpFar& R = s_far[qi];
         {
@@ -543,31 +575,60 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
           }
           ft += fmaf(R.tF, S1, fmaf(R.tX2, S4, fmaf(R.tD, S2, R.tQ * S5)));
           fv += fmaf(R.vF, S1, fmaf(R.vX, S3, fmaf(R.vD, S2, R.vQ * S5)));
-        } else {
-          const VjpNear& N = s_near[qi];
-          float acc_t = 0.0f, acc_v = 0.0f;
-#pragma unroll 1
-          for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
-            float K, Kx, Ky;
-            kgrad_mixed(x, N.y, K, Kx, Ky);
-            const float AK = N.a * K;
-            Kx *= N.a;
-            Ky *= N.a;
-            float wr = 0.0f;
-#pragma unroll
-            for (int jj = 0; jj < P; ++jj) wr = (jj == j) ? WR[jj] : wr;
-            acc_t = fmaf(wr, N.at * AK + Kx * (ax_t * x) + Ky * N.dyt, acc_t);
-            acc_v = fmaf(wr, N.av * AK + Kx * N.bxv + Ky * N.dyv, acc_v);
+        }
+        // near-centre pairs: appended to this wavefront's queue (one ballot)
+        // and drained densely after the run, instead of one lane's kgrad
+        // loop holding the whole wavefront
+        const bool nearc = live && !(fabsf(xm) >= R.hw1) && !(fabsf(xm) >= R.hw2);
+        const uint64_t msk = __ballot(nearc);
+        if (msk) {   // uniform
+          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+          const int pos = naq + (int)rank;
+          if (nearc) {
+            if (pos < kAdjQueueCap) {
+              s_aq[wv][pos] = (uint16_t)((qi << 6) | lane);
+            } else {   // queue full: evaluate here
+              float at_, av_;
+              near_pair(qi, lane, WR, at_, av_);
+              ft += at_;
+              fv += av_;
+            }
           }
-          ft += acc_t;
-          fv += acc_v;
+          naq += (int)__popcll(msk);
         }
         if (((qi - q0) & (kVjpFold - 1)) == kVjpFold - 1) {
           g_t += (double)ft;
           dv += (double)fv;
           ft = fv = 0.0f;
         }
+      }
+      naq = min(naq, kAdjQueueCap);
+      if (naq > 0) {   // uniform
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int e0 = 0; e0 < naq; e0 += 64) {   // every lane takes part (the owners' WR by shuffle)
+          const int e = e0 + lane;
+          const uint32_t ent = e < naq ? (uint32_t)s_aq[wv][e] : 0u;
+          const int o = (int)(ent & 63u);
+          float wro[P];
+#pragma unroll
+          for (int j = 0; j < P; ++j) wro[j] = __shfl(WR[j], o);
+          if (e < naq) {
+            float at_, av_;
+            near_pair((int)(ent >> 6), o, wro, at_, av_);
+            atomicAdd(&s_at[wv * 64 + o], at_);
+            atomicAdd(&s_av[wv * 64 + o], av_);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        ft += s_at[tid];
+        fv += s_av[tid];
+        s_at[tid] = 0.0f;
+        s_av[tid] = 0.0f;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
       }
       g_t += (double)ft;
       dv += (double)fv;
