@@ -67,12 +67,11 @@ def txt_to_tips(txt_path):
             parts = raw.split()
             if len(parts) < 2:
                 continue
-            try:
-                t, q = float(parts[0]), float(parts[1])
+            try:   # T first, then Q (hitran_manager.py:955-956): a bad Q leaves T appended
+                ts.append(float(parts[0]))
+                qs.append(float(parts[1]))
             except ValueError:
                 continue
-            ts.append(t)
-            qs.append(q)
     if not ts:
         raise NoLines(f"No data read from {txt_path}.")
     return np.asarray(ts, dtype=np.float64), np.asarray(qs, dtype=np.float64)

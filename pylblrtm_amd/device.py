@@ -90,8 +90,11 @@ def prepare_groups(nu: np.ndarray, sd, layers, cutoff: float, stacklevel: int = 
             raise KeyError(f"No TIPS data for (mol_id={m}, iso_id={i}). "
                            f"Available: {sorted(sd.tips.keys())}")
         t_nodes, q_nodes = sd.tips[key]
-        if len(t_nodes) == 0 or len(t_nodes) != len(q_nodes):
-            raise ValueError(f"TIPS table of (mol_id={m}, iso_id={i}) is empty or ragged")
+        # the reference's first use of the table, q_ref = Q(296) (forward.py:236,
+        # spectral_data.py:52), on the host: an empty table or one whose T and Q
+        # differ in length (txt_to_tips keeps a row's T when its Q fails) raises
+        # numpy's own ValueError here, as there; the kernels only see valid tables
+        np.interp(T_REF, np.asarray(t_nodes, np.float64), np.asarray(q_nodes, np.float64))
         groups.append(Group(int(m), int(i), gas, np.flatnonzero(sel), float(mm),
                             (np.asarray(t_nodes, np.float64), np.asarray(q_nodes, np.float64))))
     return groups

@@ -137,7 +137,12 @@ def _warn_malformed(par_path: Path, bad_lines) -> None:
 
 def txt_to_tips(txt_path) -> Tuple[np.ndarray, np.ndarray]:
     """Two-column TIPS table (T [K], Q(T)); blank lines, '#' comments and rows
-    that do not hold two numbers are skipped (hitran_manager.py:917-963)."""
+    with fewer than two fields are skipped (hitran_manager.py:917-963).
+
+    As in the reference loop (hitran_manager.py:950-958), T is appended
+    before Q is converted: a row whose first field converts and whose second
+    does not keeps its T, so the two arrays come back with different lengths
+    (and np.interp, hence ``SpectralData.Q``, rejects the table)."""
     txt_path = Path(txt_path)
     if not txt_path.exists():
         raise FileNotFoundError(f"TIPS file not found: {txt_path}")
@@ -151,11 +156,10 @@ def txt_to_tips(txt_path) -> Tuple[np.ndarray, np.ndarray]:
             if len(parts) < 2:
                 continue
             try:
-                t, q = float(parts[0]), float(parts[1])
+                t_list.append(float(parts[0]))
+                q_list.append(float(parts[1]))
             except ValueError:
                 continue
-            t_list.append(t)
-            q_list.append(q)
     if not t_list:
         raise HITRANDataError(f"No data read from {txt_path}.")
     return np.asarray(t_list, dtype=np.float64), np.asarray(q_list, dtype=np.float64)

@@ -190,3 +190,25 @@ def test_prelude_warnings_match_reference():
     assert sorted({str(x.message) for x in w}) == meta["warnings"]
     assert [(g.mol_id, g.iso_id) for g in groups] == [(1, 1), (2, 1), (2, 2), (3, 1)]
     assert [g.molar_mass for g in groups] == [18.01, 43.99, 28.97, 47.98]
+
+
+def test_prelude_rejects_tables_the_reference_rejects(tmp_path):
+    """A TIPS table whose T and Q differ in length (what txt_to_tips returns
+    for a row whose Q does not convert, hitran_manager.py:955-956) or that is
+    empty fails where the reference first interpolates it, q_ref = Q(296)
+    (forward.py:236): numpy's own ValueError, before any device work."""
+    from pylblrtm_amd import synth
+    from pylblrtm_amd.hitran_manager import txt_to_tips
+    sd, prof = synth.co2_window(n_lines=50, n_levels=4)
+    lay = prof.build_layers()
+    p = tmp_path / "tips.txt"
+    p.write_text("200 100\n296 x\n300 150\n", encoding="utf-8")
+    t, q = txt_to_tips(p)
+    assert (t.size, q.size) == (3, 2)
+    for bad in ((t, q), (np.zeros(0), np.zeros(0))):
+        sdx = SpectralData(lines=sd.lines, tips={(2, 1): bad}, molar_masses=sd.molar_masses)
+        with pytest.raises(ValueError) as exc:
+            prepare_groups(make_grid(700.0, 701.0, 0.001), sdx, lay, 25.0)
+        with pytest.raises(ValueError) as ref_exc:
+            np.interp(296.0, *bad)
+        assert str(exc.value) == str(ref_exc.value)

@@ -178,7 +178,16 @@ def test_txt_to_tips_matches_oracle(tmp_path):
     rt, rq = PO.txt_to_tips(p)
     np.testing.assert_array_equal(t, rt)
     np.testing.assert_array_equal(q, rq)
-    assert t.tolist() == [70.0, 71.0, 73.0, 74.0]
+    # hitran_manager.py:955-956 appends T before converting Q: "72 x" keeps
+    # its T and drops its Q ("bad 1" fails on T and keeps neither), so the
+    # reference returns arrays of different lengths
+    assert t.tolist() == [70.0, 71.0, 72.0, 73.0, 74.0]
+    assert q.tolist() == [12.5, 13.25, 14.0, 15.0]
+    clean = tmp_path / "clean.txt"
+    clean.write_text("70 1.5\n# c\n71 2.5e1\n", encoding="utf-8")
+    for f in (hm.txt_to_tips, PO.txt_to_tips):
+        ct, cq = f(clean)
+        assert ct.tolist() == [70.0, 71.0] and cq.tolist() == [1.5, 25.0]
     empty = tmp_path / "empty.txt"
     empty.write_text("# nothing\n")
     with pytest.raises(hm.HITRANDataError):
