@@ -122,6 +122,27 @@ int lbl_optical_depth(const lbl_lines_t *lines, const lbl_groups_t *groups,
                       const lbl_outputs_t *out, void *workspace, size_t workspace_bytes,
                       int32_t *err_layer, int32_t *err_group, void *stream);
 
+/* lbl_optical_depth without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * scratch (per-(layer, line) records, tile ranges, partial sums, error word)
+ * lives in a fixed device array of LBL_NOALLOC_SCRATCH_BYTES inside the
+ * library, one per GPU, held until the process ends; the call allocates and
+ * frees nothing.  Precision LBL_FP64 or LBL_MIXED; results, outputs and
+ * errors as lbl_optical_depth.  A problem whose lbl_workspace_bytes exceed
+ * the limit is rejected with LBL_EINVAL before any device work (the CO2
+ * window of 5,000 lines x 20 layers x 50,001 points fits; the five-gas
+ * workload does not).  The device current on the calling thread must be the
+ * stream's device.  Every call on one GPU uses that GPU's one array: calls
+ * that overlap on one GPU (other threads, other streams) race, and must be
+ * serialised by the caller.  Kernel timing (lbl_enable_timing) is not
+ * recorded by this entry point. */
+#define LBL_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int lbl_optical_depth_noalloc(const lbl_lines_t *lines, const lbl_groups_t *groups,
+                              const lbl_layers_t *layers, const lbl_grid_t *grid, int precision,
+                              const lbl_outputs_t *out, int32_t *err_layer, int32_t *err_group,
+                              void *stream);
+
 /* Network of the learned Voigt surrogate (see lbl_mlp_* below) and its gate:
  * a near-centre point uses the network iff |x| <= 50, -4 <= log10 y <= 2 and
  * A * nu_c * K_ref(x, y) <= gate (a bound on its optical-depth contribution

@@ -21,7 +21,7 @@ EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_l
             "lbl_enable_timing", "lbl_last_timings", "lbl_last_timings_split", "lbl_vjp_workspace_bytes",
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
             "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
-            "lbl_optical_depth_surrogate", "lbl_par_read")
+            "lbl_optical_depth_surrogate", "lbl_par_read", "lbl_optical_depth_noalloc")
 
 _vp = ctypes.c_void_p
 
@@ -82,6 +82,10 @@ def load() -> ctypes.CDLL:
                                       ctypes.c_int, P(LblOutputs), _vp, ctypes.c_size_t,
                                       P(ctypes.c_int32), P(ctypes.c_int32), _vp]
     lib.lbl_optical_depth.restype = ctypes.c_int
+    lib.lbl_optical_depth_noalloc.argtypes = [P(LblLines), P(LblGroups), P(LblLayers), P(LblGrid),
+                                              ctypes.c_int, P(LblOutputs), P(ctypes.c_int32),
+                                              P(ctypes.c_int32), _vp]
+    lib.lbl_optical_depth_noalloc.restype = ctypes.c_int
     lib.lbl_optical_depth_surrogate.argtypes = [P(LblLines), P(LblGroups), P(LblLayers),
                                                 P(LblGrid), P(LblOutputs), P(LblSurrogate), _vp,
                                                 ctypes.c_size_t, P(ctypes.c_int32),

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "internal.h"
 
@@ -43,6 +44,35 @@ int fail(int code, const char* fmt, ...) {
   vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
   va_end(ap);
   return code;
+}
+
+// Scratch of lbl_optical_depth_noalloc: a fixed device array in this code
+// object, one copy per GPU the library runs on, held until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is only
+// a stand-in symbol): its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = LBL_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on the
+// first call on that device, with that device current).
+int scratch_of_current_device(void** ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(LBL_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LBL_EHIP, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(LBL_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return LBL_OK;
 }
 
 }  // namespace
@@ -205,12 +235,10 @@ static hipError_t launch_near_ranges(const lbl_lines_t* lines, const lbl_grid_t*
   return lbl::launch_wave_ranges(wr, s);
 }
 
-static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* groups,
-                              const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
-                              const lbl_outputs_t* out, const lbl_surrogate_t* sur,
-                              void* workspace, size_t workspace_bytes, int32_t* err_layer,
-                              int32_t* err_group, void* stream) {
-  g_last_error[0] = '\0';
+// Argument checks shared by the optical-depth entry points (host only).
+static int check_call(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                      const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
+                      const lbl_outputs_t* out, const lbl_surrogate_t* sur) {
   int rc = check_inputs(lines, groups, layers, grid, precision);
   if (rc != LBL_OK) return rc;
   if (precision == LBL_SURROGATE && (!sur || !sur->params || !sur->wbf16 || !(sur->gate >= 0.0)))
@@ -221,6 +249,18 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   for (int m = 0; m < out->n_mu; ++m)
     if (!(out->mu[m] > 0.0 && out->mu[m] <= 1.0))
       return fail(LBL_EINVAL, "mu_view must be in (0, 1]");
+  return LBL_OK;
+}
+
+// The whole forward path on a workspace of at least make_layout's total
+// bytes (arguments already checked by check_call).  allow_timing: record
+// the per-kernel HIP events when lbl_enable_timing is on (their one-time
+// creation is left out of the entry point that must not allocate).
+static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                              const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
+                              const lbl_outputs_t* out, const lbl_surrogate_t* sur,
+                              void* workspace, size_t workspace_bytes, int32_t* err_layer,
+                              int32_t* err_group, void* stream, bool allow_timing) {
   Layout L;
   make_layout(lines, layers, grid, precision, &L);
   if (!workspace || workspace_bytes < L.total)
@@ -236,7 +276,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   int32_t* hi = reinterpret_cast<int32_t*>(ws + L.hi);
 
   hipEvent_t* ev = nullptr;
-  if (g_timing) HIP_TRY(timing_events(&ev));
+  if (g_timing && allow_timing) HIP_TRY(timing_events(&ev));
 
   HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
 
@@ -344,10 +384,36 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
                       const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
                       const lbl_outputs_t* out, void* workspace, size_t workspace_bytes,
                       int32_t* err_layer, int32_t* err_group, void* stream) {
+  g_last_error[0] = '\0';
   if (precision == LBL_SURROGATE)
     return fail(LBL_EINVAL, "precision LBL_SURROGATE needs lbl_optical_depth_surrogate");
+  int rc = check_call(lines, groups, layers, grid, precision, out, nullptr);
+  if (rc != LBL_OK) return rc;
   return optical_depth_impl(lines, groups, layers, grid, precision, out, nullptr, workspace,
-                            workspace_bytes, err_layer, err_group, stream);
+                            workspace_bytes, err_layer, err_group, stream, true);
+}
+
+int lbl_optical_depth_noalloc(const lbl_lines_t* lines, const lbl_groups_t* groups,
+                              const lbl_layers_t* layers, const lbl_grid_t* grid, int precision,
+                              const lbl_outputs_t* out, int32_t* err_layer, int32_t* err_group,
+                              void* stream) {
+  g_last_error[0] = '\0';
+  if (precision != LBL_FP64 && precision != LBL_MIXED)
+    return fail(LBL_EINVAL, "lbl_optical_depth_noalloc: precision must be LBL_FP64 or LBL_MIXED");
+  int rc = check_call(lines, groups, layers, grid, precision, out, nullptr);
+  if (rc != LBL_OK) return rc;
+  Layout L;
+  make_layout(lines, layers, grid, precision, &L);
+  if (L.total > kScratchBytes)
+    return fail(LBL_EINVAL,
+                "lbl_optical_depth_noalloc: this problem needs %zu bytes of scratch, the fixed "
+                "device array holds %zu; call lbl_optical_depth with a workspace instead",
+                L.total, kScratchBytes);
+  void* ptr = nullptr;
+  rc = scratch_of_current_device(&ptr);
+  if (rc != LBL_OK) return rc;
+  return optical_depth_impl(lines, groups, layers, grid, precision, out, nullptr, ptr,
+                            kScratchBytes, err_layer, err_group, stream, false);
 }
 
 int lbl_optical_depth_surrogate(const lbl_lines_t* lines, const lbl_groups_t* groups,
@@ -355,8 +421,11 @@ int lbl_optical_depth_surrogate(const lbl_lines_t* lines, const lbl_groups_t* gr
                                 const lbl_outputs_t* out, const lbl_surrogate_t* sur,
                                 void* workspace, size_t workspace_bytes, int32_t* err_layer,
                                 int32_t* err_group, void* stream) {
+  g_last_error[0] = '\0';
+  int rc = check_call(lines, groups, layers, grid, LBL_SURROGATE, out, sur);
+  if (rc != LBL_OK) return rc;
   return optical_depth_impl(lines, groups, layers, grid, LBL_SURROGATE, out, sur, workspace,
-                            workspace_bytes, err_layer, err_group, stream);
+                            workspace_bytes, err_layer, err_group, stream, true);
 }
 
 int lbl_vjp_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,

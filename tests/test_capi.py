@@ -64,3 +64,36 @@ def test_workspace_size_is_host_arithmetic():
     lib.lbl_workspace_bytes(ctypes.byref(lines), ctypes.byref(layers), ctypes.byref(grid), 0,
                             ctypes.byref(m))
     assert m.value >= 25_000_000 * 32          # FP64: four 8-byte arrays
+
+
+def test_noalloc_entry_point_rejects_oversize_before_device_work():
+    """lbl_optical_depth_noalloc checks the problem's scratch against its
+    fixed array on the host: the five-gas workload (~1.7 GB) is refused with
+    LBL_EINVAL, and no device call is made (none is possible here)."""
+    lib = _capi.load()
+    fake = ctypes.c_void_p(256)   # never dereferenced: the size check comes first
+    lines = _capi.LblLines(500_000, fake, fake, fake, fake, fake, fake, fake, fake)
+    groups = _capi.LblGroups(5, fake, fake, fake, fake, fake)
+    layers = _capi.LblLayers(50, 5, fake, fake, fake, fake)
+    grid = _capi.LblGrid(500.0, 0.001, 2_000_001, 2000, 25.0, 0.01)
+    mu = (ctypes.c_double * 1)(1.0)
+    out = _capi.LblOutputs(1, mu, fake, None, None)
+    for prec in (0, 1):
+        st = lib.lbl_optical_depth_noalloc(ctypes.byref(lines), ctypes.byref(groups),
+                                           ctypes.byref(layers), ctypes.byref(grid), prec,
+                                           ctypes.byref(out), None, None, None)
+        assert st == _capi.LBL_EINVAL
+        assert "scratch" in _capi.last_error() and "lbl_optical_depth" in _capi.last_error()
+    st = lib.lbl_optical_depth_noalloc(ctypes.byref(lines), ctypes.byref(groups),
+                                       ctypes.byref(layers), ctypes.byref(grid), 2,
+                                       ctypes.byref(out), None, None, None)
+    assert st == _capi.LBL_EINVAL and "precision" in _capi.last_error()
+    # the reference's own CPU case fits the array
+    n = ctypes.c_size_t(0)
+    small_lines = _capi.LblLines(5000, fake, fake, fake, fake, fake, fake, fake, fake)
+    small_layers = _capi.LblLayers(20, 1, fake, fake, fake, fake)
+    small_grid = _capi.LblGrid(700.0, 0.001, 50_001, 2000, 25.0, 0.01)
+    for prec in (0, 1):
+        assert lib.lbl_workspace_bytes(ctypes.byref(small_lines), ctypes.byref(small_layers),
+                                       ctypes.byref(small_grid), prec, ctypes.byref(n)) == 0
+        assert n.value <= 64 << 20
