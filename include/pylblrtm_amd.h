@@ -60,7 +60,10 @@ enum {
 /* Lines that survive the host prelude, SORTED BY nu0 ASCENDING.
  * Fields are those of SpectralData.lines (pylblrtm/spectral_data.py:21-26,
  * pylblrtm/hitran_manager.py:314-324); `group` replaces (mol_id, iso_id) by
- * the group's position in the host's sorted group list. */
+ * the group's position in the host's sorted group list.  Device-side indices
+ * are checked by the kernels: a group outside [0, n_groups), a gas row
+ * outside [0, n_gases) or an empty TIPS range makes the call return
+ * LBL_EINVAL (every kernel still exits normally). */
 typedef struct lbl_lines {
   int64_t n_lines;
   const double *nu0, *sref, *g_air, *g_self, *elow, *n_air, *shift;
@@ -96,6 +99,21 @@ typedef struct lbl_grid {
                                   bound of forward.py:190-191 (widens the candidate
                                   line range per tile; inclusion stays exact) */
 } lbl_grid_t;
+
+/* The grid of the seam's own argument: _build_optical_depth receives the
+ * wavenumber array nu itself (forward.py:157-163) and takes its tile size
+ * from nu[1] - nu[0] (forward.py:175-176).  Given that array (HOST memory,
+ * n_nu points), fills grid with nu_min = nu[0], n_nu, tile_samples =
+ * max(1, floor(tile_width_cm1 / (nu[1] - nu[0]) + 1e-8)) (step 0.001 when
+ * n_nu == 1, as the reference), cutoff, shift_bound = 0 (it depends on the
+ * lines; the caller sets it), and a step for which nu[0] + i*step (multiply,
+ * then add) equals every nu[i] bit for bit, so the kernels' grid is the
+ * caller's.  The arrays compute_transmittance and run_forward build
+ * (forward.py:141-142, config.py:246-247) always have one; an array that is
+ * not of that form (unevenly spaced, decreasing, non-finite) returns
+ * LBL_EINVAL.  Host code only. */
+int lbl_grid_from_nu(const double *nu, int64_t n_nu, double tile_width_cm1,
+                     double line_cutoff_cm1, lbl_grid_t *grid);
 
 /* Outputs.  `transmittance` is n_mu x n_nu; optional outputs may be NULL.
  * Element type: double for LBL_FP64, float otherwise. */
@@ -181,9 +199,15 @@ int lbl_voigt_eval(int64_t n, const void *x, const void *y, void *k_out, int pre
  *   grad_vmr[g][k]     = dL/dvmr_{k,g}      (double, n_gases x n_layers)
  * with the column of gas g in layer k taken as vmr_{k,g} * (air column of
  * layer k), i.e. lbl_layers.col must equal vmr * air.  The Voigt values are
- * recomputed (FP32), nothing of size lines x grid is stored.  A mixing ratio
- * that is exactly 0 gets gradient 0: the forward skips that gas in that
- * layer (forward.py:253-255), so its lines are not visited. */
+ * recomputed (FP32), nothing of size lines x grid is stored.  Where a
+ * mixing ratio is exactly 0 the forward skips that gas in that layer
+ * (forward.py:253-255) and this entry point leaves grad_vmr there at 0; the
+ * forward is continuous there (strength ~ vmr * air, forward.py:270-278), and
+ * the one-sided derivative is sum_nu W(nu) od_unit_k(nu), W = -sum_m G_m T_m
+ * / mu_m, od_unit the gas's layer optical depth from one lbl_optical_depth
+ * call with col = air_k for it and 0 for every other (gas, layer).  The
+ * package's host layer (pylblrtm_amd.device.OpticalDepthProblem.vjp, used by
+ * the autograd functions) adds exactly that. */
 int lbl_vjp_workspace_bytes(const lbl_lines_t *lines, const lbl_layers_t *layers,
                             const lbl_grid_t *grid, size_t *bytes);
 int lbl_transmittance_vjp(const lbl_lines_t *lines, const lbl_groups_t *groups,
@@ -243,8 +267,9 @@ int lbl_par_read(const char *path, int64_t capacity, int64_t *n_text_lines, int8
 void lbl_enable_timing(int on);
 int lbl_last_timings(double *ms_line_params, double *ms_voigt_sum);
 /* The line-sum time split (no counterpart in the reference; for
- * benchmarks): mixed modes run a far-field series kernel before the
- * near-field line sum; ms_far_series is 0 in FP64 mode. */
+ * benchmarks): every precision runs a far-field series kernel
+ * (k_far_series_*) before the near-field line sum (k_voigt_sum_*, with the
+ * layer-group epilogue); these are their durations in ms. */
 int lbl_last_timings_split(double *ms_far_series, double *ms_near_sum);
 
 /* Message of the last failure on the calling thread ("" if none). */

@@ -10,8 +10,11 @@ g in layer k being v_{k,g} * air_k.  Voigt derivatives use both parts of
 w = K + iL from scipy.special.wofz:
     dK/dx = -2 (x K - y L),   dK/dy = 2 (x L + y K) - 2/sqrt(pi).
 The partition sum is np.interp (piecewise linear), so d ln q/dT uses the
-slope of the segment holding T.  Pinned by central finite differences of the
-oracle's own forward (tests/test_adjoint_oracle.py).
+slope of the segment holding T.  At v_{k,g} = 0 (where the forward skips the
+gas) the one-sided derivative is the gas's layer optical depth with column
+air_k and no self-pressure.  Pinned by central finite differences of the
+oracle's own forward, and at v = 0 by one-sided ones
+(tests/test_adjoint_oracle.py).
 """
 
 from __future__ import annotations
@@ -61,9 +64,24 @@ def vjp(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus, G)
         for g in groups:
             gas = g["gas"]
             v = float(vmr[gas][k])
+            idx = g["idx"]
+            if v == 0.0 and float(air[k]) > 0.0:
+                # forward.py:253-255 skips the gas here, but tau is continuous in
+                # v (strength ~ col = v * air, forward.py:270-278) and every
+                # other v-dependence multiplies that strength: the one-sided
+                # derivative is the optical depth with col = air, P_self = 0
+                c, S, gl, ad = O.line_params(lines, idx, T, P, 0.0, float(air[k]),
+                                             g["q_ratio"][k], g["molar_mass"])
+                A1, s1, y1 = S * O.VOIGT_K / ad, sl / ad, sl * gl / ad
+                for t0, t1 in bounds:
+                    m = (c >= nu[t0] - cutoff) & (c <= nu[t1 - 1] + cutoff)
+                    if m.any():
+                        x = (nu[t0:t1, None] - c[m]) * s1[m]
+                        K = wofz(x + 1j * y1[m]).real
+                        gV[gas][k] += np.sum((W[t0:t1] * R[t0:t1])[:, None] * A1[m] * K)
+                continue
             if float(col[gas][k]) == 0.0 or v == 0.0:
                 continue
-            idx = g["idx"]
             c, S, gl, ad = O.line_params(lines, idx, T, P, v, float(col[gas][k]),
                                          g["q_ratio"][k], g["molar_mass"])
             nu0 = lines["nu0"][idx]

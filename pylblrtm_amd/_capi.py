@@ -21,7 +21,8 @@ EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_l
             "lbl_enable_timing", "lbl_last_timings", "lbl_last_timings_split", "lbl_vjp_workspace_bytes",
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
             "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
-            "lbl_optical_depth_surrogate", "lbl_par_read", "lbl_optical_depth_noalloc")
+            "lbl_optical_depth_surrogate", "lbl_par_read", "lbl_optical_depth_noalloc",
+            "lbl_grid_from_nu")
 
 _vp = ctypes.c_void_p
 
@@ -124,6 +125,9 @@ def load() -> ctypes.CDLL:
     lib.lbl_par_read.argtypes = [ctypes.c_char_p, ctypes.c_int64, P(ctypes.c_int64), _vp, _vp,
                                  _vp]
     lib.lbl_par_read.restype = ctypes.c_int
+    lib.lbl_grid_from_nu.argtypes = [_vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                     P(LblGrid)]
+    lib.lbl_grid_from_nu.restype = ctypes.c_int
     lib.lbl_last_error.argtypes = []
     lib.lbl_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -144,6 +148,18 @@ def raise_for(status: int, what: str) -> None:
     if status in (LBL_ESTRENGTH, LBL_ENONFINITE):
         raise FloatingPointError(msg)
     raise RuntimeError(msg)
+
+
+def grid_from_nu(nu, tile_width: float, cutoff: float) -> LblGrid:
+    """lbl_grid_from_nu on a host float64 array: the grid struct whose points
+    are exactly ``nu`` (ValueError if no evenly spaced grid reproduces it)."""
+    import numpy as np
+    a = np.ascontiguousarray(nu, dtype=np.float64)
+    g = LblGrid()
+    st = load().lbl_grid_from_nu(a.ctypes.data if a.size else None, int(a.size), float(tile_width),
+                                 float(cutoff), ctypes.byref(g))
+    raise_for(st, "lbl_grid_from_nu")
+    return g
 
 
 def enable_timing(on: bool = True) -> None:

@@ -13,7 +13,7 @@ the same layer averages:
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import Dict
+from typing import Dict, Optional
 
 import numpy as np
 
@@ -32,6 +32,10 @@ class _Layers:
     temperature_k: np.ndarray
     vmr: Dict[str, np.ndarray]
     column_cm2: Dict[str, np.ndarray]
+    # hydrostatic air column per layer [molec cm^-2] (column_cm2 = vmr * air_cm2);
+    # not a field of the reference's _Layers: the gradient at vmr = 0 needs it,
+    # since a zero mixing ratio leaves a zero column that does not carry it
+    air_cm2: Optional[np.ndarray] = None
 
     @property
     def n_layers(self) -> int:
@@ -101,7 +105,7 @@ class AtmosphericProfile:
         per_pa = AVOGADRO / (GRAVITY * M_DRY_AIR)
         air_cm2 = per_pa * (np.abs(np.diff(p)) * 100.0) / 1e4
         col = {g: v * air_cm2 for g, v in vmr_layer.items()}
-        return _Layers(p_layer, t_layer, vmr_layer, col)
+        return _Layers(p_layer, t_layer, vmr_layer, col, air_cm2)
 
     def __repr__(self) -> str:
         return (

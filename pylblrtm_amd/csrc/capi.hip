@@ -2,6 +2,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -179,6 +180,8 @@ int check_inputs(const lbl_lines_t* lines, const lbl_groups_t* groups, const lbl
     return fail(LBL_EINVAL, "null layer array");
   if (lines->n_lines > 0 && layers->n_layers > 0 && (!layers->vmr || !layers->col))
     return fail(LBL_EINVAL, "null vmr/col array");
+  if (lines->n_lines > 0 && layers->n_layers > 0 && layers->n_gases < 1)
+    return fail(LBL_EINVAL, "n_gases must be >= 1 when there are lines and layers");
   return LBL_OK;
 }
 
@@ -233,6 +236,19 @@ static hipError_t launch_near_ranges(const lbl_lines_t* lines, const lbl_grid_t*
   wr.slo = slo;
   wr.shi = shi;
   return lbl::launch_wave_ranges(wr, s);
+}
+
+// An index or table the kernels flagged as unusable (the error word's
+// lowest codes): LBL_EINVAL, nothing was computed from it.
+static int invalid_index(uint32_t code) {
+  switch (code) {
+    case lbl::kErrBadGroup:
+      return fail(LBL_EINVAL, "a line's group index is outside [0, n_groups)");
+    case lbl::kErrBadGas:
+      return fail(LBL_EINVAL, "a group's gas index is outside [0, n_gases)");
+    default:
+      return fail(LBL_EINVAL, "a group's TIPS table is empty (tips_offset not increasing)");
+  }
 }
 
 // Argument checks shared by the optical-depth entry points (host only).
@@ -367,8 +383,9 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     g_ms_far = f;
     g_ms_near = n;
   }
+  if (code != lbl::kErrNone && code < (1u << lbl::kErrGroupBits)) return invalid_index(code);
   if (code != lbl::kErrNone) {
-    const int kind2 = (int)(code >> lbl::kErrGroupBits);
+    const int kind2 = (int)(code >> lbl::kErrGroupBits) - 1;
     const int layer = kind2 >> 1;
     const int group = (int)(code & ((1u << lbl::kErrGroupBits) - 1));
     if (err_layer) *err_layer = layer;
@@ -469,6 +486,12 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   HIP_TRY(lbl::launch_fill_u32(err, 1, lbl::kErrNone, s));
   lbl::LineParamsArgs lp{*lines, *groups, *layers, rec, err, grid->step};
   HIP_TRY(lbl::launch_line_params(lp, LBL_MIXED, s));
+  // the gradient kernels index groups and gases directly: stop here if the
+  // per-(layer, line) pass found an index it could not use
+  uint32_t code = lbl::kErrNone;
+  HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (code != lbl::kErrNone && code < (1u << lbl::kErrGroupBits)) return invalid_index(code);
   lbl::TileRangeArgs tr{};
   tr.nu0 = lines->nu0;
   tr.n_lines = lines->n_lines;
@@ -494,12 +517,11 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
                           layers->n_gases,
                           *grid, L.n_tiles, n_mu, mu, transmittance, grad_transmittance,
                           optical_depth_layers, grad_t, grad_vmr, s));
-  uint32_t code = lbl::kErrNone;
   HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (code != lbl::kErrNone)
     return fail(LBL_ESTRENGTH, "vjp: non-finite or negative line strength (layer %d)",
-                (int)(code >> lbl::kErrGroupBits) >> 1);
+                ((int)(code >> lbl::kErrGroupBits) - 1) >> 1);
   return LBL_OK;
 }
 
@@ -568,6 +590,80 @@ int lbl_mlp_eval(int64_t n, const float* x, const float* log10y, const float* pa
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(lbl::launch_mlp_eval(n, x, log10y, params, wbf16, t_out, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return LBL_OK;
+}
+
+// ---------------------------------------------------------------- grid --
+// nu_min + i * step rounded after the multiply and after the add, as numpy
+// forms nu_min + np.arange(n) * step (forward.py:141-142).
+static double affine_point(double nu_min, double step, int64_t i) {
+#pragma clang fp contract(off)
+  const double m = (double)i * step;
+  return nu_min + m;
+}
+
+// Does nu_min + i * step reproduce every nu[i] bit for bit?  The last and
+// middle points first (a wrong step fails there), then all of them.
+static bool reproduces(const double* nu, int64_t n, double step) {
+  if (!(step > 0.0) || !isfinite(step)) return false;
+  const double a = nu[0];
+  if (affine_point(a, step, n - 1) != nu[n - 1] || affine_point(a, step, n / 2) != nu[n / 2])
+    return false;
+  for (int64_t i = 1; i < n; ++i)
+    if (affine_point(a, step, i) != nu[i]) return false;
+  return true;
+}
+
+// Steps that may have built nu: the decimal roundings (1 to 17 significant
+// digits) of the end-to-end step and of nu[1] - nu[0] (a caller's step is
+// usually a short decimal such as 0.001 or 5e-4), then the end-to-end step
+// and its neighbours up to 64 ulps away.
+static bool find_step(const double* nu, int64_t n, double* step) {
+  const double s_end = (nu[n - 1] - nu[0]) / (double)(n - 1);
+  const double s_one = nu[1] - nu[0];
+  const double bases[2] = {s_end, s_one};
+  char buf[64];
+  for (double b : bases) {
+    if (!(b > 0.0) || !isfinite(b)) continue;
+    for (int digits = 1; digits <= 17; ++digits) {
+      snprintf(buf, sizeof(buf), "%.*g", digits, b);
+      const double c = strtod(buf, nullptr);
+      if (reproduces(nu, n, c)) { *step = c; return true; }
+    }
+  }
+  if (!(s_end > 0.0) || !isfinite(s_end)) return false;
+  double up = s_end, down = s_end;
+  for (int k = 0; k <= 64; ++k) {
+    if (reproduces(nu, n, up)) { *step = up; return true; }
+    if (k && reproduces(nu, n, down)) { *step = down; return true; }
+    up = nextafter(up, INFINITY);
+    down = nextafter(down, 0.0);
+  }
+  return false;
+}
+
+int lbl_grid_from_nu(const double* nu, int64_t n_nu, double tile_width_cm1,
+                     double line_cutoff_cm1, lbl_grid_t* grid) {
+  g_last_error[0] = '\0';
+  if (!nu || !grid || n_nu < 1) return fail(LBL_EINVAL, "empty grid");
+  if (!isfinite(tile_width_cm1) || !(tile_width_cm1 > 0.0) || !isfinite(line_cutoff_cm1))
+    return fail(LBL_EINVAL, "invalid tile width or cutoff");
+  for (int64_t i = 0; i < n_nu; ++i)
+    if (!isfinite(nu[i])) return fail(LBL_EINVAL, "nu[%lld] is not finite", (long long)i);
+  // forward.py:175-176: the tile size follows from the first two points
+  const double ref_step = n_nu > 1 ? nu[1] - nu[0] : 0.001;
+  const double ts = floor(tile_width_cm1 / ref_step + 1e-8);
+  double step = 0.001;
+  if (n_nu > 1 && !find_step(nu, n_nu, &step))
+    return fail(LBL_EINVAL,
+                "nu is not nu[0] + i*step for any step (the line sum needs "
+                "the evenly spaced grid compute_transmittance and run_forward build)");
+  grid->nu_min = nu[0];
+  grid->step = step;
+  grid->n_nu = n_nu;
+  grid->tile_samples = ts >= 1.0 ? (int64_t)ts : 1;
+  grid->cutoff = line_cutoff_cm1;
+  grid->shift_bound = 0.0;
   return LBL_OK;
 }
 

@@ -22,10 +22,20 @@ constexpr int kMaxGroups = 1024;   // groups whose q-ratios fit the LDS table
 constexpr int kMaxMu = 8;          // viewing cosines per call
 constexpr int kBlock = 256;        // threads per workgroup of the line sum
 
-// Error word: ((2*layer + kind) << 11) | group, kind 0 = strength, 1 = od.
-// atomicMin over it yields the reference's first failure in loop order.
+// Error word, written with atomicMin so the host reads the first failure:
+//   kErrBadGroup / kErrBadGas / kErrBadTips: an index or table of the inputs
+//     that the kernels cannot use (reported before any numerical failure);
+//   ((2*layer + kind + 1) << 11) | group, kind 0 = strength, 1 = od: the
+//     reference's first numerical failure in its loop order (layer, then
+//     strength before od, then group).
 constexpr uint32_t kErrNone = 0xFFFFFFFFu;
 constexpr int kErrGroupBits = 11;
+constexpr uint32_t kErrBadGroup = 1;   // a line's group outside [0, n_groups)
+constexpr uint32_t kErrBadGas = 2;     // a group's gas row outside [0, n_gases)
+constexpr uint32_t kErrBadTips = 3;    // a group's TIPS table empty (offsets not increasing)
+__host__ __device__ __forceinline__ uint32_t err_numeric(int layer, int kind, int group) {
+  return ((uint32_t)(2 * layer + kind + 1) << kErrGroupBits) | (uint32_t)group;
+}
 
 // Hardware assigns workgroup b to XCD b % 8.  Give every XCD a contiguous
 // run of logical workgroups so neighbouring tiles (which share most of their

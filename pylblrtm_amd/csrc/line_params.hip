@@ -97,9 +97,18 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
   const double T = a.layers.t_k[k];
   const double P = a.layers.p_atm[k];
   // Q(T_ref) / Q(T_k) per group (forward.py:236-240), once per workgroup.
+  // (an empty table, or a group whose gas row does not exist, is flagged and
+  // never read: the host then returns LBL_EINVAL)
   for (int g = threadIdx.x; g < a.groups.n_groups; g += blockDim.x) {
     const int o = a.groups.tips_offset[g];
     const int n = a.groups.tips_offset[g + 1] - o;
+    const int gas = a.groups.gas[g];
+    if (gas < 0 || gas >= a.layers.n_gases) atomicMin(a.err, kErrBadGas);
+    if (o < 0 || n < 1) {
+      atomicMin(a.err, kErrBadTips);
+      s_q[g] = __builtin_nan("");
+      continue;
+    }
     const double* tt = a.groups.tips_t + o;
     const double* qq = a.groups.tips_q + o;
     s_q[g] = interp_like_numpy(kTRef, tt, qq, n) / interp_like_numpy(T, tt, qq, n);
@@ -110,9 +119,11 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
   if (l >= a.lines.n_lines) return;
   const int64_t out = (int64_t)k * a.lines.n_lines + l;
   const int g = a.lines.group[l];
-  const int gas = a.groups.gas[g];
-  const double vmr = a.layers.vmr[(int64_t)gas * a.layers.n_layers + k];
-  const double col = a.layers.col[(int64_t)gas * a.layers.n_layers + k];
+  const int gas = (g >= 0 && g < a.groups.n_groups) ? a.groups.gas[g] : -1;
+  const bool valid = gas >= 0 && gas < a.layers.n_gases;
+  if (!valid) atomicMin(a.err, (g >= 0 && g < a.groups.n_groups) ? kErrBadGas : kErrBadGroup);
+  const double vmr = valid ? a.layers.vmr[(int64_t)gas * a.layers.n_layers + k] : 0.0;
+  const double col = valid ? a.layers.col[(int64_t)gas * a.layers.n_layers + k] : 0.0;
   if (col == 0.0) {
     // forward.py:253-255: the group is skipped for this layer.  A NaN centre
     // fails every tile's inclusion test, so the record is never summed.
@@ -134,7 +145,7 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
     const double strength = a.lines.sref[l] * s_q[g] * boltz * (-expm1(-kC2 * nu0 / T)) /
                             (-expm1(-kC2 * nu0 / kTRef)) * col / radiation64(c, T);
     if (!(isfinite(strength) && strength >= 0.0)) {
-      atomicMin(a.err, ((uint32_t)(2 * k) << kErrGroupBits) | (uint32_t)g);
+      atomicMin(a.err, err_numeric(k, 0, g));
     }
     const double lorentz =
         pow(kTRef / T, a.lines.n_air[l]) * (a.lines.g_air[l] * p_air + a.lines.g_self[l] * p_self);
@@ -149,7 +160,7 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
                            (-expm1f(-c2 * nu0f / Tf)) / (-expm1f(-c2 * nu0f / (float)kTRef)) *
                            (float)col / radiation32((float)c, Tf);
     if (!(isfinite(strength) && strength >= 0.0f)) {
-      atomicMin(a.err, ((uint32_t)(2 * k) << kErrGroupBits) | (uint32_t)g);
+      atomicMin(a.err, err_numeric(k, 0, g));
     }
     const float lorentz = powf((float)kTRef / Tf, (float)a.lines.n_air[l]) *
                           ((float)a.lines.g_air[l] * (float)p_air +

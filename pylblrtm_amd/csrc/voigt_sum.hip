@@ -57,7 +57,7 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
 
 
 __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
-  atomicMin(err, (uint32_t)(2 * k + 1) << kErrGroupBits);
+  atomicMin(err, err_numeric(k, 1, 0));
 }
 
 // ------------------------------------------------------------------ FP64 --

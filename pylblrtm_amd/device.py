@@ -111,15 +111,28 @@ def _torch():
 class OpticalDepthProblem:
     """Inputs of the device path, uploaded once."""
 
-    def __init__(self, sd, layers, nu_min: float, nu_max: float, step: float, cutoff: float,
-                 tile_width: float, device=None, stacklevel: int = 4):
+    def __init__(self, sd, layers, nu_min: Optional[float], nu_max: Optional[float],
+                 step: Optional[float], cutoff: float, tile_width: float, device=None,
+                 stacklevel: int = 4, nu: Optional[np.ndarray] = None):
+        """The grid is either (nu_min, nu_max, step), built as
+        compute_transmittance builds it (forward.py:141-142), or ``nu``: the
+        wavenumber array the seam _build_optical_depth receives
+        (forward.py:157-176), used as given (its size, its tile size from
+        nu[1] - nu[0], and a step that reproduces its points bit for bit;
+        lbl_grid_from_nu); nu_min, nu_max and step are then ignored."""
         torch = _torch()
         self.lib = _capi.load()
         self.device = torch.device(device if device is not None else f"cuda:{torch.cuda.current_device()}")
-        self.nu = make_grid(nu_min, nu_max, step)
-        self.nu_min, self.step = float(nu_min), float(step)
+        if nu is not None:
+            self.nu = np.array(nu, dtype=np.float64, copy=True).reshape(-1)
+            g = _capi.grid_from_nu(self.nu, tile_width, cutoff)
+            self.nu_min, self.step = float(g.nu_min), float(g.step)
+            self.tile_samples = int(g.tile_samples)
+        else:
+            self.nu = make_grid(nu_min, nu_max, step)
+            self.nu_min, self.step = float(nu_min), float(step)
+            self.tile_samples = tile_samples_for(self.nu, tile_width)
         self.n_nu = self.nu.size
-        self.tile_samples = tile_samples_for(self.nu, tile_width)
         self.cutoff = float(cutoff)
         self.n_layers = layers.n_layers
         self.groups = prepare_groups(self.nu, sd, layers, cutoff, stacklevel=stacklevel)
@@ -179,14 +192,23 @@ class OpticalDepthProblem:
         for j, gname in enumerate(gases):
             vmr[j, : self.n_layers] = layers.vmr[gname]
             col[j, : self.n_layers] = layers.column_cm2[gname]
-        # air column per layer (col = vmr * air, atmosphere.py:133-139), recovered
-        # from the gases' columns; used when layer mixing ratios are changed.
-        num = np.zeros(nl)
-        den = np.zeros(nl)
-        for gname in gases:
-            num[: self.n_layers] += layers.column_cm2[gname]
-            den[: self.n_layers] += layers.vmr[gname]
-        self.air_cm2 = np.where(den > 0, num / np.where(den > 0, den, 1.0), 0.0)[: self.n_layers]
+        # air column per layer (col = vmr * air, atmosphere.py:133-139); used when
+        # layer mixing ratios are changed and for the gradient at vmr = 0.  This
+        # package's layers carry it; for others (the reference's _Layers) it is
+        # recovered from the gases' columns, and stays NaN in a layer where
+        # every mixing ratio is 0 (that layer's columns cannot be changed then)
+        air = getattr(layers, "air_cm2", None)
+        if air is not None:
+            self.air_cm2 = np.asarray(air, np.float64).reshape(-1)[: self.n_layers].copy()
+        else:
+            num = np.zeros(nl)
+            den = np.zeros(nl)
+            for gname in layers.vmr:
+                if gname in layers.column_cm2:
+                    num[: self.n_layers] += layers.column_cm2[gname]
+                    den[: self.n_layers] += layers.vmr[gname]
+            self.air_cm2 = np.where(den > 0, num / np.where(den > 0, den, 1.0),
+                                    np.nan)[: self.n_layers]
         self._t["p_atm"] = dev(layers.pressure_atm if self.n_layers else np.zeros(1), f8)
         self._t["t_k"] = dev(layers.temperature_k if self.n_layers else np.ones(1), f8)
         self._t["vmr"] = dev(vmr, f8)
@@ -223,7 +245,12 @@ class OpticalDepthProblem:
         if t.numel() != self.n_layers or tuple(v.shape) != (len(self.gases), self.n_layers):
             raise ValueError(f"expected temperature [{self.n_layers}] and vmr "
                              f"[{len(self.gases)}, {self.n_layers}]")
-        air = torch.as_tensor(self.air_cm2, dtype=torch.float64, device=self.device)
+        unknown = ~np.isfinite(self.air_cm2)
+        if unknown.any() and bool((v[:, torch.as_tensor(unknown, device=self.device)] != 0).any()):
+            raise ValueError("a layer whose every mixing ratio was 0 has no known air column; "
+                             "build the layers with AtmosphericProfile.build_layers")
+        air = torch.as_tensor(np.nan_to_num(self.air_cm2, nan=0.0), dtype=torch.float64,
+                              device=self.device)
         self._t["t_k"][: self.n_layers].copy_(t)
         self._t["vmr"][: len(self.gases), : self.n_layers].copy_(v)
         self._t["col"][: len(self.gases), : self.n_layers].copy_(v * air[None, :])
@@ -268,7 +295,47 @@ class OpticalDepthProblem:
                 G.data_ptr(), gt.data_ptr(), gv.data_ptr(), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
                 ctypes.c_void_p(stream))
         _capi.raise_for(st, "lbl_transmittance_vjp")
-        return gt[: self.n_layers], gv[: len(self.gases), : self.n_layers]
+        gt, gv = gt[: self.n_layers], gv[: len(self.gases), : self.n_layers]
+        self._vjp_at_zero_vmr(T, G, mus, gv)
+        return gt, gv
+
+    def _vjp_at_zero_vmr(self, T, G, mus: Sequence[float], gv) -> None:
+        """dL/dvmr_{k,g} where vmr_{k,g} = 0 (into ``gv``).
+
+        The forward skips such a gas in that layer (col = 0, forward.py:253-255)
+        but is continuous there: the strength is proportional to col =
+        vmr * air (forward.py:270-278), and every other factor of its optical
+        depth is multiplied by it.  So the one-sided derivative at vmr = 0 is
+        d tau(nu)/d vmr = od_unit_k(nu), the layer optical depth of that gas
+        with col = air_k and no self-broadening or self-shift (P_self = 0),
+        and dL/dvmr = sum_nu W(nu) od_unit_k(nu), W = -sum_m G_m T_m / mu_m.
+        One mixed forward per gas that has such layers, on the device."""
+        torch = _torch()
+        ng, nl = len(self.gases), self.n_layers
+        if ng == 0 or nl == 0:
+            return
+        vmr = self._t["vmr"][:ng, :nl]
+        air = torch.as_tensor(self.air_cm2, dtype=torch.float64, device=self.device)
+        zero = (vmr == 0) & torch.isfinite(air)[None, :] & (air > 0)[None, :]
+        if not bool(zero.any()):
+            return
+        inv_mu = torch.tensor([1.0 / float(m) for m in mus], dtype=torch.float64,
+                              device=self.device)
+        W = -(G.double() * T.double() * inv_mu[:, None]).sum(0)
+        col = self._t["col"][:ng, :nl]
+        keep = col.clone()
+        try:
+            for g in range(ng):
+                if not bool(zero[g].any()):
+                    continue
+                unit = torch.zeros_like(keep)
+                unit[g] = torch.where(zero[g], air, torch.zeros_like(air))
+                col.copy_(unit)
+                od = self.run("mixed", mus=(1.0,), want_layers=True)["optical_depth_layers"]
+                d = od.double() @ W
+                gv[g] = torch.where(zero[g], d, gv[g])
+        finally:
+            col.copy_(keep)
 
     # ------------------------------------------------------------ work count --
     def tile_edges(self):

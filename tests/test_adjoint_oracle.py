@@ -54,3 +54,39 @@ def test_adjoint_oracle_matches_finite_differences():
             vm[gas][k] -= hv
             fdv = (loss(t, vp) - loss(t, vm)) / (2 * hv)
             assert gV[gas][k] == pytest.approx(fdv, rel=2e-6, abs=1e-12), (gas, k, gV[gas][k], fdv)
+
+
+def test_adjoint_oracle_at_zero_mixing_ratio():
+    """At vmr = 0 the forward skips the gas (forward.py:253-255) but is
+    continuous there; the oracle's gradient is the one-sided derivative.
+    Pinned by one-sided differences at h and h/2 (Richardson: 2 D(h/2) - D(h)
+    removes the O(h) term), and by continuity with the gradient at a tiny
+    positive vmr."""
+    nu, lines, tips, mm, p, t, vmr, air = translucent_case()
+    mus = (1.0,)
+    vmr0 = {g: v.copy() for g, v in vmr.items()}
+    vmr0["H2O"][1] = 0.0
+    vmr0["CO2"][3] = 0.0
+    args = (nu, lines, tips, mm, p, t, vmr0, air, 25.0, 0.5, mus)
+    T0 = A.forward(*args)
+    G = np.random.default_rng(9).normal(size=T0.shape)
+    gT, gV = A.vjp(*args, G)
+
+    def loss(vv):
+        return float(np.sum(G * A.forward(nu, lines, tips, mm, p, t, vv, air, 25.0, 0.5, mus)))
+
+    L0 = loss(vmr0)
+    for gas, k, h in (("H2O", 1, 2e-6), ("CO2", 3, 2e-6)):
+        assert gV[gas][k] != 0.0
+        d = []
+        for hh in (h, h / 2):
+            vp = {g: v.copy() for g, v in vmr0.items()}
+            vp[gas][k] = hh
+            d.append((loss(vp) - L0) / hh)
+        fd = 2 * d[1] - d[0]
+        assert gV[gas][k] == pytest.approx(fd, rel=1e-5), (gas, k, gV[gas][k], fd, d)
+        # continuity: the analytic gradient at vmr = 1e-12 (the general path)
+        vt = {g: v.copy() for g, v in vmr0.items()}
+        vt[gas][k] = 1e-12
+        _, gVt = A.vjp(nu, lines, tips, mm, p, t, vt, air, 25.0, 0.5, mus, G)
+        assert gV[gas][k] == pytest.approx(gVt[gas][k], rel=1e-7)

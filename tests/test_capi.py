@@ -2,6 +2,7 @@
 argument validation happens before any device call (CPU only)."""
 
 import ctypes
+import json
 import re
 from pathlib import Path
 
@@ -97,3 +98,43 @@ def test_noalloc_entry_point_rejects_oversize_before_device_work():
         assert lib.lbl_workspace_bytes(ctypes.byref(small_lines), ctypes.byref(small_layers),
                                        ctypes.byref(small_grid), prec, ctypes.byref(n)) == 0
         assert n.value <= 64 << 20
+
+
+def test_grid_from_nu_reproduces_the_seams_grid():
+    """lbl_grid_from_nu (host code) on the arrays the seam receives: the
+    golden fixtures' grids (written by the reference) and the workload grids.
+    Its points are the caller's bit for bit, its size is nu.size and its tile
+    size follows forward.py:175-176 (the tile_samples table of the oracle's
+    tests: 1999 at 2400 cm-1, 3999 at step 5e-4 from 1200 cm-1)."""
+    import numpy as np
+    from pylblrtm_amd.device import make_grid
+    arrays = []
+    for name in ("co2_mini", "three_gas_edge", "tile1"):
+        z = np.load(ROOT / "tests" / "golden" / f"{name}.npz")
+        arrays.append((z["out_nu"], json.loads(str(z["meta_json"]))["tile"]))
+    for lo, hi, st in [(700.0, 750.0, 0.001), (500.0, 2500.0, 0.001), (500.0, 2500.0, 5e-4),
+                       (1000.0, 1100.0, 1e-4), (2400.0, 2401.0, 0.001), (1200.0, 1201.0, 5e-4),
+                       (700.0, 700.0, 0.001), (0.1, 1.0, 0.003)]:
+        arrays.append((make_grid(lo, hi, st), 2.0))
+    for nu, tile in arrays:
+        g = _capi.grid_from_nu(nu, tile, 25.0)
+        assert g.n_nu == nu.size and g.nu_min == nu[0] and g.cutoff == 25.0
+        rebuilt = g.nu_min + np.arange(g.n_nu, dtype=np.float64) * g.step
+        np.testing.assert_array_equal(rebuilt, nu)
+        ref_step = float(nu[1] - nu[0]) if nu.size > 1 else 0.001
+        assert g.tile_samples == max(1, int(np.floor(tile / ref_step + 1e-8)))
+    assert _capi.grid_from_nu(make_grid(2400.0, 2401.0, 0.001), 2.0, 25.0).tile_samples == 1999
+    assert _capi.grid_from_nu(make_grid(1200.0, 1201.0, 5e-4), 2.0, 25.0).tile_samples == 3999
+    # forward.py:141-142's own count: the seam's nu.size, not a re-derived one
+    nu = make_grid(700.0, 750.0, 0.001)
+    assert nu.size == 50_001 and _capi.grid_from_nu(nu, 2.0, 25.0).n_nu == 50_001
+
+
+def test_grid_from_nu_rejects_what_the_kernels_cannot_grid():
+    import numpy as np
+    for bad in (np.array([1.0, 2.0, 4.0]), np.array([3.0, 2.0, 1.0]), np.array([1.0, np.nan]),
+                np.zeros(0), 700.0 + np.sort(np.random.default_rng(0).uniform(0, 1, 50))):
+        with pytest.raises(ValueError):
+            _capi.grid_from_nu(bad, 2.0, 25.0)
+    with pytest.raises(ValueError):
+        _capi.grid_from_nu(np.array([700.0, 700.001]), 0.0, 25.0)

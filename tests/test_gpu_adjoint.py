@@ -28,7 +28,7 @@ def _problem(t_override=None):
     if t_override is not None:
         t = t_override
     sd = SpectralData(lines=lines, tips=tips, molar_masses=mm)
-    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()})
+    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()}, air)
     prob = OpticalDepthProblem(sd, lay, 700.0, 702.0, 0.004, 25.0, 0.5)
     return prob, (nu, lines, tips, mm, p, t, vmr, air)
 
@@ -100,6 +100,56 @@ def test_vjp_matches_central_differences_of_fp64_forward():
     say(f"  rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
     assert et < GRAD_RTOL
     assert ev < GRAD_RTOL
+
+
+def test_vjp_at_zero_mixing_ratio():
+    """A retrieval that starts a gas from nothing: at vmr = 0 the forward
+    skips the gas in that layer (forward.py:253-255) but is continuous there
+    (strength ~ vmr * air, forward.py:270-278), so dL/dvmr is the one-sided
+    derivative, not 0.  Against the FP64 adjoint oracle (pinned by one-sided
+    differences on the CPU) and one-sided differences of the FP64 GPU
+    forward (Richardson over h and h/2)."""
+    say("case: vjp at vmr = 0, translucent 2-gas 4-layer")
+    import torch
+    prob, (nu, lines, tips, mm, p, t, vmr, air) = _problem()
+    mus = (1.0, 0.5)
+    vmr0 = {g: v.copy() for g, v in vmr.items()}
+    vmr0["H2O"][1] = 0.0
+    vmr0["CO2"][3] = 0.0
+    V0 = _vmr_matrix(prob, vmr0)
+    prob.set_layers(t, V0)
+    fw = prob.run("mixed", mus=mus, want_layers=True)
+    T = fw["transmittance"]
+    G = np.random.default_rng(21).normal(size=tuple(T.shape))
+    gt, gv = prob.vjp(T, fw["optical_depth_layers"],
+                      torch.as_tensor(G, dtype=torch.float32, device=T.device), mus)
+    gv = gv.cpu().numpy()
+    rt, rv = A.vjp(nu, lines, tips, mm, p, t, vmr0, air, 25.0, 0.5, mus, G)
+    ref_v = np.stack([rv[g] for g in prob.gases])
+    et, ev = _rel(gt.cpu().numpy(), rt), _rel(gv, ref_v)
+    say(f"  rel error vs oracle: temperature {et:.2e}, mixing ratio {ev:.2e}")
+    assert et < GRAD_RTOL and ev < GRAD_RTOL
+
+    def loss(vv):
+        prob.set_layers(t, vv)
+        return float(np.sum(G * prob.run("fp64", mus=mus)["transmittance"].cpu().numpy()))
+
+    L0 = loss(V0)
+    for gas, k in (("H2O", 1), ("CO2", 3)):
+        g = prob.gases.index(gas)
+        d = []
+        for h in (2e-6, 1e-6):
+            vp = V0.copy()
+            vp[g, k] = h
+            d.append((loss(vp) - L0) / h)
+        fd = 2 * d[1] - d[0]
+        rel_o = abs(gv[g, k] - ref_v[g, k]) / abs(ref_v[g, k])
+        rel_f = abs(gv[g, k] - fd) / abs(fd)
+        say(f"  d/dvmr[{gas}][{k}] at 0: gpu {gv[g, k]:.6e}, oracle {ref_v[g, k]:.6e} ({rel_o:.1e}), "
+            f"one-sided fd {fd:.6e} ({rel_f:.1e})")
+        assert gv[g, k] != 0.0
+        assert rel_o < GRAD_RTOL and rel_f < 1e-4
+    prob.set_layers(t, _vmr_matrix(prob, vmr))
 
 
 def test_autograd_function_and_batched_profiles():
@@ -253,7 +303,7 @@ def test_vjp_several_wavefronts_per_slice():
     nu, lines, tips, mm, p, t, vmr, air = translucent_case()
     nu = A.O.grid(700.0, 702.0, 0.0005)
     sd = SpectralData(lines=lines, tips=tips, molar_masses=mm)
-    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()})
+    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()}, air)
     prob = OpticalDepthProblem(sd, lay, 700.0, 702.0, 0.0005, 25.0, 2.0)
     et, ev = _vjp_vs_oracle(prob, (nu, lines, tips, mm, p, t, vmr, air), (1.0, 0.5), 21, 2.0)
     say(f"  rel error: temperature {et:.2e}, mixing ratio {ev:.2e}")
@@ -310,7 +360,7 @@ def test_slice_exception_overflow_forward_and_vjp():
     nu, lines, tips, mm, p, t, vmr, air = case
     assert min(_slice_exceptions(case, 3)) > 1024   # the rescan path is taken
     sd = SpectralData(lines=lines, tips=tips, molar_masses=mm)
-    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()})
+    lay = _Layers(p, t, dict(vmr), {g: v * air for g, v in vmr.items()}, air)
     prob = OpticalDepthProblem(sd, lay, 2499.7, 2500.3, 1e-4, 25.0, 2.0)
     ref = A.forward(nu, lines, tips, mm, p, t, vmr, air, 25.0, 2.0, (1.0,))[0]
     T64 = prob.run("fp64", mus=(1.0,))["transmittance"][0].cpu().numpy()

@@ -104,6 +104,25 @@ def test_golden_fp64(name):
         np.testing.assert_allclose(res.transmittance, z[f"out_T_mu{j}"], rtol=1e-8, atol=1e-300)
 
 
+@pytest.mark.parametrize("name", ["co2_mini", "three_gas_edge", "tile1"])
+def test_golden_through_the_seam(name):
+    """The reference's seam called the way its callers call it
+    (forward.py:148, config.py:253-259): the golden grid array itself, the
+    layers, cutoff and tile width.  Same shape as the reference's od, same
+    values to 1e-9, same warnings."""
+    from pylblrtm_amd.seam import build_optical_depth
+    say(f"case: golden seam {name}")
+    z, sd, prof, meta = _golden(name)
+    nu = z["out_nu"]
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        od = build_optical_depth(nu, sd, prof.build_layers(), meta["cutoff"], meta["tile"])
+    assert sorted({str(x.message) for x in w}) == meta["warnings"]
+    ref = z["out_od_layers"]
+    assert od.shape == ref.shape == (prof.build_layers().n_layers, nu.size)
+    np.testing.assert_allclose(od, ref, rtol=TAU_RTOL_FP64, atol=0)
+
+
 def test_golden_mixed_transmittance():
     say("case: golden mixed co2_mini")
     z, sd, prof, meta = _golden("co2_mini")
@@ -345,3 +364,42 @@ def test_cutoff_below_near_field_radius_vs_oracle():
     relm = np.abs(om[big] - od[big]) / od[big]
     say(f"  mixed max rel tau per layer {relm.max():.2e}")
     assert relm.max() < 2e-5
+
+
+# ── device-side index checks ─────────────────────────────────────────────────
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_out_of_range_indices_return_einval(precision):
+    """A line's group outside [0, n_groups), a group's gas row outside
+    [0, n_gases) and an empty TIPS range are flagged by the per-(layer, line)
+    kernel, which then reads nothing through them; the call returns
+    LBL_EINVAL (ValueError) and the same problem runs again once repaired."""
+    say(f"case: invalid device indices {precision}")
+    sd, prof = synth.co2_window(n_lines=300, n_levels=5)
+    prob = OpticalDepthProblem(sd, prof.build_layers(), 720.0, 721.0, 0.001, 25.0, 2.0)
+    good = prob.run(precision, mus=(1.0,), want_tau=True)["optical_depth"].double().cpu()
+    cases = [("group", 17, 99, "group index"), ("group", 3, -1, "group index"),
+             ("gas", 0, 5, "gas index"), ("toff", 1, 0, "TIPS")]
+    for key, pos, bad, msg in cases:
+        keep = int(prob._t[key][pos].item())
+        prob._t[key][pos] = bad
+        with pytest.raises(ValueError, match=msg):
+            prob.run(precision, mus=(1.0,))
+        prob._t[key][pos] = keep
+        again = prob.run(precision, mus=(1.0,), want_tau=True)["optical_depth"].double().cpu()
+        assert float(((again - good).abs() / good).max()) <= 1e-6
+
+
+def test_out_of_range_group_in_vjp_returns_einval():
+    import torch
+    say("case: invalid device indices vjp")
+    sd, prof = synth.co2_window(n_lines=300, n_levels=5)
+    prob = OpticalDepthProblem(sd, prof.build_layers(), 720.0, 721.0, 0.001, 25.0, 2.0)
+    fw = prob.run("mixed", mus=(1.0,), want_layers=True)
+    G = torch.ones_like(fw["transmittance"])
+    prob._t["group"][5] = 42
+    with pytest.raises(ValueError, match="group index"):
+        prob.vjp(fw["transmittance"], fw["optical_depth_layers"], G, (1.0,))
+    prob._t["group"][5] = 0
+    gt, gv = prob.vjp(fw["transmittance"], fw["optical_depth_layers"], G, (1.0,))
+    assert torch.isfinite(gt).all() and torch.isfinite(gv).all()
