@@ -40,9 +40,15 @@ namespace {
 
 constexpr int kChunk = kBlock;  // records staged per LDS round
 constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
-constexpr int kQueueCap = 8192; // near-centre (record, thread) pairs per chunk (16-bit entries)
 constexpr int kExCap = 128;     // far-field exception list entries per wavefront
 constexpr int kPointsFp64 = 8;  // grid points per thread of the FP64 line sums
+// Mixed near-centre drain: queued points nearer than |z| = kCoreR take the
+// trapezoid + residue form (the four-node form is <= 5e-6 only from 6.6); a
+// wavefront's list of such points holds kCoreCap entries.
+constexpr float kCoreR = 7.0f;
+constexpr int kCoreCap = 512;   // >= the 64 x 8 points one pass-A round can add
+constexpr int kQueueW = 256;     // queue entries per wavefront (the loop drains at kQueueW - 128)
+constexpr int kAccStride = 256;  // s_acc row length of the mixed line sums
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
@@ -132,8 +138,9 @@ void k_voigt_sum_fp64(const SumArgs a) {
   const double t0sq = kGh4T[0] * kGh4T[0], t1sq = kGh4T[1] * kGh4T[1];
   const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
   const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
-  double* const acc = s_acc + threadIdx.x * P;
-  for (int j = 0; j < P; ++j) acc[j] = 0.0;
+  // s_acc[j][thread]: a wavefront's lanes touch consecutive words (no bank conflicts)
+  double* const acc = s_acc + threadIdx.x;
+  for (int j = 0; j < P; ++j) acc[j * kBlock] = 0.0;
   if (lane == 0) s_qn[wave] = 0;
 
   double tau[P];
@@ -322,15 +329,15 @@ void k_voigt_sum_fp64(const SumArgs a) {
           const double y = ry[r], am = ra[r];
           for (int j = 0; j < P; ++j) {
             const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P + j) - R.c) * R.s;
-            atomicAdd(&s_acc[tt * P + j], am * K_fp64_call(x, y));
+            atomicAdd(&s_acc[j * kBlock + tt], am * K_fp64_call(x, y));
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          lay[j] += acc[j];
-          acc[j] = 0.0;
+          lay[j] += acc[j * kBlock];
+          acc[j * kBlock] = 0.0;
         }
         if (lane == 0) s_qn[wave] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -995,10 +1002,16 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // atomics) into the owning lane's partial sums.
   constexpr int kWaves = kMixedBlock / 64;
   constexpr int kStage = 64;                        // records per wavefront stage
-  constexpr int kQueueCapW = kQueueCap / kWaves;    // queue entries per wavefront
   __shared__ MixedRec s_rec[kWaves][kStage];
-  __shared__ uint16_t s_q[kWaves][kQueueCapW];
-  __shared__ float s_acc[kMixedBlock * P];
+  // queue: x at the lane's first point, (record << 6) | lane
+  __shared__ float s_qx[kWaves][kQueueW];
+  __shared__ uint16_t s_qi[kWaves][kQueueW];
+  __shared__ float s_acc[kAccStride * P];
+  // queued points in the core (|z| < kCoreR): x, (record << 9) | (lane << 3) | point
+  __shared__ float s_cx[kWaves][kSurrogate ? 1 : kCoreCap];
+  __shared__ uint16_t s_ci[kWaves][kSurrogate ? 1 : kCoreCap];
+  // claim byte per owner lane (the drain's plain read-modify-write rounds)
+  __shared__ uint8_t s_claim[kWaves][64];
 
   // (layer group, tile slice), group-major so that an XCD holds neighbouring
   // tiles of one layer range
@@ -1021,7 +1034,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const int32_t rlo = a.lo[tile], rhi = a.hi[tile];
   static_assert(P == kPointsMixed, "records carry hw for kPointsMixed points per thread");
   // middle of this thread's run of P points; point j sits at x_mid + (j - (P-1)/2) hs
-  const double nu_m = __dadd_rn(grid_nu(a.nu_min, a.step, i0), 0.5 * (P - 1) * a.step);
+  const double nu_f = grid_nu(a.nu_min, a.step, i0);   // this thread's first point
+  const double nu_m = __dadd_rn(nu_f, 0.5 * (P - 1) * a.step);
   const bool live = i0 < t1;   // lanes wholly past the tile end never write
   const MixedRec* __restrict__ recs = a.rec.mix;
   // This wavefront's near range: records within kSeriesRatio H of its middle
@@ -1030,9 +1044,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
   // this wavefront's stage as an LDS-qualified pointer, so record reads stay ds_read_b128
   lds_mixed_rec* const s_rec_lds = (lds_mixed_rec*)s_rec[wave];
-  float* const acc = s_acc + threadIdx.x * P;
+  // s_acc[j][thread]: a wavefront's lanes touch consecutive words
+  float* const acc = s_acc + threadIdx.x;
 #pragma unroll
-  for (int j = 0; j < P; ++j) acc[j] = 0.0f;
+  for (int j = 0; j < P; ++j) acc[j * kAccStride] = 0.0f;
 
   // column totals (compensated) live in tau_part[grp], read and rewritten
   // once per layer, so that no registers hold them through the record loops
@@ -1059,7 +1074,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      int nq = 0;   // queue entries of this stage (uniform)
+      int nq = 0;   // queue entries not yet drained (uniform)
       // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
       // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
       // keeps each a single ds_read_b128.  Three tiers per (record, thread):
@@ -1068,7 +1083,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
       auto record = [&](const int q, const u32x4 h0, const volatile lds_u32x4* const body) {
         const double c = __hiloint2double((int)h0.y, (int)h0.x);
-        const float xm = (float)(nu_m - c) * __uint_as_float(h0.z);
+        const float sv = __uint_as_float(h0.z);
+        const float xm = (float)(nu_m - c) * sv;
         const u32x4 h1 = *body;
         const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
         const float ay = __uint_as_float(h1.z);
@@ -1120,92 +1136,162 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             }
           }
         }
-        // append the queued lanes' (q, lane) to this wavefront's queue; the
-        // count nq advances in every lane (outside the divergent tiers)
+        // append the queued lanes' entries (x at the lane's first point, from
+        // the FP64 offset as the far test forms it; record and lane) to this
+        // wavefront's queue; the count nq advances in every lane
         const uint64_t msk = __ballot(queued);
         if (msk) {   // uniform
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
               (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-          const int pos = nq + (int)rank;
           if (queued) {
-            if (pos < kQueueCapW) {
-              s_q[wave][pos] = (uint16_t)((q << 6) | lane);
-            } else {  // queue full: evaluate here
-              const float y = s_rec[wave][q].y, am = s_rec[wave][q].a;
-#pragma unroll 1
-              for (int j = 0; j < P; ++j) {
-                const float v = am * K_mixed(fmaf((float)j - 0.5f * (float)(P - 1), hs, xm), y);
-#pragma unroll
-                for (int jj = 0; jj < P; ++jj) part[jj] += (jj == j) ? v : 0.0f;
-              }
-            }
+            const int pos = nq + (int)rank;   // < kQueueW: the loop drains before it fills
+            s_qx[wave][pos] = (float)(nu_f - c) * sv;
+            s_qi[wave][pos] = (uint16_t)((q << 6) | lane);
           }
           nq += (int)__popcll(msk);
         }
       };
-      // Two records per iteration, each header read two records ahead into
-      // its own registers (no copies between iterations); kStage entries
-      // exist, so record 1 is readable even when cnt == 1.  The record
-      // address lives in a VGPR (the empty asm makes it one) so every LDS
-      // read uses an immediate offset instead of a copy from an SGPR.
-      u32x4 ha = rb[0], hb = rb[3];
-      const volatile lds_u32x4* pr = rb;
-      asm volatile("" : "+v"(pr));
-      for (int q = 0; q < cnt; q += 2, pr += 6) {
-        record(q, ha, pr + 1);
-        if (q + 2 < cnt) ha = pr[6];
-        if (q + 1 < cnt) {
-          record(q + 1, hb, pr + 4);
-          if (q + 3 < cnt) hb = pr[9];
-        }
-        if (((q + 2) & (kFoldEvery - 1)) == 0) {   // after records 31, 63
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            kahan_add(lay[j], layc[j], part[j]);
-            part[j] = 0.0f;
-          }
-        }
-      }
-      nq = min(nq, kQueueCapW);
-      if (nq > 0) {   // uniform
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      // The queue, drained by this wavefront.  kSurrogate: every queued point
+      // goes to the network (inside its domain and gate) or to the full FP32
+      // dispatch.  Otherwise two passes, each dense and free of divergent
+      // regions.  Pass A: one entry per lane, all P points in the four-node
+      // form (<= 5e-6 for |z| >= 6.6, faddeeva.h); points with |z| < kCoreR
+      // (the core) get weight 0 and go to the wavefront's core list (x and the
+      // target kept in the entry).  Pass B: the core list, one point per lane,
+      // trapezoid + residue form.  Results reach the owning lane's sums
+      // through s_acc (LDS float atomics).
+      auto drain = [&](const int n) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
         __builtin_amdgcn_wave_barrier();
-        // dense drain by this wavefront: entry = (record, lane); points at
-        // their exact grid values
-        for (int e = lane; e < nq; e += 64) {
-          const uint32_t ent = s_q[wave][e];
-          const int q = (int)(ent >> 6), t = (int)(ent & 63);
-          const MixedRec& R = s_rec[wave][q];
-          const int tt = wave * 64 + t;   // owning thread
-          const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P);
-          const float x0 = (float)(nu_t - R.c) * R.s;
-          const float ac = R.a * (float)R.c;
-          int exact = 0;
-          for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j, R.hs, x0);
-            if (kSurrogate && surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
-            atomicAdd(&s_acc[tt * P + j], R.a * K_mixed(x, R.y));
-            ++exact;
+        if constexpr (!kSurrogate) {
+          int ncore = 0;   // uniform
+          auto core_pass = [&](const int m) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
+            __builtin_amdgcn_wave_barrier();
+            // two points per lane per round, so the list and record reads of
+            // both are in flight together
+            for (int b = lane; b < m; b += 128) {
+              const bool two = b + 64 < m;
+              const float xa = s_cx[wave][b], xb = two ? s_cx[wave][b + 64] : 0.0f;
+              const uint32_t ca = s_ci[wave][b], cb = two ? s_ci[wave][b + 64] : ca;
+              const u32x4 ha2 = rb[3 * (ca >> 9) + 2], hb2 = rb[3 * (cb >> 9) + 2];   // y, a, ds1, sh
+              const float va = __uint_as_float(ha2.y) * K_cr32(xa, __uint_as_float(ha2.x));
+              const float vb = __uint_as_float(hb2.y) * K_cr32(xb, __uint_as_float(hb2.x));
+              atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + ((ca >> 3) & 63)], va);
+              if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + ((cb >> 3) & 63)], vb);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
+            __builtin_amdgcn_wave_barrier();
+          };
+          const float t0 = (float)(kGh4T[0] * kGh4T[0]), t1 = (float)(kGh4T[1] * kGh4T[1]);
+          for (int e0 = 0; e0 < n; e0 += 64) {
+            const int e = e0 + lane;
+            const bool has = e < n;
+            float x0 = 0.0f;
+            uint32_t qi = 0;
+            if (has) {
+              x0 = s_qx[wave][e];
+              qi = s_qi[wave][e];
+            }
+            const int t = (int)(qi & 63);
+            const u32x4 h1 = rb[3 * (qi >> 6) + 1];   // hs, c1, ay, hw2
+            const float hs = __uint_as_float(h1.x), y2 = __uint_as_float(h1.y) + 1.5f;
+            // four-node form: A K = (2 A y / pi) sum_k W_k (r + t_k^2) / D_k, and
+            // A y = sqrt(pi) ay
+            const float c2 = has ? 2.0f * (float)kInvSqrtPi * __uint_as_float(h1.z) : 0.0f;
+            const float w0 = c2 * (float)kGh4W[0], w1 = c2 * (float)kGh4W[1];
+            const float fy2 = 4.0f * y2;
+            uint32_t cmask = 0;
+            float v[P];
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              const float x = fmaf((float)j, hs, x0);
+              const float x2 = x * x;
+              const float r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+              const bool core = has && r < kCoreR * kCoreR;
+              const float u0 = xmy - t0, u1 = xmy - t1;
+              const float D0 = fmaf(u0, u0, fxy), D1 = fmaf(u1, u1, fxy);
+              const float N = fmaf(w0 * (r + t0), D1, w1 * (r + t1) * D0);
+              v[j] = core ? 0.0f : N * __builtin_amdgcn_rcpf(D0 * D1);
+              cmask |= core ? (1u << j) : 0u;
+            }
+            // Deliver the P values to the owner's accumulators with plain LDS
+            // reads and writes (LDS float atomics cost far more): in each
+            // round every pending lane writes its id to its owner's claim
+            // byte, the lanes that read their own id back hold distinct
+            // owners and update, the rest try again (entries of several
+            // records for one owner take one round each).
+            bool pending = has;
+            while (__ballot(pending)) {   // uniform
+              if (pending) s_claim[wave][t] = (uint8_t)lane;
+              __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+              if (pending && s_claim[wave][t] == (uint8_t)lane) {
+                float* const dst = s_acc + wave * 64 + t;
+#pragma unroll
+                for (int j = 0; j < P; ++j) dst[j * kAccStride] += v[j];
+                pending = false;
+              }
+              __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            }
+            // this lane's core points go to the list at an exclusive prefix sum
+            // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
+            const uint32_t cnt = __builtin_popcount(cmask);
+            uint32_t pre = 0, total = 0;
+#pragma unroll
+            for (int bit = 0; bit < 4; ++bit) {
+              const uint64_t m = __ballot((cnt >> bit) & 1u);
+              pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+              total += (uint32_t)__popcll(m) << bit;
+            }
+            if (total) {   // uniform
+              if (ncore + (int)total > kCoreCap) {
+                core_pass(ncore);
+                ncore = 0;
+              }
+              int off = ncore + (int)pre;
+#pragma unroll
+              for (int j = 0; j < P; ++j) {
+                if (cmask & (1u << j)) {
+                  s_cx[wave][off] = fmaf((float)j, hs, x0);
+                  s_ci[wave][off] = (uint16_t)((qi >> 6) << 9 | (uint32_t)t << 3 | (uint32_t)j);
+                  ++off;
+                }
+              }
+              ncore += (int)total;
+            }
           }
-          if (kSurrogate && a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
-        }
-        if constexpr (kSurrogate) {
+          if (ncore > 0) core_pass(ncore);
+        } else {
+          for (int e = lane; e < n; e += 64) {
+            const float x0 = s_qx[wave][e];
+            const uint32_t qi = s_qi[wave][e];
+            const MixedRec& R = s_rec[wave][qi >> 6];
+            const int tt = wave * 64 + (int)(qi & 63);   // owning thread
+            const float ac = R.a * (float)R.c;
+            int exact = 0;
+            for (int j = 0; j < P; ++j) {
+              const float x = fmaf((float)j, R.hs, x0);
+              if (surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
+              atomicAdd(&s_acc[j * kAccStride + tt], R.a * K_mixed(x, R.y));
+              ++exact;
+            }
+            if (a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
+          }
           // network pass: 64 / P queue entries (64 points) per round (lane =
           // entry (l / P), point (l % P)); all lanes stay active through the
           // MFMA evaluation.
           constexpr int kPer = 64 / P;
-          for (int g = 0; g * kPer < nq; ++g) {
+          for (int g = 0; g * kPer < n; ++g) {
             const int e = g * kPer + lane / P, j = lane % P;
             bool use = false;
             float x = 0.0f, y = 1.0f, am = 0.0f;
             int tt = 0;
-            if (e < nq) {
-              const uint32_t ent = s_q[wave][e];
-              const int q = (int)(ent >> 6);
-              tt = wave * 64 + (int)(ent & 63);
-              const MixedRec& R = s_rec[wave][q];
-              const double nu_t = grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P);
-              x = fmaf((float)j, R.hs, (float)(nu_t - R.c) * R.s);
+            if (e < n) {
+              const uint32_t qi = s_qi[wave][e];
+              tt = wave * 64 + (int)(qi & 63);
+              const MixedRec& R = s_rec[wave][qi >> 6];
+              x = fmaf((float)j, R.hs, s_qx[wave][e]);
               y = R.y;
               am = R.a;
               use = surrogate_point(x, y, am * (float)R.c, a.mlp_gate);
@@ -1216,16 +1302,54 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             const float u = use ? mlp::input_u(x) : 0.0f;
             const float v = use ? mlp::input_v(__log10f(y)) : 0.0f;
             const float tv = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
-            if (use) atomicAdd(&s_acc[tt * P + j], am * mlp::kref(x, y) * __expf(tv));
+            if (use) atomicAdd(&s_acc[j * kAccStride + tt], am * mlp::kref(x, y) * __expf(tv));
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          part[j] += acc[j];
-          acc[j] = 0.0f;
+          part[j] += acc[j * kAccStride];
+          acc[j * kAccStride] = 0.0f;
         }
+      };
+      // Two records per iteration, each header read two records ahead into
+      // its own registers (no copies between iterations); kStage entries
+      // exist, so record 1 is readable even when cnt == 1.  The record
+      // address lives in a VGPR (the empty asm makes it one) so every LDS
+      // read uses an immediate offset instead of a copy from an SGPR.  The
+      // queue is drained when it could not take two more records, and at the
+      // end of the stage.
+      u32x4 ha = rb[0], hb = rb[3];
+      const volatile lds_u32x4* pr = rb;
+      asm volatile("" : "+v"(pr));
+      int q = 0;
+      for (;;) {
+        for (; q < cnt; q += 2, pr += 6) {
+          record(q, ha, pr + 1);
+          if (q + 2 < cnt) ha = pr[6];
+          if (q + 1 < cnt) {
+            record(q + 1, hb, pr + 4);
+            if (q + 3 < cnt) hb = pr[9];
+          }
+          if (((q + 2) & (kFoldEvery - 1)) == 0) {   // after records 31, 63
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              kahan_add(lay[j], layc[j], part[j]);
+              part[j] = 0.0f;
+            }
+          }
+          if (nq > kQueueW - 128) {   // uniform
+            q += 2;
+            pr += 6;
+            break;
+          }
+        }
+        if (nq > 0) {   // uniform
+          drain(nq);
+          nq = 0;
+        }
+        if (q >= cnt) break;
       }
 #pragma unroll
       for (int j = 0; j < P; ++j) {
