@@ -272,6 +272,24 @@ int lbl_last_timings(double *ms_line_params, double *ms_voigt_sum);
  * layer-group epilogue); these are their durations in ms. */
 int lbl_last_timings_split(double *ms_far_series, double *ms_near_sum);
 
+/* Work counting (no counterpart in the reference; for measurement and for
+ * tests of the work counts).  When enabled, lbl_optical_depth (LBL_FP64 and
+ * LBL_MIXED) runs the counting variants of its line-sum kernels, which give
+ * the same results and add up what they evaluate; lbl_last_work_count then
+ * returns, for the calling thread's last call, up to n of these counts:
+ *   [0] near-field point evaluations (records inside the tile window x the
+ *       points of each wavefront that takes them point by point)
+ *   [1] (record, slice) series pairs      [2] (record, wavefront) series pairs
+ *   [3] records the slice level passes on (its form does not hold there)
+ *   [4] records the wavefront level evaluates point by point
+ *   [5] their point evaluations           [6] queued near-centre (record, lane) pairs
+ *   [7] queued core points (LBL_MIXED)    [8] slice-list overflows
+ *   [9] wavefront-level candidates taken from the slice level's list [3]
+ *       (one per record and live wavefront of its slice)
+ * All counts are 0 after a call made without counting. */
+void lbl_enable_work_count(int on);
+int lbl_last_work_count(uint64_t *counts, int32_t n);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char *lbl_last_error(void);
 

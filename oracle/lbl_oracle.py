@@ -251,25 +251,3 @@ def compute_transmittance(sd, profile, nu_min, nu_max, *, step_cm1=0.001, line_c
     od = optical_depth(nu, sd.lines, sd.tips, sd.molar_masses, p_atm, t_lay, v_lay, col,
                        line_cutoff_cm1, tile_width_cm1, tiles=tiles)
     return nu, transmittance(od, mu_view), od
-
-
-def count_evals(nu, lines, tips, molar_masses, p_atm, t_lay, vmr_lay, col_lay, cutoff, tile_width):
-    """Exact E = sum_k sum_tiles |L_k(tile)| * tile_pts under the reference rule."""
-    ts = tile_samples(nu, tile_width)
-    bounds = tile_bounds(nu, ts)
-    groups = prepare_groups(nu, lines, tips, molar_masses, p_atm, t_lay, col_lay.keys(), cutoff)
-    lo_b = np.array([nu[a] - cutoff for a, _ in bounds])
-    hi_b = np.array([nu[b - 1] + cutoff for _, b in bounds])
-    npts = np.array([b - a for a, b in bounds], np.int64)
-    total = 0
-    for k in range(len(p_atm)):
-        for g in groups:
-            col = float(col_lay[g["gas"]][k])
-            if col == 0.0:
-                continue
-            p_self = float(vmr_lay[g["gas"]][k]) * float(p_atm[k])
-            p_air = float(p_atm[k]) - p_self
-            sc = np.sort(lines["nu0"][g["idx"]] + lines["shift"][g["idx"]] * p_air)
-            cnt = np.searchsorted(sc, hi_b, side="right") - np.searchsorted(sc, lo_b, side="left")
-            total += int(np.sum(np.maximum(cnt, 0) * npts))
-    return total

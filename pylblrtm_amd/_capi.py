@@ -22,7 +22,7 @@ EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_l
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
             "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
             "lbl_optical_depth_surrogate", "lbl_par_read", "lbl_optical_depth_noalloc",
-            "lbl_grid_from_nu")
+            "lbl_grid_from_nu", "lbl_enable_work_count", "lbl_last_work_count")
 
 _vp = ctypes.c_void_p
 
@@ -128,6 +128,10 @@ def load() -> ctypes.CDLL:
     lib.lbl_grid_from_nu.argtypes = [_vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                      P(LblGrid)]
     lib.lbl_grid_from_nu.restype = ctypes.c_int
+    lib.lbl_enable_work_count.argtypes = [ctypes.c_int]
+    lib.lbl_enable_work_count.restype = None
+    lib.lbl_last_work_count.argtypes = [P(ctypes.c_uint64), ctypes.c_int32]
+    lib.lbl_last_work_count.restype = ctypes.c_int
     lib.lbl_last_error.argtypes = []
     lib.lbl_last_error.restype = ctypes.c_char_p
     _lib = lib
@@ -160,6 +164,23 @@ def grid_from_nu(nu, tile_width: float, cutoff: float) -> LblGrid:
                                  float(cutoff), ctypes.byref(g))
     raise_for(st, "lbl_grid_from_nu")
     return g
+
+
+WORK_COUNT_NAMES = ("near_evals", "slice_series", "wave_series", "slice_exceptions",
+                    "wave_exceptions", "exception_evals", "queued", "core_points", "rescans",
+                    "wave_from_slice_list")
+
+
+def enable_work_count(on: bool = True) -> None:
+    load().lbl_enable_work_count(1 if on else 0)
+
+
+def last_work_count() -> dict:
+    """The line-sum kernels' own work counts of the calling thread's last
+    lbl_optical_depth call made with work counting enabled (names above)."""
+    arr = (ctypes.c_uint64 * len(WORK_COUNT_NAMES))()
+    raise_for(load().lbl_last_work_count(arr, len(WORK_COUNT_NAMES)), "lbl_last_work_count")
+    return {n: int(v) for n, v in zip(WORK_COUNT_NAMES, arr)}
 
 
 def enable_timing(on: bool = True) -> None:

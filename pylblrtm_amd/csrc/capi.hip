@@ -22,6 +22,10 @@ hipEvent_t g_events[kMaxDevices][5];
 bool g_events_made[kMaxDevices] = {};
 thread_local double g_ms_line_params = 0.0, g_ms_voigt_sum = 0.0;
 thread_local double g_ms_far = 0.0, g_ms_near = 0.0;
+// Work counting (lbl_enable_work_count): the line sums run their counting
+// variants and the counts of the calling thread's last call are kept here.
+bool g_count = false;
+thread_local uint64_t g_work[lbl::kWorkSlots] = {};
 
 hipError_t timing_events(hipEvent_t** ev) {
   int dev = 0;
@@ -103,7 +107,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, err, tau_part, far, total;
+  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, err, work, tau_part, far, total;
   size_t g[5];   // adjoint coefficient arrays (only with_grad)
   int64_t n_tiles, n_wg;
   int32_t slices_per_tile, layer_groups;
@@ -135,6 +139,7 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->slo = off; off = align_up(off + (size_t)L->n_wg * 4);
   L->shi = off; off = align_up(off + (size_t)L->n_wg * 4);
   L->err = off; off = align_up(off + 4);
+  L->work = off; off = align_up(off + lbl::kWorkSlots * sizeof(unsigned long long));
   // partial tau per layer group (float sum + compensation, or double): 8 B
   L->tau_part = off;
   // (the mixed sums keep their column totals here even with one group)
@@ -202,6 +207,14 @@ int lbl_last_timings(double* ms_line_params, double* ms_voigt_sum) {
 int lbl_last_timings_split(double* ms_far_series, double* ms_near_sum) {
   if (ms_far_series) *ms_far_series = g_ms_far;
   if (ms_near_sum) *ms_near_sum = g_ms_near;
+  return LBL_OK;
+}
+
+void lbl_enable_work_count(int on) { g_count = on != 0; }
+
+int lbl_last_work_count(uint64_t* counts, int32_t n) {
+  if (!counts || n < 0) return fail(LBL_EINVAL, "lbl_last_work_count: null counts");
+  for (int i = 0; i < n; ++i) counts[i] = i < lbl::kWorkSlots ? g_work[i] : 0;
   return LBL_OK;
 }
 
@@ -361,6 +374,12 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     sa.mlp_count = reinterpret_cast<unsigned long long*>(sur->n_network);
     sa.exact_count = reinterpret_cast<unsigned long long*>(sur->n_exact_near);
   }
+  unsigned long long* work = nullptr;
+  if (g_count && allow_timing && precision != LBL_SURROGATE) {
+    work = reinterpret_cast<unsigned long long*>(ws + L.work);
+    HIP_TRY(lbl::launch_fill_u32(reinterpret_cast<uint32_t*>(work), 2 * lbl::kWorkSlots, 0u, s));
+    sa.work = work;
+  }
   if (ev) HIP_TRY(hipEventRecord(ev[2], s));
   HIP_TRY(lbl::launch_far_series(sa, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[4], s));
@@ -370,7 +389,9 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
 
   uint32_t code = lbl::kErrNone;
   HIP_TRY(hipMemcpyAsync(&code, err, sizeof(code), hipMemcpyDeviceToHost, s));
+  if (work) HIP_TRY(hipMemcpyAsync(g_work, work, sizeof(g_work), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (!work) memset(g_work, 0, sizeof(g_work));
   if (ev) {
     float a = 0.f, b = 0.f;
     HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));

@@ -195,6 +195,32 @@ struct SumArgs {
   // far-field sum per (layer, point), FP32 (mixed modes) or FP64, written
   // by k_far_series_* and added by the line sum
   void* far_sum;
+  // when set, the counting variants of the line-sum kernels add their work
+  // units to these kWorkSlots counters (same results otherwise)
+  unsigned long long* work;
+};
+
+// Work units the line-sum kernels count in their counting variant:
+//   near-field point evaluations (records inside the tile window x points of
+//   the wavefront), (record, slice) and (record, wavefront) series pairs,
+//   records that fail the slice level's validity test (passed on to the
+//   wavefront level) and the wavefront level's (evaluated point by point,
+//   and their point evaluations), queued (record, lane) pairs, queued core
+//   points (mixed), slice-list overflows (rescans), and the wavefront-level
+//   candidates taken from the slice level's list (each of its records is one
+//   for every live wavefront of the slice).
+enum WorkCount {
+  kWorkNearEvals = 0,
+  kWorkSliceSeries,
+  kWorkWaveSeries,
+  kWorkSliceExceptions,
+  kWorkWaveExceptions,
+  kWorkExceptionEvals,
+  kWorkQueued,
+  kWorkCorePoints,
+  kWorkRescans,
+  kWorkWaveFromSliceList,
+  kWorkSlots
 };
 
 // Workgroups a line-sum launch aims for.  Many short workgroups (a layer

@@ -62,6 +62,12 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
 }
 
 
+// Work counting variant (kCount): per-wavefront totals added once to the
+// WorkCount slots of a.work (internal.h).
+__device__ __forceinline__ void count_add(unsigned long long* w, int slot, uint64_t v) {
+  if (v) atomicAdd(w + slot, (unsigned long long)v);
+}
+
 __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
   atomicMin(err, err_numeric(k, 1, 0));
 }
@@ -96,7 +102,7 @@ constexpr int kQueueCap64 = 3840;
 // near-centre LDS queue, which the whole workgroup drains at the end of the
 // chunk with the full FP64 dispatch (8-node form and trapezoid+residue),
 // adding the results into the owner's sums with LDS FP64 atomics.
-template <int P>
+template <int P, bool kCount>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_fp64(const SumArgs a) {
   // every wavefront works alone on its own near range (k_far_series_fp64 has
@@ -146,6 +152,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
   double tau[P];
 #pragma unroll
   for (int j = 0; j < P; ++j) tau[j] = 0.0;
+  uint64_t n_in = 0, n_queued = 0;   // kCount: records in the window, queued pairs
 
   for (int k = k_begin; k < k_end; ++k) {
     double lay[P];
@@ -155,10 +162,12 @@ void k_voigt_sum_fp64(const SumArgs a) {
     for (int32_t base = nlo; base < nhi; base += kStage) {
       const int cnt = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      bool inside = false;
       if (lane < cnt) {
         const int64_t r = kb + base + lane;
         double c = a.rec.c[r], sv = rs[r], y = ry[r], am = ra[r];
-        if (!(c >= lo_b && c <= hi_b)) {  // outside the tile window (or skipped): adds 0
+        inside = c >= lo_b && c <= hi_b;
+        if (!inside) {  // outside the tile window (or skipped): adds 0
           c = 0.0; sv = 0.0; y = 1.0e3; am = 0.0;
         }
         Rec64 R;
@@ -172,6 +181,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       __builtin_amdgcn_wave_barrier();
+      if constexpr (kCount) n_in += (uint64_t)__popcll(__ballot(inside));
       const int ncnt = cnt;
       // Records are read from LDS as 16-byte pieces: (c, s) and (hw1, hs) two
       // records ahead into their own registers (two records per iteration),
@@ -286,6 +296,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
           int qb = 0;
           if (lane == 0) qb = atomicAdd(&s_qn[wave], (int)__popcll(msk));
           qb = __builtin_amdgcn_readfirstlane(qb);
+          if constexpr (kCount) n_queued += (uint64_t)__popcll(msk);
           const int pos = qb + (int)rank;
           if (queued) {
             if (pos < kQueueCapW) {
@@ -372,6 +383,14 @@ void k_voigt_sum_fp64(const SumArgs a) {
       if (a.tau) static_cast<double*>(a.tau)[i] = tau[j];
     }
   }
+  if constexpr (kCount) {
+    if (lane == 0) {
+      const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
+      const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
+      count_add(a.work, kWorkNearEvals, n_in * w_pts);
+      count_add(a.work, kWorkQueued, n_queued);
+    }
+  }
 }
 
 // ----------------------------------------------------------------- mixed --
@@ -402,7 +421,7 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
 // cubed); a record that fails at the slice level goes on to the wavefront
 // level, and one that fails there is evaluated point by point with the full
 // FP64 dispatch.
-template <int P>
+template <int P, bool kCount>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_far_series_fp64(const SumArgs a) {
   static_assert(P == kPointsFp64 && 64 * P == kWavePoints && kBlock * P == kSlicePoints,
@@ -521,6 +540,7 @@ void k_far_series_fp64(const SumArgs a) {
       return v < n_below ? (int64_t)rlo + v : (int64_t)shi + (v - n_below);
     };
     double nc = __builtin_nan(""), ns = 0.0, ny = 1.0, na = 0.0;
+    uint64_t n_sser = 0, n_sexc = 0;   // kCount
     if ((int)threadIdx.x < n_sl) {
       const int64_t i = sl_index(threadIdx.x);
       nc = rc[i]; ns = rs[i]; ny = ry[i]; na = ra[i];
@@ -540,6 +560,16 @@ void k_far_series_fp64(const SumArgs a) {
       if (inw & !ser) {   // on to the wavefront level
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
+      }
+      if constexpr (kCount) {
+        n_sser += (uint64_t)__popcll(__ballot(ser));
+        n_sexc += (uint64_t)__popcll(__ballot(inw & !ser));
+      }
+    }
+    if constexpr (kCount) {
+      if (lane == 0) {
+        count_add(a.work, kWorkSliceSeries, n_sser);
+        count_add(a.work, kWorkSliceExceptions, n_sexc);
       }
     }
 #pragma unroll
@@ -629,6 +659,7 @@ void k_far_series_fp64(const SumArgs a) {
     }
   };
   int n_ex = 0;
+  uint64_t n_wser = 0, n_wexc = 0, n_wlist = 0;   // kCount
   double nc = __builtin_nan(""), ns = 0.0, ny = 1.0, na = 0.0;
   if (lane < n_far) {
     const int64_t i = far_index(lane);
@@ -653,6 +684,11 @@ void k_far_series_fp64(const SumArgs a) {
     if (ser) series_gh4(C, d, sv, y, am, H);
     const bool exc = inw & !ser;
     const uint64_t ex = __ballot(exc);
+    if constexpr (kCount) {
+      n_wser += (uint64_t)__popcll(__ballot(ser));
+      n_wexc += (uint64_t)__popcll(ex);
+      n_wlist += (uint64_t)__popcll(__ballot(inw & (v >= nA + nB) & (v < nA + nB + nC)));
+    }
     if (ex) {   // uniform
       if (n_ex + 64 > kExCap) {
         far_exceptions(n_ex);
@@ -679,6 +715,16 @@ void k_far_series_fp64(const SumArgs a) {
     for (int n = kN - 2; n >= 0; --n) f = fma(f, t, C[n]);
     const int64_t i = i0 + j;
     if (i < t1) out[i] = f + exv[j];
+  }
+  if constexpr (kCount) {
+    if (lane == 0) {
+      const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
+      count_add(a.work, kWorkWaveSeries, n_wser);
+      count_add(a.work, kWorkWaveExceptions, n_wexc);
+      count_add(a.work, kWorkExceptionEvals, n_wexc * w_pts);
+      count_add(a.work, kWorkWaveFromSliceList, n_wlist);
+      if (wave == 0 && rescan) count_add(a.work, kWorkRescans, 1);
+    }
   }
 }
 
@@ -708,7 +754,7 @@ constexpr int kMixedBlock = 256;
 // series covers: |nu_m - c| >= ds1 + (Hu - H).  A record that fails this at
 // the slice level goes on to the wavefront level; one that fails it there is
 // evaluated at every lane's own points with the near field's tiers.
-template <int P>
+template <int P, bool kCount>
 __global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_far_series_mixed(const SumArgs a) {
   static_assert(P == kPointsMixed && 64 * P == kWavePoints && kMixedBlock * P == kSlicePoints,
@@ -793,6 +839,7 @@ void k_far_series_mixed(const SumArgs a) {
     };
     FarRec nx;
     nx.c = __builtin_nan("");
+    uint64_t n_sser = 0, n_sexc = 0;   // kCount
     if ((int)threadIdx.x < n_sl) nx = far_load(sl_index(threadIdx.x));
     const int32_t n_it = (n_sl + kMixedBlock - 1) / kMixedBlock;
     for (int32_t it = 0; it < n_it; ++it) {
@@ -807,6 +854,10 @@ void k_far_series_mixed(const SumArgs a) {
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
       }
+      if constexpr (kCount) {
+        n_sser += (uint64_t)__popcll(__ballot(ser));
+        n_sexc += (uint64_t)__popcll(__ballot(inw & !ser));
+      }
       if ((it & 15) == 15) {
         kahan_add(S0, S0c, C[0]);
         kahan_add(S1, S1c, C[1]);
@@ -815,6 +866,12 @@ void k_far_series_mixed(const SumArgs a) {
     }
     kahan_add(S0, S0c, C[0]);
     kahan_add(S1, S1c, C[1]);
+    if constexpr (kCount) {
+      if (lane == 0) {
+        count_add(a.work, kWorkSliceSeries, n_sser);
+        count_add(a.work, kWorkSliceExceptions, n_sexc);
+      }
+    }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
       S0 += __shfl_xor(S0, off);
@@ -927,6 +984,7 @@ void k_far_series_mixed(const SumArgs a) {
     }
   };
   int n_ex = 0;
+  uint64_t n_wser = 0, n_wexc = 0, n_wlist = 0;   // kCount
   FarRec nx;
   nx.c = __builtin_nan("");
   if (lane < n_far) nx = far_load(far_index(lane));
@@ -943,6 +1001,11 @@ void k_far_series_mixed(const SumArgs a) {
     if (ser) series(C, m, d, m.sh);
     const bool exc = inw & !ser;
     const uint64_t ex = __ballot(exc);
+    if constexpr (kCount) {
+      n_wser += (uint64_t)__popcll(__ballot(ser));
+      n_wexc += (uint64_t)__popcll(ex);
+      n_wlist += (uint64_t)__popcll(__ballot(inw & (v >= nA + nB) & (v < nA + nB + nC)));
+    }
     if (ex) {   // uniform
       if (n_ex + 64 > kExCap) {
         far_exceptions(n_ex);
@@ -990,9 +1053,19 @@ void k_far_series_mixed(const SumArgs a) {
     const int64_t i = i0 + j;
     if (i < t1) static_cast<float*>(a.far_sum)[(int64_t)k * a.n_nu + i] = lay[j] - layc[j];
   }
+  if constexpr (kCount) {
+    if (lane == 0) {
+      const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
+      count_add(a.work, kWorkWaveSeries, n_wser);
+      count_add(a.work, kWorkWaveExceptions, n_wexc);
+      count_add(a.work, kWorkExceptionEvals, n_wexc * w_pts);
+      count_add(a.work, kWorkWaveFromSliceList, n_wlist);
+      if (wave == 0 && rescan) count_add(a.work, kWorkRescans, 1);
+    }
+  }
 }
 
-template <int P, bool kSurrogate>
+template <int P, bool kSurrogate, bool kCount>
 __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // Every wavefront works alone (no workgroup barriers): it streams its own
   // near range through its own LDS stage of 64 records, and (record, lane)
@@ -1052,6 +1125,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // column totals (compensated) live in tau_part[grp], read and rewritten
   // once per layer, so that no registers hold them through the record loops
   float2* const tau_acc = static_cast<float2*>(a.tau_part) + (int64_t)grp * a.n_nu;
+  // kCount: records in the window, queued (record, lane) pairs, core points
+  uint64_t n_in = 0, n_queued = 0, n_core = 0;
 
   for (int k = k_begin; k < k_end; ++k) {
     float lay[P], layc[P], part[P];
@@ -1063,9 +1138,11 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       // stage: lane l copies record base + l (the previous stage's readers
       // are this wavefront's own, earlier in program order)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      bool inside = false;
       if (lane < cnt) {
         MixedRec m = recs[kb + base + lane];
-        if (!(m.c >= lo_b && m.c <= hi_b)) {  // outside the tile window (or skipped):
+        inside = m.c >= lo_b && m.c <= hi_b;
+        if (!inside) {  // outside the tile window (or skipped):
           // x = 0 passes the far test and adds ay / (0 + c1) = 0 exactly
           m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.c1 = 1.0f;
           m.ay = 0.0f; m.hw1 = 0.0f; m.a = 0.0f;
@@ -1074,6 +1151,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       __builtin_amdgcn_wave_barrier();
+      if constexpr (kCount) n_in += (uint64_t)__popcll(__ballot(inside));
       int nq = 0;   // queue entries not yet drained (uniform)
       // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
       // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
@@ -1259,6 +1337,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
                 }
               }
               ncore += (int)total;
+              if constexpr (kCount) n_core += total;
             }
           }
           if (ncore > 0) core_pass(ncore);
@@ -1346,6 +1425,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           }
         }
         if (nq > 0) {   // uniform
+          if constexpr (kCount) n_queued += (uint64_t)nq;
           drain(nq);
           nq = 0;
         }
@@ -1392,16 +1472,25 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       if (a.tau) static_cast<float*>(a.tau)[i] = t;
     }
   }
+  if constexpr (kCount) {
+    if (lane == 0) {
+      const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
+      const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
+      count_add(a.work, kWorkNearEvals, n_in * w_pts);
+      count_add(a.work, kWorkQueued, n_queued);
+      count_add(a.work, kWorkCorePoints, n_core);
+    }
+  }
 }
 
 // Four waves per SIMD (<= 128 VGPRs): every tile of a 2 cm^-1 x 0.001 grid
 // (1001 workgroups) is resident at once on 256 CUs.  (Measured: 3 waves/SIMD
 // left a second partial round, +12%; 16 points x 128 threads at 2 waves/SIMD
 // hid latency worse, +20%.)
-template <int P>
+template <int P, bool kCount>
 __global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_mixed(const SumArgs a) {
-  voigt_sum_mixed_body<P, false>(a);
+  voigt_sum_mixed_body<P, false, kCount>(a);
 }
 
 // Same register budget: the network's MFMA tiles spill around the (rarely
@@ -1409,7 +1498,7 @@ void k_voigt_sum_mixed(const SumArgs a) {
 template <int P>
 __global__ __launch_bounds__(kMixedBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_surrogate(const SumArgs a) {
-  voigt_sum_mixed_body<P, true>(a);
+  voigt_sum_mixed_body<P, true, false>(a);
 }
 
 // --------------------------------------------------------- test entry --
@@ -1465,10 +1554,17 @@ hipError_t launch_far_series(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0 || a.n_layers <= 0) return hipSuccess;
   const int64_t n_far = a.n_wg * a.n_layers;
   if (n_far * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
-  if (precision == LBL_FP64)
-    k_far_series_fp64<kPointsFp64><<<(unsigned)n_far, kBlock, 0, s>>>(a);
-  else
-    k_far_series_mixed<kPointsMixed><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
+  // a.work set: the counting variants (same results, per-unit work counts)
+  if (precision == LBL_FP64) {
+    if (a.work)
+      k_far_series_fp64<kPointsFp64, true><<<(unsigned)n_far, kBlock, 0, s>>>(a);
+    else
+      k_far_series_fp64<kPointsFp64, false><<<(unsigned)n_far, kBlock, 0, s>>>(a);
+  } else if (a.work) {
+    k_far_series_mixed<kPointsMixed, true><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
+  } else {
+    k_far_series_mixed<kPointsMixed, false><<<(unsigned)n_far, kMixedBlock, 0, s>>>(a);
+  }
   return hipGetLastError();
 }
 
@@ -1476,12 +1572,18 @@ hipError_t launch_voigt_sum(const SumArgs& a, int precision, hipStream_t s) {
   if (a.n_wg <= 0) return hipSuccess;
   const int64_t n_blocks = a.n_wg * a.layer_groups;
   if (a.layer_groups < 1 || n_blocks * kBlock > (int64_t)UINT32_MAX) return hipErrorInvalidConfiguration;
-  if (precision == LBL_FP64)
-    k_voigt_sum_fp64<kPointsFp64><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
-  else if (precision == LBL_SURROGATE)
+  if (precision == LBL_FP64) {
+    if (a.work)
+      k_voigt_sum_fp64<kPointsFp64, true><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
+    else
+      k_voigt_sum_fp64<kPointsFp64, false><<<(unsigned)n_blocks, kBlock, 0, s>>>(a);
+  } else if (precision == LBL_SURROGATE) {
     k_voigt_sum_surrogate<kPointsMixed><<<(unsigned)n_blocks, kMixedBlock, 0, s>>>(a);
-  else
-    k_voigt_sum_mixed<kPointsMixed><<<(unsigned)n_blocks, kMixedBlock, 0, s>>>(a);
+  } else if (a.work) {
+    k_voigt_sum_mixed<kPointsMixed, true><<<(unsigned)n_blocks, kMixedBlock, 0, s>>>(a);
+  } else {
+    k_voigt_sum_mixed<kPointsMixed, false><<<(unsigned)n_blocks, kMixedBlock, 0, s>>>(a);
+  }
   return hipGetLastError();
 }
 

@@ -432,6 +432,20 @@ class OpticalDepthProblem:
         return {"near_evals": near_evals, "slice_pairs": slice_pairs, "wave_pairs": wave_pairs,
                 "far_pairs": slice_pairs + wave_pairs}
 
+    def kernel_work(self, precision: str = "mixed") -> dict:
+        """The line-sum kernels' own work counts for one run of this problem
+        (their counting variants, lbl_enable_work_count): near-field point
+        evaluations, series pairs per level, the records each level passes on
+        or evaluates point by point, queued near-centre pairs and core points.
+        ``count_series_work`` predicts the first three from the host; the
+        tests hold the two to equality."""
+        _capi.enable_work_count(True)
+        try:
+            self.run(precision, mus=(1.0,))
+            return _capi.last_work_count()
+        finally:
+            _capi.enable_work_count(False)
+
     # ------------------------------------------------------------------ run --
     def workspace(self, precision: int):
         torch = _torch()

@@ -187,6 +187,58 @@ def test_inclusion_edges_are_inclusive():
     assert st["evals"] == 1
 
 
+def test_evaluation_count_brute_force():
+    """stats['evals'] (the work unit of the headline rate) against a plain
+    loop over (layer, group, tile, line) with the reference's tile-level
+    rule (forward.py:295-304): a line counts for every point of a tile when
+    its shifted centre lies in [tile first - cut, tile last + cut]; groups
+    with a zero column are skipped (forward.py:253-255).  Tiles of 7 points
+    over a 25-point grid (a short last tile), two gases, one of them absent
+    from a layer, centres on and one ulp outside a tile edge."""
+    rng = np.random.default_rng(11)
+    n = 30
+    L = _lines(n, seed=11)
+    L["nu0"] = rng.uniform(699.5, 700.8, n)
+    nu = O.grid(700.0, 700.24, 0.01)                # 25 points
+    cut = 0.2
+    # a CO2 line whose shifted centre is exactly tile 0's first point - cut
+    # (it is in), and one an ulp below it (it is out) in the top layer
+    L["shift"][:2] = 0.0
+    L["nu0"][0] = nu[0] - cut
+    L["nu0"][1] = np.nextafter(nu[0] - cut, 0.0)
+    L["mol_id"] = np.where(np.arange(n) % 3 == 0, 1, 2).astype(np.int32)
+    L["mol_id"][:2] = 2
+    L["iso_id"] = np.ones(n, np.int32)
+    T = np.arange(70.0, 401.0)
+    tips = {(1, 1): (T, T), (2, 1): (T, 286.1 * (T / 296) ** 1.5)}
+    mm = {(1, 1): 18.01, (2, 1): 43.99}
+    vmr = {"CO2": np.full(4, 4e-4), "H2O": np.array([0.0, 0.0, 1e-3, 2e-3])}
+    pa, tl, vl, col = O.build_layers(np.array([10.0, 100.0, 500.0, 1000.0]),
+                                     np.array([220.0, 240.0, 270.0, 290.0]), vmr)
+    assert col["H2O"][0] == 0.0 and col["H2O"][1] > 0.0   # layer 0 skips H2O
+    st = {}
+    O.optical_depth(nu, L, tips, mm, pa, tl, vl, col, cut, 0.07, stats=st)
+    ts = 7
+    expect = 0
+    for k in range(len(pa)):
+        for mol, gas in ((1, "H2O"), (2, "CO2")):
+            if col[gas][k] == 0.0:
+                continue
+            for t0 in range(0, nu.size, ts):
+                t1 = min(t0 + ts, nu.size)
+                for i in range(n):
+                    if L["mol_id"][i] != mol:
+                        continue
+                    p_self = vl[gas][k] * pa[k]
+                    c = L["nu0"][i] + L["shift"][i] * (pa[k] - p_self)
+                    if nu[t0] - cut <= c <= nu[t1 - 1] + cut:
+                        expect += t1 - t0
+    assert O.tile_samples(nu, 0.07) == ts
+    assert st["evals"] == expect
+    # the edge line is counted in every layer for tile 0, its neighbour never
+    assert expect > 0
+
+
 # ── integrated path vs the reference's own outputs ────────────────────────────
 
 @pytest.mark.parametrize("name", ["co2_mini", "three_gas_edge", "tile1"])
