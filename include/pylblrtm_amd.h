@@ -271,6 +271,10 @@ int lbl_last_timings(double *ms_line_params, double *ms_voigt_sum);
  * (k_far_series_*) before the near-field line sum (k_voigt_sum_*, with the
  * layer-group epilogue); these are their durations in ms. */
 int lbl_last_timings_split(double *ms_far_series, double *ms_near_sum);
+/* The part of ms_near_sum spent in the layer-group epilogue (the sum of the
+ * groups' partial optical depths and T = exp(-tau/mu) over the grid; 0 when
+ * the line sum writes T itself).  No counterpart in the reference. */
+int lbl_last_timing_epilogue(double *ms_epilogue);
 
 /* Work counting (no counterpart in the reference; for measurement and for
  * tests of the work counts).  When enabled, lbl_optical_depth (LBL_FP64 and

@@ -22,7 +22,8 @@ EXPORTED = ("lbl_workspace_bytes", "lbl_optical_depth", "lbl_voigt_eval", "lbl_l
             "lbl_transmittance_vjp", "lbl_mlp_num_params", "lbl_mlp_make_data", "lbl_mlp_pack",
             "lbl_mlp_train_workspace_bytes", "lbl_mlp_train_step", "lbl_mlp_eval",
             "lbl_optical_depth_surrogate", "lbl_par_read", "lbl_optical_depth_noalloc",
-            "lbl_grid_from_nu", "lbl_enable_work_count", "lbl_last_work_count")
+            "lbl_grid_from_nu", "lbl_enable_work_count", "lbl_last_work_count",
+            "lbl_last_timing_epilogue")
 
 _vp = ctypes.c_void_p
 
@@ -128,6 +129,8 @@ def load() -> ctypes.CDLL:
     lib.lbl_grid_from_nu.argtypes = [_vp, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                      P(LblGrid)]
     lib.lbl_grid_from_nu.restype = ctypes.c_int
+    lib.lbl_last_timing_epilogue.argtypes = [P(ctypes.c_double)]
+    lib.lbl_last_timing_epilogue.restype = ctypes.c_int
     lib.lbl_enable_work_count.argtypes = [ctypes.c_int]
     lib.lbl_enable_work_count.restype = None
     lib.lbl_last_work_count.argtypes = [P(ctypes.c_uint64), ctypes.c_int32]
@@ -202,3 +205,10 @@ def last_timings_split():
     a, b = ctypes.c_double(0), ctypes.c_double(0)
     load().lbl_last_timings_split(ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def last_timing_epilogue() -> float:
+    """ms of the layer-group epilogue (inside last_timings_split()[1])."""
+    a = ctypes.c_double(0)
+    load().lbl_last_timing_epilogue(ctypes.byref(a))
+    return a.value

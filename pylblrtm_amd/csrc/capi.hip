@@ -18,10 +18,11 @@ thread_local char g_last_error[512] = "";
 // the caller's stream (one event set per device, created on first use).
 constexpr int kMaxDevices = 64;
 bool g_timing = false;
-hipEvent_t g_events[kMaxDevices][5];
+constexpr int kEvents = 6;
+hipEvent_t g_events[kMaxDevices][kEvents];
 bool g_events_made[kMaxDevices] = {};
 thread_local double g_ms_line_params = 0.0, g_ms_voigt_sum = 0.0;
-thread_local double g_ms_far = 0.0, g_ms_near = 0.0;
+thread_local double g_ms_far = 0.0, g_ms_near = 0.0, g_ms_epilogue = 0.0;
 // Work counting (lbl_enable_work_count): the line sums run their counting
 // variants and the counts of the calling thread's last call are kept here.
 bool g_count = false;
@@ -33,7 +34,7 @@ hipError_t timing_events(hipEvent_t** ev) {
   if (e != hipSuccess) return e;
   if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
   if (!g_events_made[dev]) {
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < kEvents; ++i) {
       e = hipEventCreate(&g_events[dev][i]);
       if (e != hipSuccess) return e;
     }
@@ -201,6 +202,11 @@ void lbl_enable_timing(int on) { g_timing = on != 0; }
 int lbl_last_timings(double* ms_line_params, double* ms_voigt_sum) {
   if (ms_line_params) *ms_line_params = g_ms_line_params;
   if (ms_voigt_sum) *ms_voigt_sum = g_ms_voigt_sum;
+  return LBL_OK;
+}
+
+int lbl_last_timing_epilogue(double* ms_epilogue) {
+  if (ms_epilogue) *ms_epilogue = g_ms_epilogue;
   return LBL_OK;
 }
 
@@ -384,6 +390,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   HIP_TRY(lbl::launch_far_series(sa, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[4], s));
   HIP_TRY(lbl::launch_voigt_sum(sa, precision, s));
+  if (ev) HIP_TRY(hipEventRecord(ev[5], s));
   HIP_TRY(lbl::launch_tau_epilogue(sa, precision, s));
   if (ev) HIP_TRY(hipEventRecord(ev[3], s));
 
@@ -403,6 +410,9 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
     HIP_TRY(hipEventElapsedTime(&n, ev[4], ev[3]));
     g_ms_far = f;
     g_ms_near = n;
+    float ep = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ep, ev[5], ev[3]));
+    g_ms_epilogue = ep;
   }
   if (code != lbl::kErrNone && code < (1u << lbl::kErrGroupBits)) return invalid_index(code);
   if (code != lbl::kErrNone) {
