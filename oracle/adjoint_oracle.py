@@ -42,12 +42,25 @@ def forward(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus
     return np.stack([np.exp(-tau / m) for m in mus])
 
 
+def weight(Tm, G, mus):
+    """W(nu) = dL/d tau(nu) = -sum_m G_m T_m / mu_m for L = sum G * T."""
+    return -np.sum(np.asarray(G) * np.asarray(Tm) / np.asarray(mus, np.float64)[:, None], axis=0)
+
+
 def vjp(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus, G):
     """(dL/dT_k [n_layers], {gas: dL/dv_{k,gas} [n_layers]}) for L = sum G * T."""
+    Tm = forward(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus)
+    return vjp_layers(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width,
+                      weight(Tm, G, mus), range(len(p_atm)))
+
+
+def vjp_layers(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, W, layers,
+               block: int = 256):
+    """The layers ``layers`` of vjp for a given W(nu) (``weight``); the other
+    entries stay 0.  Layers are independent given W, so a caller may split
+    them over processes.  Lines are taken ``block`` at a time per tile."""
     gases = list(vmr)
     col = {g: np.asarray(vmr[g]) * air for g in gases}
-    Tm = forward(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus)
-    W = -np.sum(G * Tm / np.asarray(mus)[:, None], axis=0)
     nl = len(p_atm)
     gT = np.zeros(nl)
     gV = {g: np.zeros(nl) for g in gases}
@@ -55,7 +68,12 @@ def vjp(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus, G)
     bounds = O.tile_bounds(nu, ts)
     groups = O.prepare_groups(nu, lines, tips, mm, p_atm, t_lay, gases, cutoff)
     sl, sp = O.SQRT_LN2, math.sqrt(math.pi)
-    for k in range(nl):
+
+    def in_tile(c, t0, t1):
+        sel = np.flatnonzero((c >= nu[t0] - cutoff) & (c <= nu[t1 - 1] + cutoff))
+        return [sel[b0:b0 + block] for b0 in range(0, sel.size, block)]
+
+    for k in layers:
         T, P = float(t_lay[k]), float(p_atm[k])
         R = O.radiation_term(nu, T)
         u_grid = O.C2 * nu / T
@@ -74,8 +92,7 @@ def vjp(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus, G)
                                              g["q_ratio"][k], g["molar_mass"])
                 A1, s1, y1 = S * O.VOIGT_K / ad, sl / ad, sl * gl / ad
                 for t0, t1 in bounds:
-                    m = (c >= nu[t0] - cutoff) & (c <= nu[t1 - 1] + cutoff)
-                    if m.any():
+                    for m in in_tile(c, t0, t1):
                         x = (nu[t0:t1, None] - c[m]) * s1[m]
                         K = wofz(x + 1j * y1[m]).real
                         gV[gas][k] += np.sum((W[t0:t1] * R[t0:t1])[:, None] * A1[m] * K)
@@ -101,18 +118,16 @@ def vjp(nu, lines, tips, mm, p_atm, t_lay, vmr, air, cutoff, tile_width, mus, G)
             gscale = (O.T_REF / T) ** n_air
             dyv = sl * gscale * P * (lines["g_self"][idx] - lines["g_air"][idx]) / ad
             for t0, t1 in bounds:
-                m = (c >= nu[t0] - cutoff) & (c <= nu[t1 - 1] + cutoff)
-                if not m.any():
-                    continue
-                x = (nu[t0:t1, None] - c[m]) * s[m]
-                w = wofz(x + 1j * y[m])
-                K, L = w.real, w.imag
-                Kx = -2.0 * (x * K - y[m] * L)
-                Ky = 2.0 * (x * L + y[m] * K) - 2.0 / sp
-                AK = A[m] * K
-                lay[t0:t1] += AK.sum(1)
                 wr = (W[t0:t1] * R[t0:t1])[:, None]
-                gT[k] += np.sum(wr * (at[m] * AK + A[m] * Kx * (-0.5 / T) * x + A[m] * Ky * dyt[m]))
-                gV[gas][k] += np.sum(wr * (av[m] * AK + A[m] * Kx * bxv[m] + A[m] * Ky * dyv[m]))
+                for m in in_tile(c, t0, t1):
+                    x = (nu[t0:t1, None] - c[m]) * s[m]
+                    w = wofz(x + 1j * y[m])
+                    K, L = w.real, w.imag
+                    Kx = -2.0 * (x * K - y[m] * L)
+                    Ky = 2.0 * (x * L + y[m] * K) - 2.0 / sp
+                    AK = A[m] * K
+                    lay[t0:t1] += AK.sum(1)
+                    gT[k] += np.sum(wr * (at[m] * AK + A[m] * Kx * (-0.5 / T) * x + A[m] * Ky * dyt[m]))
+                    gV[gas][k] += np.sum(wr * (av[m] * AK + A[m] * Kx * bxv[m] + A[m] * Ky * dyv[m]))
         gT[k] += np.sum(W * R * dlnR_grid * lay)
     return gT, gV

@@ -54,47 +54,65 @@ def layer_transmittance(problem: OpticalDepthProblem, t_layers: torch.Tensor,
 
 class BatchTransmittance(torch.autograd.Function):
     """B profiles in one autograd node: T[b] [n_mu, n_nu] for
-    t_layers[b] [n_layers] and vmr_layers[b] [n_gases, n_layers].  Each
-    profile's forward keeps its transmittance and per-layer optical depth
-    (B x n_layers x n_nu floats) for the backward; the profiles run one after
-    another, each launch filling the GPU on its own."""
+    t_layers[b] [n_layers] and vmr_layers[b] [n_gases, n_layers].  The
+    profiles run one after another, each launch filling the GPU on its own.
+
+    With ``save_layers`` each profile's per-layer optical depth (B x n_layers
+    x n_nu floats, 12.8 GB for 64 profiles of 50 layers on 1,000,001 points)
+    is kept for the backward; without it only T (B x n_mu x n_nu) is kept and
+    the backward runs each profile's forward again for its layer optical
+    depth (one forward per profile more, n_layers x n_nu floats at a time)."""
 
     @staticmethod
     def forward(ctx, t_layers: torch.Tensor, vmr_layers: torch.Tensor,
-                problem: OpticalDepthProblem, mus: Sequence[float] = (1.0,)):
+                problem: OpticalDepthProblem, mus: Sequence[float] = (1.0,),
+                save_layers: bool = True):
         mus = tuple(mus)
         Ts, ods = [], []
         for b in range(t_layers.shape[0]):
             problem.set_layers(t_layers[b].detach(), vmr_layers[b].detach())
-            out = problem.run("mixed", mus=mus, want_layers=True)
+            out = problem.run("mixed", mus=mus, want_layers=save_layers)
             Ts.append(out["transmittance"])
-            ods.append(out["optical_depth_layers"])
+            if save_layers:
+                ods.append(out["optical_depth_layers"])
+            del out
         T = torch.stack(Ts)
+        del Ts
         ctx.problem, ctx.mus = problem, mus
-        ctx.save_for_backward(t_layers.detach().clone(), vmr_layers.detach().clone(), T,
-                              torch.stack(ods))
+        saved = [t_layers.detach().clone(), vmr_layers.detach().clone(), T]
+        if save_layers:
+            saved.append(torch.stack(ods))
+        ctx.save_for_backward(*saved)
         return T
 
     @staticmethod
     def backward(ctx, grad_T: torch.Tensor):
-        t_layers, vmr_layers, T, ods = ctx.saved_tensors
+        t_layers, vmr_layers, T = ctx.saved_tensors[:3]
+        ods = ctx.saved_tensors[3] if len(ctx.saved_tensors) > 3 else None
         prob = ctx.problem
         gts, gvs = [], []
         for b in range(t_layers.shape[0]):
             prob.set_layers(t_layers[b], vmr_layers[b])
-            gt, gv = prob.vjp(T[b], ods[b], grad_T[b], ctx.mus)
+            if ods is not None:
+                od = ods[b]
+            else:
+                od = prob.run("mixed", mus=ctx.mus, want_layers=True)["optical_depth_layers"]
+            gt, gv = prob.vjp(T[b], od, grad_T[b], ctx.mus)
+            del od
             gts.append(gt)
             gvs.append(gv)
         return (torch.stack(gts).to(t_layers.dtype), torch.stack(gvs).to(vmr_layers.dtype),
-                None, None)
+                None, None, None)
 
 
 def batch_layer_transmittance(problem: OpticalDepthProblem, t_layers: torch.Tensor,
                               vmr_layers: torch.Tensor,
-                              mus: Sequence[float] = (1.0,)) -> torch.Tensor:
+                              mus: Sequence[float] = (1.0,),
+                              save_layers: bool = True) -> torch.Tensor:
     """T [B, n_mu, n_nu] for a batch of layer states [B, n_layers] and
-    [B, n_gases, n_layers]."""
-    return BatchTransmittance.apply(t_layers, vmr_layers, problem, tuple(mus))
+    [B, n_gases, n_layers]; ``save_layers=False`` trades one more forward
+    per profile in the backward for B x n_layers x n_nu floats of memory."""
+    return BatchTransmittance.apply(t_layers, vmr_layers, problem, tuple(mus), bool(save_layers))
 
 
 def level_layers(t_levels: torch.Tensor, vmr_levels: torch.Tensor):
