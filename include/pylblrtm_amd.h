@@ -110,8 +110,10 @@ typedef struct lbl_grid {
  * then add) equals every nu[i] bit for bit, so the kernels' grid is the
  * caller's.  The arrays compute_transmittance and run_forward build
  * (forward.py:141-142, config.py:246-247) always have one; an array that is
- * not of that form (unevenly spaced, decreasing, non-finite) returns
- * LBL_EINVAL.  Host code only. */
+ * not of that form (unevenly spaced, decreasing, non-finite) or longer than
+ * 2^40 points returns LBL_EINVAL.  (tile_samples is capped at 2^62; the
+ * entry points below treat any tile longer than the grid as the whole
+ * grid.)  Host code only. */
 int lbl_grid_from_nu(const double *nu, int64_t n_nu, double tile_width_cm1,
                      double line_cutoff_cm1, lbl_grid_t *grid);
 

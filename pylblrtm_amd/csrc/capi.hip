@@ -114,10 +114,17 @@ struct Layout {
   int32_t slices_per_tile, layer_groups;
 };
 
+// Largest grid the size arithmetic below handles without overflow (with at
+// most 2^19 layers and 2^31 lines every product stays below 2^63 bytes).
+constexpr int64_t kMaxPoints = (int64_t)1 << 40;
+
 bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl_grid_t* grid,
                  int precision, Layout* L, bool with_grad = false) {
   if (!lines || !layers || !grid) return false;
-  if (grid->tile_samples <= 0 || grid->n_nu <= 0) return false;
+  if (grid->tile_samples <= 0 || grid->n_nu <= 0 || grid->n_nu > kMaxPoints) return false;
+  if (lines->n_lines < 0 || lines->n_lines > INT32_MAX) return false;
+  if (layers->n_layers < 0 || layers->n_layers > (1 << 19)) return false;
+  if (precision != LBL_FP64 && precision != LBL_MIXED && precision != LBL_SURROGATE) return false;
   const size_t nrec = (size_t)std::max<int64_t>(lines->n_lines, 0) *
                       (size_t)std::max<int32_t>(layers->n_layers, 0);
   const bool fp64 = precision == LBL_FP64;
@@ -172,6 +179,8 @@ int check_inputs(const lbl_lines_t* lines, const lbl_groups_t* groups, const lbl
     return fail(LBL_EINVAL, "n_groups %d exceeds %d", groups->n_groups, lbl::kMaxGroups);
   if (grid->n_nu < 1 || grid->tile_samples < 1)
     return fail(LBL_EINVAL, "empty grid or tile");
+  if (grid->n_nu > kMaxPoints)
+    return fail(LBL_EINVAL, "n_nu %lld exceeds 2^40 points", (long long)grid->n_nu);
   if (!(grid->step > 0.0) || !isfinite(grid->nu_min) || !isfinite(grid->cutoff))
     return fail(LBL_EINVAL, "invalid grid parameters");
   if (lines->n_lines > 0) {
@@ -189,6 +198,15 @@ int check_inputs(const lbl_lines_t* lines, const lbl_groups_t* groups, const lbl
   if (lines->n_lines > 0 && layers->n_layers > 0 && layers->n_gases < 1)
     return fail(LBL_EINVAL, "n_gases must be >= 1 when there are lines and layers");
   return LBL_OK;
+}
+
+// A tile at least as long as the grid is the whole grid: the kernels get
+// tile_samples = min(tile_samples, n_nu), the same tiling without the empty
+// workgroups of the part of the tile past the grid's end.
+lbl_grid_t effective_grid(const lbl_grid_t* grid) {
+  lbl_grid_t g = *grid;
+  if (g.tile_samples > g.n_nu) g.tile_samples = g.n_nu;
+  return g;
 }
 
 }  // namespace
@@ -226,6 +244,8 @@ int lbl_last_work_count(uint64_t* counts, int32_t n) {
 
 int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
                         const lbl_grid_t* grid, int precision, size_t* bytes) {
+  lbl_grid_t eg{};
+  if (grid) { eg = effective_grid(grid); grid = &eg; }
   Layout L;
   if (!bytes || !make_layout(lines, layers, grid, precision, &L))
     return fail(LBL_EINVAL, "invalid arguments to lbl_workspace_bytes");
@@ -437,6 +457,8 @@ int lbl_optical_depth(const lbl_lines_t* lines, const lbl_groups_t* groups,
     return fail(LBL_EINVAL, "precision LBL_SURROGATE needs lbl_optical_depth_surrogate");
   int rc = check_call(lines, groups, layers, grid, precision, out, nullptr);
   if (rc != LBL_OK) return rc;
+  const lbl_grid_t eg = effective_grid(grid);
+  grid = &eg;
   return optical_depth_impl(lines, groups, layers, grid, precision, out, nullptr, workspace,
                             workspace_bytes, err_layer, err_group, stream, true);
 }
@@ -450,6 +472,8 @@ int lbl_optical_depth_noalloc(const lbl_lines_t* lines, const lbl_groups_t* grou
     return fail(LBL_EINVAL, "lbl_optical_depth_noalloc: precision must be LBL_FP64 or LBL_MIXED");
   int rc = check_call(lines, groups, layers, grid, precision, out, nullptr);
   if (rc != LBL_OK) return rc;
+  const lbl_grid_t eg = effective_grid(grid);
+  grid = &eg;
   Layout L;
   make_layout(lines, layers, grid, precision, &L);
   if (L.total > kScratchBytes)
@@ -472,12 +496,16 @@ int lbl_optical_depth_surrogate(const lbl_lines_t* lines, const lbl_groups_t* gr
   g_last_error[0] = '\0';
   int rc = check_call(lines, groups, layers, grid, LBL_SURROGATE, out, sur);
   if (rc != LBL_OK) return rc;
+  const lbl_grid_t eg = effective_grid(grid);
+  grid = &eg;
   return optical_depth_impl(lines, groups, layers, grid, LBL_SURROGATE, out, sur, workspace,
                             workspace_bytes, err_layer, err_group, stream, true);
 }
 
 int lbl_vjp_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
                             const lbl_grid_t* grid, size_t* bytes) {
+  lbl_grid_t eg{};
+  if (grid) { eg = effective_grid(grid); grid = &eg; }
   Layout L;
   if (!bytes || !make_layout(lines, layers, grid, LBL_MIXED, &L, true))
     return fail(LBL_EINVAL, "invalid arguments to lbl_vjp_workspace_bytes");
@@ -494,6 +522,8 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   g_last_error[0] = '\0';
   int rc = check_inputs(lines, groups, layers, grid, LBL_MIXED);
   if (rc != LBL_OK) return rc;
+  const lbl_grid_t eg = effective_grid(grid);
+  grid = &eg;
   if (n_mu < 1 || n_mu > lbl::kMaxMu || !mu || !transmittance || !optical_depth_layers ||
       !grad_transmittance || !grad_t || !grad_vmr)
     return fail(LBL_EINVAL, "vjp: need 1..%d viewing cosines and all gradient buffers",
@@ -677,6 +707,7 @@ int lbl_grid_from_nu(const double* nu, int64_t n_nu, double tile_width_cm1,
                      double line_cutoff_cm1, lbl_grid_t* grid) {
   g_last_error[0] = '\0';
   if (!nu || !grid || n_nu < 1) return fail(LBL_EINVAL, "empty grid");
+  if (n_nu > kMaxPoints) return fail(LBL_EINVAL, "n_nu %lld exceeds 2^40 points", (long long)n_nu);
   if (!isfinite(tile_width_cm1) || !(tile_width_cm1 > 0.0) || !isfinite(line_cutoff_cm1))
     return fail(LBL_EINVAL, "invalid tile width or cutoff");
   for (int64_t i = 0; i < n_nu; ++i)
@@ -692,7 +723,9 @@ int lbl_grid_from_nu(const double* nu, int64_t n_nu, double tile_width_cm1,
   grid->nu_min = nu[0];
   grid->step = step;
   grid->n_nu = n_nu;
-  grid->tile_samples = ts >= 1.0 ? (int64_t)ts : 1;
+  // (a step so small that the tile exceeds 2^62 points keeps 2^62: either way
+  // the tile holds the whole grid, and the conversion stays in range)
+  grid->tile_samples = !(ts >= 1.0) ? 1 : ts < 0x1p62 ? (int64_t)ts : ((int64_t)1 << 62);
   grid->cutoff = line_cutoff_cm1;
   grid->shift_bound = 0.0;
   return LBL_OK;
