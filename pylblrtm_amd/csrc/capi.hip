@@ -150,8 +150,8 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->work = off; off = align_up(off + lbl::kWorkSlots * sizeof(unsigned long long));
   // partial tau per layer group (float sum + compensation, or double): 8 B
   L->tau_part = off;
-  // (the mixed sums keep their column totals here even with one group)
-  if (L->layer_groups > 1 || (!fp64 && !with_grad))
+  // (the line sums keep their column totals here even with one group)
+  if (L->layer_groups > 1 || !with_grad)
     off = align_up(off + (size_t)std::max<int32_t>(L->layer_groups, 1) * (size_t)grid->n_nu * 8);
   // far-field sum per (layer, point): FP32 (mixed modes) or FP64
   L->far = off;

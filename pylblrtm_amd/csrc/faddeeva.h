@@ -42,6 +42,12 @@ constexpr double kInvSqrtPi = 0.56418958354775628695;
 constexpr double kFp64Gh2R2 = 490000.0; // |z| >= 700: 2 nodes, rel err <= 1.04e-11
 constexpr double kFp64Gh4R2 = 1225.0;   // |z| >= 35 : 4 nodes, rel err <= 5.1e-12
 constexpr double kFp64Gh8R2 = 225.0;    // |z| >= 15 : 8 nodes, rel err <= 1e-15
+// |z| >= 7: 10 nodes, rel err <= 1e-11 for y >= 1e-8 (measured against
+// scipy's wofz: the 1e-11 boundary lies at |z| = 6.71 for y in [1e-3, 1],
+// 6.72 for [1e-5, 1e-3), 6.98 for [1e-8, 1e-5) and below it for larger y;
+// nearer the real axis exp(-x^2), which no node sum holds, keeps it out)
+constexpr double kFp64Gh10R2 = 49.0;
+constexpr double kFp64Gh10Ymin = 1e-8;
 constexpr float kF32Gh2R2 = 625.0f;     // |z| >= 25: 2 nodes, rel err <= 6.41e-6
 constexpr float kF32Gh4R2 = 64.0f;      // |z| >= 8 : 4 nodes, rel err <= 1e-6
 
@@ -93,6 +99,13 @@ constexpr double kGh8T[4] = {0.3811869902073221, 1.1571937124467802, 1.981656756
 constexpr double kGh8W[4] = {0.6611470125582415, 0.20780232581489183, 0.017077983007413467,
                              0.00019960407221136783};
 
+// 10-point Gauss-Hermite nodes and weights (t > 0), refined by Newton steps on
+// H_10 in 50-digit arithmetic.
+constexpr double kGh10T[5] = {0.34290132722370461, 1.0366108297895137, 1.7566836492998818,
+                              2.5327316742327898, 3.4361591188377376};
+constexpr double kGh10W[5] = {0.6108626337353258, 0.24013861108231469, 0.033874394455481063,
+                              0.0013436457467812327, 7.6404328552326206e-6};
+
 // exp(-t^2) at the trapezoid nodes: t = n h (n >= 1) and t = (n + 1/2) h (n >= 0).
 constexpr double kCr64H = 0.5;
 constexpr double kCr64E0[12] = {
@@ -127,6 +140,30 @@ __device__ __forceinline__ double gh_pair64(double r, double xmy, double fxy, do
   return w * (r + t2) * rcp64_1nr(fma(u, u, fxy));
 }
 
+// sum_k W_k (r + t_k^2) / D_k over N node pairs as a single quotient: every
+// term is positive, so the running numerator and denominator (the product of
+// the D_k) keep their relative accuracy, and one reciprocal serves all pairs.
+// The caller keeps the D_k away from 0 and the product inside the FP64 range
+// (D_k = (r - t_k^2)^2 + 4 y^2 t_k^2: for |z| >= 7 and t_k <= 3.44 each lies
+// in [1.3e3, ~r^2]).
+template <int N>
+__device__ __forceinline__ double gh_sum64(double r, double xmy, double fxy, const double (&T)[N],
+                                           const double (&W)[N]) {
+  double t2 = T[0] * T[0];
+  double u = xmy - t2;
+  double den = fma(u, u, fxy);
+  double num = W[0] * (r + t2);
+#pragma unroll
+  for (int k = 1; k < N; ++k) {
+    t2 = T[k] * T[k];
+    u = xmy - t2;
+    const double D = fma(u, u, fxy);
+    num = fma(num, D, W[k] * (r + t2) * den);
+    den *= D;
+  }
+  return num * rcp64_1nr(den);
+}
+
 // 2-node form, |z| >= 700: K = y (w + 1) / (sqrt(pi) (w^2 + 2 y^2)), w = x^2 + y^2 - 1/2.
 __device__ __forceinline__ double K_gh2_64(double x2, double y) {
   const double y2 = y * y;
@@ -151,7 +188,13 @@ __device__ __forceinline__ double K_gh8_64(double x2, double y) {
   return 2.0 * y * kInvPi * s;
 }
 
-// Trapezoid + residue, |z| < 15.
+// 10-node form, |z| >= 7 (y >= 1e-8).
+__device__ __forceinline__ double K_gh10_64(double x2, double y) {
+  const double y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+  return 2.0 * y * kInvPi * gh_sum64<5>(r, xmy, fxy, kGh10T, kGh10W);
+}
+
+// Trapezoid + residue, |z| < 7 (and |z| < 15 for y < 1e-8).
 __device__ inline double K_cr64(double x, double y) {
   const double h = kCr64H;
   const double q = fabs(x) * (1.0 / h);
@@ -159,18 +202,24 @@ __device__ inline double K_cr64(double x, double y) {
   const bool half = !(fr >= 0.25 && fr <= 0.75);
   const double x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
   // One loop for both lattices (nodes (n + 1/2) h or n h, weights selected per
-  // lane; the integer lattice's n = 0 slot has weight 0), so a wavefront whose
-  // lanes chose differently runs it once.  Terms are added in the same order.
+  // lane), so a wavefront whose lanes chose differently runs it once.  The
+  // integer lattice's node t = 0 counts once: its slot has weight 1/2 of the
+  // pair form, (r + 0)/D = 1/r.  The pairs are summed as one quotient (all
+  // terms positive; on the chosen lattice x stays h/4 from every node, so
+  // each D >= 3.9e-3, and for |z| < 15 each D = (r - t^2)^2 + 4 y^2 t^2 <
+  // 1.3e5: 13 factors stay inside the FP64 range).
   const double off = half ? 0.5 : 0.0;
-  double s = 0.0;
+  double num = 0.0, den = 1.0;
 #pragma unroll
   for (int n = 12; n >= 0; --n) {
-    const double w = half ? kCr64E5[n] : (n >= 1 ? kCr64E0[n - 1] : 0.0);
-    s += gh_pair64(r, xmy, fxy, (n + off) * h, w);
+    const double w = half ? kCr64E5[n] : (n >= 1 ? kCr64E0[n - 1] : 0.5);
+    const double t = (n + off) * h, t2 = t * t;
+    const double u = xmy - t2;
+    const double D = fma(u, u, fxy);
+    num = fma(num, D, w * (r + t2) * den);
+    den *= D;
   }
-  s *= 2.0 * y;
-  if (!half) s += y * rcp64_1nr(r);
-  double K = s * h * kInvPi;
+  double K = 2.0 * y * h * kInvPi * num * rcp64_1nr(den);
   // The residue is at most 4M in size: its numerator is at most 2M (1 + rho)
   // and, on the lattice chosen above (cos theta > 0 on the half lattice,
   // <= 0 on the integer one), its denominator is at least 1.  Where 4M is
@@ -181,7 +230,9 @@ __device__ inline double K_cr64(double x, double y) {
     const double rho = exp(-2.0 * kPi * y / h);
     double st, ct, s2, c2;
     sincospi(2.0 * x / h, &st, &ct);
-    sincos(2.0 * x * y, &s2, &c2);
+    // 2xy as a multiple of pi: the reduction is exact, and the two roundings
+    // of the argument move the phase by < 3e-14 for 2xy < 90 (y < pi/h, |x| < 7)
+    sincospi(2.0 * x * y * kInvPi, &s2, &c2);
     const double cphi = ct * c2 + st * s2;  // cos(theta - 2xy)
     K += 2.0 * M * (sg * cphi + rho * c2) * rcp64_1nr(1.0 + 2.0 * sg * rho * ct + rho * rho);
   }
@@ -195,6 +246,7 @@ __device__ inline double K_fp64(double x, double y) {
   if (r >= kFp64Gh2R2) return K_gh2_64(x2, y);
   if (r >= kFp64Gh4R2) return K_gh4_64(x2, y);
   if (r >= kFp64Gh8R2) return K_gh8_64(x2, y);
+  if (r >= kFp64Gh10R2 && y >= kFp64Gh10Ymin) return K_gh10_64(x2, y);
   return K_cr64(x, y);
 }
 

@@ -92,29 +92,33 @@ struct alignas(16) Rec64 {
 static_assert(sizeof(Rec64) == 64, "four 16-byte LDS reads per record");
 typedef __attribute__((address_space(3))) Rec64 lds_rec64;
 // 4 workgroups per CU need under 40 KB of LDS each: 16 KB of records, 16 KB
-// of FP64 partial sums and this many 16-bit queue entries.
-constexpr int kQueueCap64 = 3840;
+// of FP64 partial sums, and per wavefront a queue of (record, lane) entries
+// and a list of core points.
+constexpr int kQueueW64 = 256;    // queue entries per wavefront (the loop drains at kQueueW64 - 128)
+constexpr int kCoreCap64 = 512;   // >= the 64 x 8 points one pass-A round can add
 
 // FP64 line sum.  Same tiling as the mixed kernel: per record and thread
 // the middle x of the thread's P points decides, for all of them at once,
 // between the shifted one-node form and its second-order refinement
 // (faddeeva.h, error <= 1e-11), the 4-node form (|z| >= 35) and the
-// near-centre LDS queue, which the whole workgroup drains at the end of the
-// chunk with the full FP64 dispatch (8-node form and trapezoid+residue),
-// adding the results into the owner's sums with LDS FP64 atomics.
+// near-centre queue.  The wavefront drains its queue when it could not take
+// two more records, in two dense passes: pass A evaluates every queued point
+// with the 10-node form and delivers it to the owner's sums with plain LDS
+// read-modify-writes; the points it cannot hold (|z| < 7, the core) go to a
+// list that pass B evaluates one point per lane with trapezoid + residue.
 template <int P, bool kCount>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
 void k_voigt_sum_fp64(const SumArgs a) {
   // every wavefront works alone on its own near range (k_far_series_fp64 has
   // summed the rest of the tile): its own LDS stage of kStage records, its
-  // own queue drained by itself, no workgroup barriers
+  // own queue and core list drained by itself, no workgroup barriers
   constexpr int kWaves = kBlock / 64;
   constexpr int kStage = 64;
-  constexpr int kQueueCapW = kQueueCap64 / kWaves;
   __shared__ Rec64 s_rec[kWaves][kStage];
-  __shared__ uint16_t s_q[kWaves][kQueueCapW];
+  __shared__ uint16_t s_q[kWaves][kQueueW64];      // (record << 6) | lane
+  __shared__ uint16_t s_ci[kWaves][kCoreCap64];    // (record << 9) | (lane << 3) | point
   __shared__ double s_acc[kBlock * P];
-  __shared__ int s_qn[kWaves];
+  __shared__ uint8_t s_claim[kWaves][64];          // claim byte per owner lane
 
   // (layer group, tile slice), group-major so that an XCD holds neighbouring
   // tiles of one layer range
@@ -147,12 +151,11 @@ void k_voigt_sum_fp64(const SumArgs a) {
   // s_acc[j][thread]: a wavefront's lanes touch consecutive words (no bank conflicts)
   double* const acc = s_acc + threadIdx.x;
   for (int j = 0; j < P; ++j) acc[j * kBlock] = 0.0;
-  if (lane == 0) s_qn[wave] = 0;
 
-  double tau[P];
-#pragma unroll
-  for (int j = 0; j < P; ++j) tau[j] = 0.0;
-  uint64_t n_in = 0, n_queued = 0;   // kCount: records in the window, queued pairs
+  // column totals live in tau_part[grp], read and rewritten once per layer,
+  // so that no registers hold them through the record loops and the drains
+  double* const tau_acc = static_cast<double*>(a.tau_part) + (int64_t)grp * a.n_nu;
+  uint64_t n_in = 0, n_queued = 0, n_core = 0;   // kCount: records in the window, queued pairs, core points
 
   for (int k = k_begin; k < k_end; ++k) {
     double lay[P];
@@ -183,6 +186,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
       __builtin_amdgcn_wave_barrier();
       if constexpr (kCount) n_in += (uint64_t)__popcll(__ballot(inside));
       const int ncnt = cnt;
+      int nq = 0;   // queue entries not yet drained (uniform)
       // Records are read from LDS as 16-byte pieces: (c, s) and (hw1, hs) two
       // records ahead into their own registers (two records per iteration),
       // (c1, ay) at the record and (hw2, hw4) only off the one-node branch.
@@ -287,62 +291,113 @@ void k_voigt_sum_fp64(const SumArgs a) {
             lay[j + 1] = fma(Np[1] * Dp[0], rp, lay[j + 1]);
           }
         }
+        // append the queued lanes' (record, lane) entries; nq advances in
+        // every lane (the loop drains before the queue could overflow)
         const bool queued = near && live;
         const uint64_t msk = __ballot(queued);
         if (msk) {   // uniform
-          // append (q, lane) of the queued lanes with one LDS atomic per wavefront
           const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
               (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-          int qb = 0;
-          if (lane == 0) qb = atomicAdd(&s_qn[wave], (int)__popcll(msk));
-          qb = __builtin_amdgcn_readfirstlane(qb);
-          if constexpr (kCount) n_queued += (uint64_t)__popcll(msk);
-          const int pos = qb + (int)rank;
-          if (queued) {
-            if (pos < kQueueCapW) {
-              s_q[wave][pos] = (uint16_t)((q << 6) | lane);
-            } else {  // queue full: evaluate here
-              const int64_t r = kb + base + q;
-              const double y = ry[r], am = ra[r];
-#pragma unroll 1
-              for (int j = 0; j < P; ++j) {
-                const double v =
-                    am * K_fp64_call((grid_nu(a.nu_min, a.step, i0 + j) - rc) * rsv, y);
-#pragma unroll
-                for (int jj = 0; jj < P; ++jj) lay[jj] += (jj == j) ? v : 0.0;
-              }
-            }
-          }
+          if (queued) s_q[wave][nq + (int)rank] = (uint16_t)((q << 6) | lane);
+          nq += (int)__popcll(msk);
         }
       };
-      u32x4 ha0 = rb[0], ha1 = rb[1], hb0 = rb[4], hb1 = rb[5];
-      const volatile lds_u32x4* pr = rb;
-      asm volatile("" : "+v"(pr));
-      for (int q = 0; q < ncnt; q += 2, pr += 8) {
-        record(q, ha0, ha1, pr + 2);
-        if (q + 2 < ncnt) { ha0 = pr[8]; ha1 = pr[9]; }
-        if (q + 1 < ncnt) {
-          record(q + 1, hb0, hb1, pr + 6);
-          if (q + 3 < ncnt) { hb0 = pr[12]; hb1 = pr[13]; }
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      const int nq = __builtin_amdgcn_readfirstlane(min(s_qn[wave], kQueueCapW));
-      if (nq > 0) {
-        // dense drain by this wavefront: entry = (record, lane); points at
-        // their exact grid values
-        for (int e = lane; e < nq; e += 64) {
-          const uint32_t ent = s_q[wave][e];
-          const int q = (int)(ent >> 6), tt = wave * 64 + (int)(ent & 63);
-          const Rec64& R = s_rec[wave][q];
-          const int64_t r = kb + base + q;
-          const double y = ry[r], am = ra[r];
+      // The queue, drained by this wavefront in two dense passes.  Pass A: one
+      // entry per lane, its P points at their exact grid values in the
+      // 10-node form (<= 1e-11 for |z| >= 7, y >= 1e-8); core points get
+      // weight 0 and go to the core list.  Pass B: the core list, one point
+      // per lane, trapezoid + residue.
+      auto drain = [&](const int n) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
+        __builtin_amdgcn_wave_barrier();
+        int ncore = 0;   // uniform
+        auto core_pass = [&](const int m) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
+          __builtin_amdgcn_wave_barrier();
+          for (int b = lane; b < m; b += 64) {
+            const uint32_t ci = s_ci[wave][b];
+            const int qq = (int)(ci >> 9), t = (int)((ci >> 3) & 63), j = (int)(ci & (P - 1));
+            const u32x4 h0 = rb[4 * qq];   // c, s
+            const int64_t r = kb + base + qq;
+            const double y = ry[r], am = ra[r];
+            const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)(wave * 64 + t) * P + j) -
+                              f64(h0.x, h0.y)) * f64(h0.z, h0.w);
+            atomicAdd(&s_acc[j * kBlock + wave * 64 + t], am * K_cr64(x, y));
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
+          __builtin_amdgcn_wave_barrier();
+        };
+        for (int e0 = 0; e0 < n; e0 += 64) {
+          const int e = e0 + lane;
+          const bool has = e < n;
+          const uint32_t qi = has ? s_q[wave][e] : 0u;
+          const int qq = (int)(qi >> 6), t = (int)(qi & 63);
+          const u32x4 h0 = rb[4 * qq], h2 = rb[4 * qq + 2];   // (c, s), (c1, ay)
+          const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
+          // A (2y/pi) = (2/sqrt(pi)) ay; idle lanes add 0
+          const double cw = has ? 2.0 * kInvSqrtPi * f64(h2.z, h2.w) : 0.0;
+          const double y = has ? ry[kb + base + qq] : 1.0;
+          const double y2 = y * y;
+          const double rcore = y >= kFp64Gh10Ymin ? kFp64Gh10R2 : kFp64Gh8R2;
+          const int64_t ib = s0 + (int64_t)(wave * 64 + t) * P;
+          uint32_t cmask = 0;
+          double v[P];
+#pragma unroll
           for (int j = 0; j < P; ++j) {
-            const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)tt * P + j) - R.c) * R.s;
-            atomicAdd(&s_acc[j * kBlock + tt], am * K_fp64_call(x, y));
+            const double x = (grid_nu(a.nu_min, a.step, ib + j) - rc) * rsv;
+            const double x2 = x * x;
+            const double r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+            const bool core = has && r < rcore;
+            // (a select: at a core point the node sum may be infinite)
+            const double g = gh_sum64<5>(r, xmy, fxy, kGh10T, kGh10W);
+            v[j] = core ? 0.0 : cw * g;
+            cmask |= core ? (1u << j) : 0u;
+          }
+          // Deliver the P values to the owner's sums with plain LDS reads and
+          // writes: in each round every pending lane writes its id to its
+          // owner's claim byte, the lanes that read their own id back hold
+          // distinct owners and update, the rest try again.
+          bool pending = has;
+          while (__ballot(pending)) {   // uniform
+            if (pending) s_claim[wave][t] = (uint8_t)lane;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (pending && s_claim[wave][t] == (uint8_t)lane) {
+              double* const dst = s_acc + wave * 64 + t;
+#pragma unroll
+              for (int j = 0; j < P; ++j) dst[j * kBlock] += v[j];
+              pending = false;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          }
+          // this lane's core points go to the list at an exclusive prefix sum
+          // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
+          const uint32_t cnt8 = __builtin_popcount(cmask);
+          uint32_t pre = 0, total = 0;
+#pragma unroll
+          for (int bit = 0; bit < 4; ++bit) {
+            const uint64_t m = __ballot((cnt8 >> bit) & 1u);
+            pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+            total += (uint32_t)__popcll(m) << bit;
+          }
+          if (total) {   // uniform
+            if (ncore + (int)total > kCoreCap64) {
+              core_pass(ncore);
+              ncore = 0;
+            }
+            int off = ncore + (int)pre;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              if (cmask & (1u << j)) {
+                s_ci[wave][off] = (uint16_t)((uint32_t)qq << 9 | (uint32_t)t << 3 | (uint32_t)j);
+                ++off;
+              }
+            }
+            ncore += (int)total;
+            if constexpr (kCount) n_core += total;
           }
         }
+        if (ncore > 0) core_pass(ncore);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -350,8 +405,36 @@ void k_voigt_sum_fp64(const SumArgs a) {
           lay[j] += acc[j * kBlock];
           acc[j * kBlock] = 0.0;
         }
-        if (lane == 0) s_qn[wave] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      };
+      u32x4 ha0 = rb[0], ha1 = rb[1], hb0 = rb[4], hb1 = rb[5];
+      const volatile lds_u32x4* pr = rb;
+      asm volatile("" : "+v"(pr));
+      int q = 0;
+      for (;;) {
+        for (; q < ncnt; q += 2, pr += 8) {
+          record(q, ha0, ha1, pr + 2);
+          if (q + 2 < ncnt) { ha0 = pr[8]; ha1 = pr[9]; }
+          if (q + 1 < ncnt) {
+            record(q + 1, hb0, hb1, pr + 6);
+            if (q + 3 < ncnt) { hb0 = pr[12]; hb1 = pr[13]; }
+          }
+          if (nq > kQueueW64 - 128) {   // uniform
+            q += 2;
+            pr += 8;
+            break;
+          }
+        }
+        if (nq > 0) {   // uniform
+          if constexpr (kCount) n_queued += (uint64_t)nq;
+          drain(nq);
+          nq = 0;
+          // the next two records' headers again from LDS (not held in
+          // registers through the drain)
+          if (q < ncnt) { ha0 = pr[0]; ha1 = pr[1]; }
+          if (q + 1 < ncnt) { hb0 = pr[4]; hb1 = pr[5]; }
+        }
+        if (q >= ncnt) break;
       }
     }
     const double T = a.t_k[k];
@@ -364,7 +447,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
         const double nu = grid_nu(a.nu_min, a.step, i);
         const double v = (lay[j] + far_k[i]) * (nu * tanh(0.5 * kC2 * nu / T));
         bad |= !isfinite(v);
-        tau[j] += v;
+        tau_acc[i] = (k > k_begin ? tau_acc[i] : 0.0) + v;
         if (a.tau_layers) static_cast<double*>(a.tau_layers)[(int64_t)k * a.n_nu + i] = v;
       }
     }
@@ -374,13 +457,12 @@ void k_voigt_sum_fp64(const SumArgs a) {
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
     if (i < t1) {
-      if (a.layer_groups > 1) {
-        static_cast<double*>(a.tau_part)[(int64_t)grp * a.n_nu + i] = tau[j];
-        continue;
-      }
+      if (k_begin == k_end) tau_acc[i] = 0.0;   // no layers in this group
+      if (a.layer_groups > 1) continue;   // k_tau_epilogue64 adds the groups
+      const double t = tau_acc[i];
       for (int m = 0; m < a.n_mu; ++m)
-        static_cast<double*>(a.trans)[(int64_t)m * a.n_nu + i] = exp(-tau[j] * a.inv_mu[m]);
-      if (a.tau) static_cast<double*>(a.tau)[i] = tau[j];
+        static_cast<double*>(a.trans)[(int64_t)m * a.n_nu + i] = exp(-t * a.inv_mu[m]);
+      if (a.tau) static_cast<double*>(a.tau)[i] = t;
     }
   }
   if constexpr (kCount) {
@@ -389,6 +471,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
       const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
       count_add(a.work, kWorkNearEvals, n_in * w_pts);
       count_add(a.work, kWorkQueued, n_queued);
+      count_add(a.work, kWorkCorePoints, n_core);
     }
   }
 }
