@@ -210,3 +210,51 @@ def test_two_node_constants_from_c1_are_exact():
     assert np.array_equal((c1 + np.float32(1.0)).astype(np.float32), (y2 - np.float32(0.5)).astype(np.float32))
     d2 = (np.float32(2.0) * c1 + np.float32(3.0)).astype(np.float32)
     assert np.array_equal(d2, (np.float32(2.0) * y2).astype(np.float32))
+
+
+# 10-point Gauss-Hermite nodes and weights as faddeeva.h holds them (t > 0)
+GH10_T = np.array([0.34290132722370461, 1.0366108297895137, 1.7566836492998818,
+                   2.5327316742327898, 3.4361591188377376])
+GH10_W = np.array([0.6108626337353258, 0.24013861108231469, 0.033874394455481063,
+                   0.0013436457467812327, 7.6404328552326206e-6])
+
+
+def k_gh10(x, y):
+    """faddeeva.h gh_sum64 with the 10 nodes: the pair terms summed as one
+    running quotient (numerator and product of the denominators)."""
+    x2, y2 = x * x, y * y
+    r, xmy, fxy = x2 + y2, x2 - y2, 4.0 * x2 * y2
+    num = np.zeros_like(x)
+    den = np.ones_like(x)
+    for k, (t, w) in enumerate(zip(GH10_T, GH10_W)):
+        u = xmy - t * t
+        D = u * u + fxy
+        num = w * (r + t * t) * den if k == 0 else num * D + w * (r + t * t) * den
+        den = D if k == 0 else den * D
+    return 2.0 * y / math.pi * num / den
+
+
+def test_ten_node_constants_are_gauss_hermite():
+    t, w = np.polynomial.hermite.hermgauss(10)
+    np.testing.assert_allclose(GH10_T, t[t > 0], rtol=1e-15)
+    np.testing.assert_allclose(GH10_W, w[t > 0], rtol=1e-13)
+    assert abs(2 * GH10_W.sum() - SQRT_PI) < 1e-15
+
+
+def test_ten_node_form_within_1e11_from_7():
+    """FP64 near-centre pass A: the 10-node form holds K to 1e-11 for |z| >= 7
+    and y >= 1e-8 (faddeeva.h kFp64Gh10R2, kFp64Gh10Ymin), against scipy's
+    wofz; and it does not hold much nearer (so the radius is not slack)."""
+    rng = np.random.default_rng(23)
+    n = 400000
+    y = np.exp(rng.uniform(math.log(1e-8), math.log(60.0), n))
+    r = 7.0 * np.exp(rng.uniform(0.0, math.log(8.0), n))
+    r[: n // 5] = 7.0
+    ok = r > y
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    err = np.abs(k_gh10(x, y) / wofz(x + 1j * y).real - 1.0)
+    assert err.max() <= 1e-11, (err.max(), x[err.argmax()], y[err.argmax()])
+    xs = np.linspace(5.8, 6.4, 61)
+    ys = np.full_like(xs, 1e-2)
+    assert np.max(np.abs(k_gh10(xs, ys) / wofz(xs + 1j * ys).real - 1.0)) > 1e-10
