@@ -277,32 +277,69 @@ __device__ __forceinline__ float K_gh4_32(float x2, float y) {
   return 2.0f * y * (float)kInvPi * N * __builtin_amdgcn_rcpf(D0 * D1);
 }
 
-// Trapezoid + residue, |z| < 8.
-__device__ inline float K_cr32(float x, float y) {
+// 6-point Gauss-Hermite nodes (t > 0), their squares and weights (50-digit
+// Newton refinement on H_6, rounded to FP32).
+constexpr float kGh6T2[3] = {0.19016350919348813f, 1.7844927485432516f, 5.5253437422632603f};
+constexpr float kGh6W[3] = {0.72462959522439252f, 0.15706732032285664f, 0.0045300099055088456f};
+// Squared radius below which the six-node form misses 5e-6 (FP32
+// arithmetic), as a function of y^2: 7.0 for y < 1e-4 (near the real axis exp(-x^2) takes over), 5.5 for y < 1e-3,
+// 5.0 for y < 2, 3.6 above (measured worst outside it: 3.2e-6).
+__device__ __forceinline__ float gh6_core_r2(float y2) {
+  return y2 < 1e-8f ? 49.0f : y2 < 1e-6f ? 30.25f : y2 < 4.0f ? 25.0f : 12.96f;
+}
+// sum_k W_k (r + t_k^2) / D_k over the six nodes as one quotient (all terms
+// positive; for |z| >= 3.6 each D_k >= 52, and a product of three stays far
+// inside the FP32 range for |z| < 1e6).
+__device__ __forceinline__ float gh6_sum32(float r, float xmy, float fxy) {
+  float u = xmy - kGh6T2[0];
+  float den = fmaf(u, u, fxy);
+  float num = kGh6W[0] * (r + kGh6T2[0]);
+#pragma unroll
+  for (int k = 1; k < 3; ++k) {
+    u = xmy - kGh6T2[k];
+    const float D = fmaf(u, u, fxy);
+    num = fmaf(num, D, kGh6W[k] * (r + kGh6T2[k]) * den);
+    den *= D;
+  }
+  return num * __builtin_amdgcn_rcpf(den);
+}
+
+// exp(-2 pi y / h), the residue's rho, per line (the near-field stage keeps it
+// with the record).
+__device__ __forceinline__ float cr32_rho(float y) {
+  return expf(-(2.0f * (float)kPi / kCr32H) * y);
+}
+
+// Trapezoid + residue, |z| < 8, with rho = cr32_rho(y).
+__device__ inline float K_cr32(float x, float y, float rho) {
   const float h = kCr32H;
   const float q = fabsf(x) * (1.0f / h);
   const float fr = q - floorf(q);
   const bool half = !(fr >= 0.25f && fr <= 0.75f);
   const float x2 = x * x, y2 = y * y, r = x2 + y2, xmy = x2 - y2, fxy = 4.0f * x2 * y2;
-  // One loop for both lattices, as in K_cr64: half-lattice slots n = 4..0,
-  // integer-lattice slots n = 5..1 (the unused slot of each has weight 0).
-  const float off = half ? 0.5f : 0.0f;
-  float s = 0.0f;
+  // One loop for both lattices, as in K_cr64: half-lattice nodes (n + 1/2) h
+  // in slots n = 0..4 (slot 5 has weight 0), integer-lattice nodes n h in
+  // slots 0..5 (the node t = 0 counts once: weight 1/2 of the pair form,
+  // (r + 0)/D = 1/r).  The pairs are summed as one quotient: all terms are
+  // positive, on the chosen lattice x stays h/4 from every node so each
+  // D >= 1.6e-3, and for |z| < 8 each D < 9.1e3 (six factors < 6e23).
+  float num = 0.0f, den = 1.0f;
 #pragma unroll
   for (int n = 5; n >= 0; --n) {
-    const float w = half ? (n <= 4 ? kCr32E5[n] : 0.0f) : (n >= 1 ? kCr32E0[n - 1] : 0.0f);
-    const float t = (n + off) * h;
-    s += gh_pair32(r, xmy, fxy, t * t, w);
+    const float w = half ? (n <= 4 ? kCr32E5[n] : 0.0f) : (n >= 1 ? kCr32E0[n - 1] : 0.5f);
+    const float th = (n + 0.5f) * h, ti = n * h;
+    const float t2 = half ? th * th : ti * ti;
+    const float u = xmy - t2;
+    const float D = fmaf(u, u, fxy);
+    num = fmaf(num, D, w * (r + t2) * den);
+    den *= D;
   }
-  s *= 2.0f * y;
-  if (!half) s += y * __builtin_amdgcn_rcpf(r);
-  float K = s * h * (float)kInvPi;
+  float K = 2.0f * h * (float)kInvPi * y * num * __builtin_amdgcn_rcpf(den);
   // |residue| <= 4M, as in K_cr64; left out where below 1e-8 of the pair sum
   const float a = 2.0f * (float)kPi / h;
   const float M = expf(fmaf(y, y - a, -x2));
   if (y < (float)kPi / h && 4.0f * M > 1e-8f * K) {
     const float sg = half ? 1.0f : -1.0f;
-    const float rho = expf(-a * y);
     // theta = 2 pi x / h and 2 x y by the hardware sine and cosine, whose
     // argument is in revolutions (x / h and x y / pi; |x| < 8 and x y < 32
     // here, far inside their +-256 range): absolute error <= 1.2e-7 over
@@ -315,6 +352,8 @@ __device__ inline float K_cr32(float x, float y) {
   }
   return K;
 }
+
+__device__ inline float K_cr32(float x, float y) { return K_cr32(x, y, cr32_rho(y)); }
 
 // Full FP32 region dispatch: relative error <= ~6.4e-6 everywhere.
 __device__ inline float K_mixed(float x, float y) {

@@ -42,10 +42,9 @@ constexpr int kChunk = kBlock;  // records staged per LDS round
 constexpr int kFoldEvery = 32;  // records between compensated folds (mixed)
 constexpr int kExCap = 128;     // far-field exception list entries per wavefront
 constexpr int kPointsFp64 = 8;  // grid points per thread of the FP64 line sums
-// Mixed near-centre drain: queued points nearer than |z| = kCoreR take the
-// trapezoid + residue form (the four-node form is <= 5e-6 only from 6.6); a
+// Mixed near-centre drain: queued points nearer than the six-node form's
+// radius (gh6_core_r2, 3.6 to 7 by y) take the trapezoid + residue form; a
 // wavefront's list of such points holds kCoreCap entries.
-constexpr float kCoreR = 7.0f;
 constexpr int kCoreCap = 512;   // >= the 64 x 8 points one pass-A round can add
 constexpr int kQueueW = 256;     // queue entries per wavefront (the loop drains at kQueueW - 128)
 constexpr int kAccStride = 256;  // s_acc row length of the mixed line sums
@@ -1163,7 +1162,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ float s_qx[kWaves][kQueueW];
   __shared__ uint16_t s_qi[kWaves][kQueueW];
   __shared__ float s_acc[kAccStride * P];
-  // queued points in the core (|z| < kCoreR): x, (record << 9) | (lane << 3) | point
+  // queued points in the core (gh6_core_r2): x, (record << 9) | (lane << 3) | point
   __shared__ float s_cx[kWaves][kSurrogate ? 1 : kCoreCap];
   __shared__ uint16_t s_ci[kWaves][kSurrogate ? 1 : kCoreCap];
   // claim byte per owner lane (the drain's plain read-modify-write rounds)
@@ -1230,6 +1229,9 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.c1 = 1.0f;
           m.ay = 0.0f; m.hw1 = 0.0f; m.a = 0.0f;
         }
+        // the near field reads ds1 as the core form's rho = exp(-2 pi y / h)
+        // (ds1 and sh serve only the far-field series)
+        m.ds1 = cr32_rho(m.y);
         s_rec[wave][lane] = m;
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1315,12 +1317,13 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       // The queue, drained by this wavefront.  kSurrogate: every queued point
       // goes to the network (inside its domain and gate) or to the full FP32
       // dispatch.  Otherwise two passes, each dense and free of divergent
-      // regions.  Pass A: one entry per lane, all P points in the four-node
-      // form (<= 5e-6 for |z| >= 6.6, faddeeva.h); points with |z| < kCoreR
-      // (the core) get weight 0 and go to the wavefront's core list (x and the
-      // target kept in the entry).  Pass B: the core list, one point per lane,
-      // trapezoid + residue form.  Results reach the owning lane's sums
-      // through s_acc (LDS float atomics).
+      // regions.  Pass A: one entry per lane, all P points in the six-node
+      // form (<= 3.2e-6 outside gh6_core_r2, faddeeva.h); points inside (the
+      // core) get weight 0 and go to the wavefront's core list (x and the
+      // target kept in the entry).  Pass B: the core list, two points per lane
+      // per round, trapezoid + residue form.  Pass A's results reach the
+      // owning lane's sums through claimed plain LDS updates, pass B's
+      // through LDS float atomics.
       auto drain = [&](const int n) {
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
         __builtin_amdgcn_wave_barrier();
@@ -1335,16 +1338,17 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
               const bool two = b + 64 < m;
               const float xa = s_cx[wave][b], xb = two ? s_cx[wave][b + 64] : 0.0f;
               const uint32_t ca = s_ci[wave][b], cb = two ? s_ci[wave][b + 64] : ca;
-              const u32x4 ha2 = rb[3 * (ca >> 9) + 2], hb2 = rb[3 * (cb >> 9) + 2];   // y, a, ds1, sh
-              const float va = __uint_as_float(ha2.y) * K_cr32(xa, __uint_as_float(ha2.x));
-              const float vb = __uint_as_float(hb2.y) * K_cr32(xb, __uint_as_float(hb2.x));
+              const u32x4 ha2 = rb[3 * (ca >> 9) + 2], hb2 = rb[3 * (cb >> 9) + 2];   // y, a, rho, sh
+              const float va = __uint_as_float(ha2.y) *
+                               K_cr32(xa, __uint_as_float(ha2.x), __uint_as_float(ha2.z));
+              const float vb = __uint_as_float(hb2.y) *
+                               K_cr32(xb, __uint_as_float(hb2.x), __uint_as_float(hb2.z));
               atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + ((ca >> 3) & 63)], va);
               if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + ((cb >> 3) & 63)], vb);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
             __builtin_amdgcn_wave_barrier();
           };
-          const float t0 = (float)(kGh4T[0] * kGh4T[0]), t1 = (float)(kGh4T[1] * kGh4T[1]);
           for (int e0 = 0; e0 < n; e0 += 64) {
             const int e = e0 + lane;
             const bool has = e < n;
@@ -1357,11 +1361,20 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             const int t = (int)(qi & 63);
             const u32x4 h1 = rb[3 * (qi >> 6) + 1];   // hs, c1, ay, hw2
             const float hs = __uint_as_float(h1.x), y2 = __uint_as_float(h1.y) + 1.5f;
-            // four-node form: A K = (2 A y / pi) sum_k W_k (r + t_k^2) / D_k, and
+            // six-node form: A K = (2 A y / pi) sum_k W_k (r + t_k^2) / D_k, and
             // A y = sqrt(pi) ay
-            const float c2 = has ? 2.0f * (float)kInvSqrtPi * __uint_as_float(h1.z) : 0.0f;
-            const float w0 = c2 * (float)kGh4W[0], w1 = c2 * (float)kGh4W[1];
+            const float ay = has ? __uint_as_float(h1.z) : 0.0f;
+            const float c2 = 2.0f * (float)kInvSqrtPi * ay;
             const float fy2 = 4.0f * y2;
+            // core radius of the six-node form; y^2 = c1 + 3/2 carries y^2 to
+            // 1.2e-7, which only ever picks the larger radius
+            const float rcore = gh6_core_r2(y2);
+            // a product of three D ~ |z|^12 overflows FP32 beyond |z| = 1e3
+            // (broad lines, y > 1e3, or grids much coarser than a Doppler
+            // width): such points take the shifted one-node form, within 2e-6
+            // there
+            const float xf = fmaxf(fabsf(x0), fabsf(fmaf((float)(P - 1), hs, x0)));
+            const bool far_pts = __ballot(has && fmaf(xf, xf, y2) > 1e6f) != 0;
             uint32_t cmask = 0;
             float v[P];
 #pragma unroll
@@ -1369,11 +1382,12 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
               const float x = fmaf((float)j, hs, x0);
               const float x2 = x * x;
               const float r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
-              const bool core = has && r < kCoreR * kCoreR;
-              const float u0 = xmy - t0, u1 = xmy - t1;
-              const float D0 = fmaf(u0, u0, fxy), D1 = fmaf(u1, u1, fxy);
-              const float N = fmaf(w0 * (r + t0), D1, w1 * (r + t1) * D0);
-              v[j] = core ? 0.0f : N * __builtin_amdgcn_rcpf(D0 * D1);
+              const bool core = has && r < rcore;
+              // (a select: at a core point the node sum may be infinite)
+              float g = c2 * gh6_sum32(r, xmy, fxy);
+              if (far_pts)   // uniform
+                g = r > 1e6f ? ay * __builtin_amdgcn_rcpf(r - 1.5f) : g;
+              v[j] = core ? 0.0f : g;
               cmask |= core ? (1u << j) : 0u;
             }
             // Deliver the P values to the owner's accumulators with plain LDS
@@ -1511,6 +1525,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           if constexpr (kCount) n_queued += (uint64_t)nq;
           drain(nq);
           nq = 0;
+          // the next two records' headers again from LDS (not held in
+          // registers through the drain)
+          if (q < cnt) ha = pr[0];
+          if (q + 1 < cnt) hb = pr[3];
         }
         if (q >= cnt) break;
       }
