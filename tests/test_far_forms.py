@@ -258,3 +258,56 @@ def test_ten_node_form_within_1e11_from_7():
     xs = np.linspace(5.8, 6.4, 61)
     ys = np.full_like(xs, 1e-2)
     assert np.max(np.abs(k_gh10(xs, ys) / wofz(xs + 1j * ys).real - 1.0)) > 1e-10
+
+
+# 6-point Gauss-Hermite nodes (squared) and weights as faddeeva.h holds them (FP32)
+GH6_T2 = np.float32([0.19016350919348813, 1.7844927485432516, 5.5253437422632603])
+GH6_W = np.float32([0.72462959522439252, 0.15706732032285664, 0.0045300099055088456])
+
+
+def k_gh6_f32(x, y):
+    """faddeeva.h gh6_sum32 times 2y/pi, evaluated in float32 like the kernel."""
+    f32 = np.float32
+    x2, y2 = f32(x) * f32(x), f32(y) * f32(y)
+    r, xmy, fxy = x2 + y2, x2 - y2, f32(4) * x2 * y2
+    for k in range(3):
+        u = xmy - GH6_T2[k]
+        D = u * u + fxy
+        n = GH6_W[k] * (r + GH6_T2[k])
+        num, den = (n, D) if k == 0 else (num * D + n * den, den * D)
+    return f32(2 / np.pi) * f32(y) * num / den
+
+
+def gh6_core_r2(y2):
+    """faddeeva.h gh6_core_r2 (y^2 as the kernel forms it, c1 + 3/2 in FP32)."""
+    return np.where(y2 < 1e-8, 49.0, np.where(y2 < 1e-6, 30.25, np.where(y2 < 4.0, 25.0, 12.96)))
+
+
+def test_six_node_constants_are_gauss_hermite():
+    t, w = np.polynomial.hermite.hermgauss(6)
+    np.testing.assert_allclose(GH6_T2, (t[t > 0] ** 2).astype(np.float32), rtol=1e-7)
+    np.testing.assert_allclose(GH6_W, w[t > 0].astype(np.float32), rtol=1e-7)
+
+
+def test_six_node_form_outside_its_core_radius():
+    """Mixed near-centre pass A: in FP32 the six-node form is within 5e-6 of
+    scipy's wofz wherever |z|^2 >= gh6_core_r2(y^2) (y^2 carried as
+    (y^2 - 3/2) + 3/2 in FP32, as the record holds it), for y from 1e-10 to
+    50; and the radius is needed (the form misses 5e-6 just inside it for
+    small y)."""
+    f32 = np.float32
+    rng = np.random.default_rng(29)
+    n = 2_000_000
+    y = np.exp(rng.uniform(math.log(1e-10), math.log(50.0), n))
+    y2k = (f32(y) * f32(y) - f32(1.5)) + f32(1.5)        # the kernel's y^2
+    rc = np.sqrt(gh6_core_r2(y2k.astype(np.float64)))
+    r = rc * np.exp(rng.uniform(0.0, math.log(10.0), n))
+    r[: n // 4] = rc[: n // 4] * (1 + 1e-6)
+    ok = r > y
+    y, r = y[ok], r[ok]
+    x = np.sqrt(r * r - y * y)
+    err = np.abs(k_gh6_f32(x, y).astype(np.float64) / wofz(x + 1j * y).real - 1.0)
+    assert err.max() <= 5e-6, (err.max(), x[err.argmax()], y[err.argmax()])
+    xs = np.linspace(4.3, 4.7, 41)
+    ys = np.full_like(xs, 1e-2)
+    assert np.max(np.abs(k_gh6_f32(xs, ys) / wofz(xs + 1j * ys).real - 1.0)) > 5e-6
