@@ -1401,9 +1401,13 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
               if (pending) s_claim[wave][t] = (uint8_t)lane;
               __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
               if (pending && s_claim[wave][t] == (uint8_t)lane) {
+                // all P reads first, then all P writes: one LDS round trip
                 float* const dst = s_acc + wave * 64 + t;
+                float cur[P];
 #pragma unroll
-                for (int j = 0; j < P; ++j) dst[j * kAccStride] += v[j];
+                for (int j = 0; j < P; ++j) cur[j] = dst[j * kAccStride];
+#pragma unroll
+                for (int j = 0; j < P; ++j) dst[j * kAccStride] = cur[j] + v[j];
                 pending = false;
               }
               __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
