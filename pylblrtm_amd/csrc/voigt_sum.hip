@@ -46,7 +46,7 @@ constexpr int kPointsFp64 = 8;  // grid points per thread of the FP64 line sums
 // radius (gh6_core_r2, 3.6 to 7 by y) take the trapezoid + residue form; a
 // wavefront's list of such points holds kCoreCap entries.
 constexpr int kCoreCap = 512;   // >= the 64 x 8 points one pass-A round can add
-constexpr int kQueueW = 256;     // queue entries per wavefront (the loop drains at kQueueW - 128)
+constexpr int kQueueW = 512;     // queue entries per wavefront (the loop drains at kQueueW - 128)
 constexpr int kAccStride = 256;  // s_acc row length of the mixed line sums
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -1162,8 +1162,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ float s_qx[kWaves][kQueueW];
   __shared__ uint16_t s_qi[kWaves][kQueueW];
   __shared__ float s_acc[kAccStride * P];
-  // queued points in the core (gh6_core_r2): x, (record << 9) | (lane << 3) | point
-  __shared__ float s_cx[kWaves][kSurrogate ? 1 : kCoreCap];
+  // queued points in the core (gh6_core_r2): (queue entry << 3) | point
   __shared__ uint16_t s_ci[kWaves][kSurrogate ? 1 : kCoreCap];
   // claim byte per owner lane (the drain's plain read-modify-write rounds)
   __shared__ uint8_t s_claim[kWaves][64];
@@ -1332,19 +1331,23 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           auto core_pass = [&](const int m) {
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
             __builtin_amdgcn_wave_barrier();
-            // two points per lane per round, so the list and record reads of
-            // both are in flight together
+            // two points per lane per round, so the list, queue and record
+            // reads of both are in flight together; x as pass A formed it
             for (int b = lane; b < m; b += 128) {
               const bool two = b + 64 < m;
-              const float xa = s_cx[wave][b], xb = two ? s_cx[wave][b + 64] : 0.0f;
               const uint32_t ca = s_ci[wave][b], cb = two ? s_ci[wave][b + 64] : ca;
-              const u32x4 ha2 = rb[3 * (ca >> 9) + 2], hb2 = rb[3 * (cb >> 9) + 2];   // y, a, rho, sh
+              const float xa0 = s_qx[wave][ca >> 3], xb0 = s_qx[wave][cb >> 3];
+              const uint32_t qa = s_qi[wave][ca >> 3], qb = s_qi[wave][cb >> 3];
+              const u32x4 ha1 = rb[3 * (qa >> 6) + 1], hb1 = rb[3 * (qb >> 6) + 1];   // hs, c1, ay, hw2
+              const u32x4 ha2 = rb[3 * (qa >> 6) + 2], hb2 = rb[3 * (qb >> 6) + 2];   // y, a, rho, sh
+              const float xa = fmaf((float)(ca & (P - 1)), __uint_as_float(ha1.x), xa0);
+              const float xb = fmaf((float)(cb & (P - 1)), __uint_as_float(hb1.x), xb0);
               const float va = __uint_as_float(ha2.y) *
                                K_cr32(xa, __uint_as_float(ha2.x), __uint_as_float(ha2.z));
               const float vb = __uint_as_float(hb2.y) *
                                K_cr32(xb, __uint_as_float(hb2.x), __uint_as_float(hb2.z));
-              atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + ((ca >> 3) & 63)], va);
-              if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + ((cb >> 3) & 63)], vb);
+              atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + (qa & 63)], va);
+              if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + (qb & 63)], vb);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
             __builtin_amdgcn_wave_barrier();
@@ -1432,8 +1435,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
 #pragma unroll
               for (int j = 0; j < P; ++j) {
                 if (cmask & (1u << j)) {
-                  s_cx[wave][off] = fmaf((float)j, hs, x0);
-                  s_ci[wave][off] = (uint16_t)((qi >> 6) << 9 | (uint32_t)t << 3 | (uint32_t)j);
+                  s_ci[wave][off] = (uint16_t)((uint32_t)e << 3 | (uint32_t)j);
                   ++off;
                 }
               }
