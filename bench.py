@@ -50,6 +50,14 @@ FLOP_PER_EVAL = 16            # per point-by-point evaluation, every mode charge
 #   FP64 wavefront level (four-node form as two pole pairs, one 28-term
 #     recurrence each): 10 shared + 2 x (22 + 219) = 492.
 FLOP_PER_PAIR = {"mixed": {"slice": 76, "wave": 76}, "fp64": {"slice": 246, "wave": 492}}
+# The mixed near field's near-centre forms, counted from the source (an fma
+# counts 2, a transcendental 1): a queued point in pass A (six-node form: x,
+# r, x^2 - y^2, 4 x^2 y^2; three pole pairs combined into one quotient; one
+# reciprocal) 35; a core point in pass B (trapezoid: six pole pairs into one
+# quotient; the residue's exp, four hardware sines/cosines and its quotient)
+# 95.  Each is on top of the point's FLOP_PER_EVAL in the near-field tiers.
+FLOP_PER_QUEUED_POINT = 35
+FLOP_PER_CORE_POINT = 95
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -122,13 +130,16 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
     peak = FP64_PEAK_TFLOPS if base == "fp64" else FP32_PEAK_TFLOPS
     near_name = {"fp64": "k_voigt_sum_fp64", "mixed": "k_voigt_sum_mixed",
                  "surrogate": "k_voigt_sum_surrogate"}[precision]
-    pairs = kw["slice_series"] + kw["wave_series"]
-    far_flop = (kw["slice_series"] * fp["slice"] + kw["wave_series"] * fp["wave"]
-                + kw["exception_evals"] * FLOP_PER_EVAL)
+    pairs = kw["slice_series"] + kw["wave_series"] + kw.get("group_series", 0)
+    # a group pair runs the wavefront level's recurrence (same terms)
+    far_flop = (kw["slice_series"] * fp["slice"] + (kw["wave_series"] + kw.get("group_series", 0))
+                * fp["wave"] + kw["exception_evals"] * FLOP_PER_EVAL)
     kern = {
         f"k_far_series_{base}": {
-            "ms": far_ms, "units": pairs, "unit": "(record, slice or wavefront) series pairs",
+            "ms": far_ms, "units": pairs,
+            "unit": "(record, slice, wavefront or group) series pairs",
             "slice_pairs": kw["slice_series"], "wave_pairs": kw["wave_series"],
+            "group_pairs": kw.get("group_series", 0),
             "exception_point_evals": kw["exception_evals"],
             "flop_per_unit": far_flop / max(pairs, 1),
             "flop_per_pair": dict(fp)},
@@ -136,6 +147,15 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
                     "flop_per_unit": FLOP_PER_EVAL, "queued_near_centre_pairs": kw["queued"],
                     "queued_core_points": kw["core_points"]},
     }
+    if base == "mixed" and kw.get("queued") is not None:
+        # the near-centre forms' work on top of the tiers' (per evaluation,
+        # averaged over the launch's point evaluations)
+        near_flop = (kw["near_evals"] * FLOP_PER_EVAL + kw["queued"] * 8 * FLOP_PER_QUEUED_POINT
+                     + kw["core_points"] * FLOP_PER_CORE_POINT)
+        kern[near_name]["flop_per_unit"] = near_flop / max(kw["near_evals"], 1)
+        kern[near_name]["flop_breakdown"] = {
+            "tiers": FLOP_PER_EVAL, "per_queued_point": FLOP_PER_QUEUED_POINT,
+            "per_core_point": FLOP_PER_CORE_POINT}
     for name, d in kern.items():
         d["achieved_tflops"] = d["units"] * d["flop_per_unit"] / (max(d["ms"], 1e-9) * 1e-3) / 1e12
         d["peak"] = peak
@@ -647,12 +667,15 @@ def run(args, pool) -> int:
     # the work units as the kernels themselves count them (counting variant,
     # untimed), next to the host's prediction of the same split
     kw = prob.kernel_work(args.precision) if args.precision != "surrogate" else None
-    work = prob.count_series_work()
+    work = prob.count_series_work(args.precision)
     result["series_work_host"] = work
     if kw is not None:
         result["kernel_work"] = kw
+        # (the near field also takes the (line, group) pairs the group series
+        # leaves to it, which only the kernels can count)
         result["kernel_work_equals_host"] = bool(
-            kw["near_evals"] == work["near_evals"]
+            kw["near_evals"] == work["near_evals"] + kw["group_deferred_evals"]
+            and kw["group_series"] + kw["group_deferred"] == work["group_pairs"]
             and kw["slice_series"] + kw["slice_exceptions"] == work["slice_pairs"]
             and kw["wave_series"] + kw["wave_exceptions"]
             == work["wave_pairs"] + kw["wave_from_slice_list"])
@@ -660,20 +683,20 @@ def run(args, pool) -> int:
         result["performed"] = {
             "note": "what the line sums evaluate per second of step (kernel counts): point-by-point "
                     "Voigt evaluations (near field + far-field exceptions) and far-field series "
-                    "pairs (one per record and slice or wavefront it is summed over); `value` "
-                    "counts the reference's evaluations instead",
+                    "pairs (one per record and slice, wavefront or 64-point group it is summed "
+                    "over); `value` counts the reference's evaluations instead",
             "point_evals_per_step": kw["near_evals"] + kw["exception_evals"],
             "point_evals_per_s": (kw["near_evals"] + kw["exception_evals"]) / sps,
-            "series_pairs_per_step": kw["slice_series"] + kw["wave_series"],
-            "series_pairs_per_s": (kw["slice_series"] + kw["wave_series"]) / sps,
+            "series_pairs_per_step": kw["slice_series"] + kw["wave_series"] + kw["group_series"],
+            "series_pairs_per_s": (kw["slice_series"] + kw["wave_series"] + kw["group_series"]) / sps,
             "near_field_point_evals_per_s_of_its_kernel": kw["near_evals"] / (near_ms * 1e-3),
             "reference_evals_per_step": E,
             "fraction_of_reference_evals_point_by_point": (kw["near_evals"] + kw["exception_evals"]) / E,
         }
     else:
         kw = {"near_evals": work["near_evals"], "slice_series": work["slice_pairs"],
-              "wave_series": work["wave_pairs"], "exception_evals": 0, "queued": None,
-              "core_points": None}
+              "wave_series": work["wave_pairs"], "group_series": work["group_pairs"],
+              "exception_evals": 0, "queued": None, "core_points": None}
     kern = kernel_rooflines(args.precision, split_ms, kw, traffic)
     dom = max(kern, key=lambda n: kern[n]["ms"])
     d = kern[dom]

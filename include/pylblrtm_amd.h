@@ -284,7 +284,8 @@ int lbl_last_timing_epilogue(double *ms_epilogue);
  * the same results and add up what they evaluate; lbl_last_work_count then
  * returns, for the calling thread's last call, up to n of these counts:
  *   [0] near-field point evaluations (records inside the tile window x the
- *       points of each wavefront that takes them point by point)
+ *       points of each wavefront (LBL_FP64) or group of 64 points
+ *       (LBL_MIXED) that takes them point by point)
  *   [1] (record, slice) series pairs      [2] (record, wavefront) series pairs
  *   [3] records the slice level passes on (its form does not hold there)
  *   [4] records the wavefront level evaluates point by point
@@ -292,6 +293,10 @@ int lbl_last_timing_epilogue(double *ms_epilogue);
  *   [7] queued core points (LBL_MIXED)    [8] slice-list overflows
  *   [9] wavefront-level candidates taken from the slice level's list [3]
  *       (one per record and live wavefront of its slice)
+ *   LBL_MIXED, group level (records of a wavefront's near range outside a
+ *   group's own): [10] (record, group) series pairs, [11] pairs left to the
+ *   near field (the one-node form fails at some point of the group), [12]
+ *   the near field's point evaluations of [11] (part of [0])
  * All counts are 0 after a call made without counting. */
 void lbl_enable_work_count(int on);
 int lbl_last_work_count(uint64_t *counts, int32_t n);

@@ -108,7 +108,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, err, work, tau_part, far, total;
+  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, glo, ghi, err, work, tau_part, far, total;
   size_t g[5];   // adjoint coefficient arrays (only with_grad)
   int64_t n_tiles, n_wg;
   int32_t slices_per_tile, layer_groups;
@@ -146,6 +146,8 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->whi = off; off = align_up(off + n_waves * 4);
   L->slo = off; off = align_up(off + (size_t)L->n_wg * 4);
   L->shi = off; off = align_up(off + (size_t)L->n_wg * 4);
+  L->glo = off; off = align_up(off + n_waves * lbl::kGroupsPerWave * 4);
+  L->ghi = off; off = align_up(off + n_waves * lbl::kGroupsPerWave * 4);
   L->err = off; off = align_up(off + 4);
   L->work = off; off = align_up(off + lbl::kWorkSlots * sizeof(unsigned long long));
   // partial tau per layer group (float sum + compensation, or double): 8 B
@@ -253,12 +255,12 @@ int lbl_workspace_bytes(const lbl_lines_t* lines, const lbl_layers_t* layers,
   return LBL_OK;
 }
 
-// Near ranges of the far-field series: per (slice, wavefront) and per slice,
-// the lines whose nu0 lies within kSeriesRatio half-spans (+ the shift bound)
-// of the unit's middle point.
+// Near ranges of the far-field series: per (slice, wavefront), per slice and
+// per group, the lines whose nu0 lies within kSeriesRatio half-spans (+ the
+// shift bound) of the unit's middle point.
 static hipError_t launch_near_ranges(const lbl_lines_t* lines, const lbl_grid_t* grid,
-                                     double margin, const Layout& L, int precision, int32_t* wlo,
-                                     int32_t* whi, int32_t* slo, int32_t* shi, hipStream_t s) {
+                                     double margin, const Layout& L, int precision, char* ws,
+                                     hipStream_t s) {
   lbl::WaveRangeArgs wr{};
   wr.nu0 = lines->nu0;
   wr.n_lines = lines->n_lines;
@@ -266,14 +268,17 @@ static hipError_t launch_near_ranges(const lbl_lines_t* lines, const lbl_grid_t*
   wr.step = grid->step;
   wr.half_width = lbl::kSeriesRatio * 0.5 * lbl::kWavePoints * grid->step + margin;
   wr.slice_half_width = lbl::kSeriesRatio * 0.5 * lbl::kSlicePoints * grid->step + margin;
+  wr.group_half_width = lbl::kSeriesRatio * 0.5 * lbl::kGroupPoints * grid->step + margin;
   wr.ts = grid->tile_samples;
   wr.n_wg = L.n_wg;
   wr.slices_per_tile = L.slices_per_tile;
   wr.waves_per_slice = lbl::sum_block(precision) / 64;
-  wr.wlo = wlo;
-  wr.whi = whi;
-  wr.slo = slo;
-  wr.shi = shi;
+  wr.wlo = reinterpret_cast<int32_t*>(ws + L.wlo);
+  wr.whi = reinterpret_cast<int32_t*>(ws + L.whi);
+  wr.slo = reinterpret_cast<int32_t*>(ws + L.slo);
+  wr.shi = reinterpret_cast<int32_t*>(ws + L.shi);
+  wr.glo = reinterpret_cast<int32_t*>(ws + L.glo);
+  wr.ghi = reinterpret_cast<int32_t*>(ws + L.ghi);
   return lbl::launch_wave_ranges(wr, s);
 }
 
@@ -362,7 +367,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
   int32_t* slo = reinterpret_cast<int32_t*>(ws + L.slo);
   int32_t* shi = reinterpret_cast<int32_t*>(ws + L.shi);
-  HIP_TRY(launch_near_ranges(lines, grid, tr.margin, L, precision, wlo, whi, slo, shi, s));
+  HIP_TRY(launch_near_ranges(lines, grid, tr.margin, L, precision, ws, s));
 
   lbl::SumArgs sa{};
   sa.rec = rec;
@@ -370,6 +375,8 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   sa.hi = hi;
   sa.wlo = wlo;
   sa.whi = whi;
+  sa.glo = reinterpret_cast<const int32_t*>(ws + L.glo);
+  sa.ghi = reinterpret_cast<const int32_t*>(ws + L.ghi);
   sa.slo = slo;
   sa.shi = shi;
   sa.t_k = layers->t_k;
@@ -571,7 +578,7 @@ int lbl_transmittance_vjp(const lbl_lines_t* lines, const lbl_groups_t* groups,
   int32_t* whi = reinterpret_cast<int32_t*>(ws + L.whi);
   int32_t* slo = reinterpret_cast<int32_t*>(ws + L.slo);
   int32_t* shi = reinterpret_cast<int32_t*>(ws + L.shi);
-  HIP_TRY(launch_near_ranges(lines, grid, tr.margin, L, LBL_MIXED, wlo, whi, slo, shi, s));
+  HIP_TRY(launch_near_ranges(lines, grid, tr.margin, L, LBL_MIXED, ws, s));
   HIP_TRY(lbl::launch_line_grad(*lines, *groups, *layers, gc[0], gc[1], gc[2], gc[3], gc[4], s));
   HIP_TRY(lbl::launch_vjp(rec, gc[0], gc[1], gc[2], gc[3], gc[4], lines->group, groups->gas, lo,
                           hi, wlo, whi, slo, shi, layers->t_k, lines->n_lines, layers->n_layers,

@@ -75,6 +75,20 @@ constexpr double kSeriesRatio = 3.0;
 constexpr int kSeriesTerms = 18;     // mixed: one-node form, FP32
 constexpr int kSeriesTerms64 = 28;   // FP64: second-order shifted form
 
+// A third level for the records inside a wavefront's near range (mixed modes):
+// the series about the middle of each group of kGroupPoints points (8 lanes),
+// half-span Hg = kGroupPoints/2 * step = H/8, for records at least
+// kSeriesRatio * Hg from it; the near field evaluates point by point only the
+// records within that distance of some group of the wavefront.
+constexpr int kGroupPoints = 8 * kPointsMixed;
+constexpr int kGroupsPerWave = kWavePoints / kGroupPoints;
+
+// Middle of the group whose first grid point is i: grid(i) + (kGroupPoints-1)/2 step.
+__device__ __forceinline__ double group_middle(double nu_min, double step, int64_t i) {
+  return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
+                   0.5 * (double)(kGroupPoints - 1) * step);
+}
+
 // Middle of the wavefront whose first grid point is i: grid(i) + (kWavePoints-1)/2 step.
 __device__ __forceinline__ double wave_middle(double nu_min, double step, int64_t i) {
   return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
@@ -102,7 +116,12 @@ __device__ __forceinline__ double slice_middle(double nu_min, double step, int64
 //   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D,
 //   ds1 = H + sqrt(max(s1_r2_min(y^2) - y^2, 0)) / s: the one-node form holds
 //        at all kWavePoints points of a wavefront iff |nu_w - c| >= ds1,
-//   sh = s H (H = kWavePoints/2 * step, the far-field series' unit).
+//   sh = s H (H = kWavePoints/2 * step, the far-field series' unit); at the
+//        group level the form holds at all points of a group iff
+//        |nu_g - c| >= ds1 - (H - Hg), and the unit is sh/8.
+// (The near field's LDS copy carries other values in ds1 and sh: the core
+// form's rho and the mask of the wavefront's groups that evaluate the
+// record point by point.)
 // A skipped (zero-column) record has c = NaN, ay = a = 0.
 struct alignas(16) MixedRec {
   double c;
@@ -144,17 +163,20 @@ struct TileRangeArgs {
 // lies within half_width of a wavefront's middle point, one entry per
 // (workgroup slice, wavefront) in launch order; [slo, shi) those within
 // slice_half_width of a slice's middle point (kSlicePoints points), one per
-// slice.
+// slice; [glo, ghi) those within group_half_width of a group's middle point
+// (kGroupPoints points), kGroupsPerWave per wavefront entry.
 struct WaveRangeArgs {
   const double* nu0;   // sorted ascending
   int64_t n_lines;
-  double nu_min, step, half_width, slice_half_width;
+  double nu_min, step, half_width, slice_half_width, group_half_width;
   int64_t ts, n_wg;
   int32_t slices_per_tile, waves_per_slice;
   int32_t* wlo;
   int32_t* whi;
   int32_t* slo;
   int32_t* shi;
+  int32_t* glo;
+  int32_t* ghi;
 };
 
 struct SumArgs {
@@ -165,6 +187,8 @@ struct SumArgs {
   const int32_t* whi;
   const int32_t* slo;   // near range of every slice
   const int32_t* shi;
+  const int32_t* glo;   // near range of every group (kGroupsPerWave per wavefront)
+  const int32_t* ghi;
   const double* t_k;
   int64_t n_lines;
   int32_t n_layers;
@@ -220,6 +244,9 @@ enum WorkCount {
   kWorkCorePoints,
   kWorkRescans,
   kWorkWaveFromSliceList,
+  kWorkGroupSeries,         // (record, group) pairs summed as group series
+  kWorkGroupDeferred,       // ... left to the near field (the form fails at the group)
+  kWorkGroupDeferredEvals,  // the near field's point evaluations of those
   kWorkSlots
 };
 

@@ -244,6 +244,25 @@ __global__ void k_wave_ranges(const WaveRangeArgs a) {
     a.slo[wg] = (int32_t)f;
     a.shi[wg] = (int32_t)l;
   }
+  // the wavefront's groups (records are sorted by nu0, and the groups' ranges
+  // move up with their middles)
+  for (int grp = 0; grp < kGroupsPerWave; ++grp) {
+    const double nu_g = group_middle(a.nu_min, a.step, first + (int64_t)grp * kGroupPoints);
+    const double lo_g = nu_g - a.group_half_width, hi_g = nu_g + a.group_half_width;
+    int64_t l = 0, h = a.n_lines;
+    while (l < h) {
+      const int64_t mid = (l + h) >> 1;
+      if (a.nu0[mid] < lo_g) l = mid + 1; else h = mid;
+    }
+    const int64_t f = l;
+    h = a.n_lines;
+    while (l < h) {
+      const int64_t mid = (l + h) >> 1;
+      if (a.nu0[mid] <= hi_g) l = mid + 1; else h = mid;
+    }
+    a.glo[g * kGroupsPerWave + grp] = (int32_t)f;
+    a.ghi[g * kGroupsPerWave + grp] = (int32_t)l;
+  }
 }
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {

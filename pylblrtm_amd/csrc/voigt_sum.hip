@@ -52,6 +52,7 @@ constexpr int kAccStride = 256;  // s_acc row length of the mixed line sums
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 typedef __attribute__((address_space(3))) MixedRec lds_mixed_rec;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
   const float y = v - comp;
@@ -1130,6 +1131,80 @@ void k_far_series_mixed(const SumArgs a) {
     kahan_add(lay[j], layc[j], S0);
     kahan_add(lay[j], layc[j], f - S0c);
   }
+
+  // ---- group level: the records of the wavefront's near range [nlo, nhi)
+  // outside a group's own near range [g_lo, g_hi), about the group's middle
+  // (Hu = Hg = H/8, unit sh/8).  Lane = (group lane/8, record slot lane%8):
+  // eight records per round, each tested for all eight groups.  Where the
+  // one-node form fails at some point of the group the pair is left to the
+  // near field, which applies this same test.
+  uint64_t n_gser = 0, n_gdef = 0, n_gdef_pts = 0;   // kCount
+  {
+    const int grp = lane >> 3, r8 = lane & 7;
+    const int64_t g_first = w_first + (int64_t)grp * kGroupPoints;
+    const bool g_live = g_first < t1;
+    const uint64_t g_pts = (uint64_t)max((int64_t)0, min((int64_t)kGroupPoints, t1 - g_first));
+    const double nu_g = group_middle(a.nu_min, a.step, g_first);
+    const int32_t* const gl = a.glo + (wg * kWaves + wave) * kGroupsPerWave;
+    const int32_t* const gh = a.ghi + (wg * kWaves + wave) * kGroupsPerWave;
+    const int32_t g_lo = min(max(gl[grp], nlo), nhi), g_hi = max(min(gh[grp], nhi), g_lo);
+    const float dHg = 0.5f * (float)(kWavePoints - kGroupPoints) * (float)a.step;
+    float G[kN];
+#pragma unroll
+    for (int n = 0; n < kN; ++n) G[n] = 0.0f;
+    float G0 = 0.0f, G0c = 0.0f, G1 = 0.0f, G1c = 0.0f;
+    auto candidate = [&](int32_t r) { return g_live && r < nhi && (r < g_lo || r >= g_hi); };
+    FarRec gx;
+    gx.c = __builtin_nan("");
+    gx.s = gx.c1 = gx.ay = gx.ds1 = gx.sh = 0.0f;
+    if (candidate(nlo + r8)) gx = far_load(nlo + r8);
+    const int32_t n_rounds = (nhi - nlo + 7) >> 3;
+    for (int32_t it = 0; it < n_rounds; ++it) {
+      const int32_t r = nlo + it * 8 + r8;
+      const FarRec m = gx;
+      if (candidate(r + 8)) gx = far_load(r + 8);   // the next round's record, ahead
+      const bool act = candidate(r) && (m.c >= lo_b) && (m.c <= hi_b);
+      const float d = (float)(nu_g - m.c);
+      const bool valid = fabsf(d) >= m.ds1 - dHg;
+      const bool ser = act && valid;
+      if (ser) series(G, m, d, 0.125f * m.sh);
+      if constexpr (kCount) {
+        n_gser += ser ? 1u : 0u;
+        n_gdef += act && !valid ? 1u : 0u;
+        n_gdef_pts += act && !valid ? g_pts : 0u;
+      }
+      if ((it & 15) == 15) {
+        kahan_add(G0, G0c, G[0]);
+        kahan_add(G1, G1c, G[1]);
+        G[0] = G[1] = 0.0f;
+      }
+    }
+    kahan_add(G0, G0c, G[0]);
+    kahan_add(G1, G1c, G[1]);
+    // totals over the group's eight lanes, in each of them
+#pragma unroll
+    for (int off = 4; off >= 1; off >>= 1) {
+      G0 += __shfl_xor(G0, off);
+      G0c += __shfl_xor(G0c, off);
+      G1 += __shfl_xor(G1, off);
+      G1c += __shfl_xor(G1c, off);
+#pragma unroll
+      for (int n = 2; n < kN; ++n) G[n] += __shfl_xor(G[n], off);
+    }
+    const float g1t = G1 - G1c;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      // tau of point (lane % 8) * P + j of the group (exact in FP32)
+      const float t = ((float)(r8 * P + j) - 0.5f * (float)(kGroupPoints - 1)) *
+                      (2.0f / (float)kGroupPoints);
+      float f = G[kN - 1];
+#pragma unroll
+      for (int n = kN - 2; n >= 2; --n) f = fmaf(f, t, G[n]);
+      f = fmaf(f, t, g1t) * t;
+      kahan_add(lay[j], layc[j], G0);
+      kahan_add(lay[j], layc[j], f - G0c);
+    }
+  }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
     const int64_t i = i0 + j;
@@ -1143,6 +1218,17 @@ void k_far_series_mixed(const SumArgs a) {
       count_add(a.work, kWorkExceptionEvals, n_wexc * w_pts);
       count_add(a.work, kWorkWaveFromSliceList, n_wlist);
       if (wave == 0 && rescan) count_add(a.work, kWorkRescans, 1);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      n_gser += __shfl_xor(n_gser, off);
+      n_gdef += __shfl_xor(n_gdef, off);
+      n_gdef_pts += __shfl_xor(n_gdef_pts, off);
+    }
+    if (lane == 0) {
+      count_add(a.work, kWorkGroupSeries, n_gser);
+      count_add(a.work, kWorkGroupDeferred, n_gdef);
+      count_add(a.work, kWorkGroupDeferredEvals, n_gdef_pts);
     }
   }
 }
@@ -1193,9 +1279,21 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const bool live = i0 < t1;   // lanes wholly past the tile end never write
   const MixedRec* __restrict__ recs = a.rec.mix;
   // This wavefront's near range: records within kSeriesRatio H of its middle
-  // point by nu0 (k_far_series_mixed has summed all others of the tile).
+  // point by nu0 (k_far_series_mixed has summed all others of the tile), and
+  // in it each group's: records within kSeriesRatio Hg of the group's middle.
+  // The series kernel has summed, per group, the records outside the group's
+  // range where the one-node form holds at all of the group's points; per
+  // lane this kernel evaluates the records in its group's range and those
+  // the group series could not take, and skips a record no group needs.
   const int32_t nlo = min(max(a.wlo[wg * kWaves + wave], rlo), rhi);
   const int32_t nhi = max(min(a.whi[wg * kWaves + wave], rhi), nlo);
+  const int32_t* const gl = a.glo + (wg * kWaves + wave) * kGroupsPerWave;
+  const int32_t* const gh = a.ghi + (wg * kWaves + wave) * kGroupsPerWave;
+  const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
+  const float dHg = 0.5f * (float)(kWavePoints - kGroupPoints) * (float)a.step;
+  const uint32_t my_group = (uint32_t)lane >> 3;
+  // points of this lane (0 for a lane wholly past the tile end)
+  const int pts_lane = (int)max((int64_t)0, min((int64_t)P, t1 - i0));
   // this wavefront's stage as an LDS-qualified pointer, so record reads stay ds_read_b128
   lds_mixed_rec* const s_rec_lds = (lds_mixed_rec*)s_rec[wave];
   // s_acc[j][thread]: a wavefront's lanes touch consecutive words
@@ -1206,8 +1304,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // column totals (compensated) live in tau_part[grp], read and rewritten
   // once per layer, so that no registers hold them through the record loops
   float2* const tau_acc = static_cast<float2*>(a.tau_part) + (int64_t)grp * a.n_nu;
-  // kCount: records in the window, queued (record, lane) pairs, core points
-  uint64_t n_in = 0, n_queued = 0, n_core = 0;
+  // kCount: point evaluations of this lane, queued (record, lane) pairs, core points
+  uint64_t n_pts = 0, n_queued = 0, n_core = 0;
 
   for (int k = k_begin; k < k_end; ++k) {
     float lay[P], layc[P], part[P];
@@ -1215,27 +1313,46 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     for (int j = 0; j < P; ++j) lay[j] = layc[j] = part[j] = 0.0f;
     const int64_t kb = (int64_t)k * a.n_lines;
     for (int32_t base = nlo; base < nhi; base += kStage) {
-      const int cnt = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
-      // stage: lane l copies record base + l (the previous stage's readers
-      // are this wavefront's own, earlier in program order)
+      const int n_here = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
+      // stage: lane l reads record base + l; the records some group of this
+      // wavefront evaluates point by point go to the stage, packed (the
+      // previous stage's readers are this wavefront's own, earlier in
+      // program order)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      bool inside = false;
-      if (lane < cnt) {
-        MixedRec m = recs[kb + base + lane];
-        inside = m.c >= lo_b && m.c <= hi_b;
-        if (!inside) {  // outside the tile window (or skipped):
-          // x = 0 passes the far test and adds ay / (0 + c1) = 0 exactly
-          m.c = 0.0; m.s = 0.0f; m.hs = 0.0f; m.c1 = 1.0f;
-          m.ay = 0.0f; m.hw1 = 0.0f; m.a = 0.0f;
+      MixedRec m;
+      uint32_t gmask = 0;
+      if (lane < n_here) {
+        m = recs[kb + base + lane];
+        // The groups that evaluate the record point by point: it lies in
+        // the group's range, or the one-node form fails at some point of the
+        // group (the same test, in the same arithmetic, as
+        // k_far_series_mixed's group level); none for a record outside the
+        // tile window (or skipped).
+        if (m.c >= lo_b && m.c <= hi_b) {
+          const int32_t r = base + lane;
+#pragma unroll
+          for (int grp = 0; grp < kGroupsPerWave; ++grp) {
+            const int32_t g_lo = min(max(gl[grp], nlo), nhi), g_hi = max(min(gh[grp], nhi), g_lo);
+            const double nu_g =
+                group_middle(a.nu_min, a.step, w_first + (int64_t)grp * kGroupPoints);
+            const bool direct = (r >= g_lo && r < g_hi) ||
+                                !(fabsf((float)(nu_g - m.c)) >= m.ds1 - dHg);
+            gmask |= direct ? (1u << grp) : 0u;
+          }
         }
-        // the near field reads ds1 as the core form's rho = exp(-2 pi y / h)
-        // (ds1 and sh serve only the far-field series)
+      }
+      const uint64_t keep = __ballot(gmask != 0);
+      const int cnt = (int)__popcll(keep);   // records in the stage (uniform)
+      if (gmask != 0) {
+        // the LDS copy carries the mask in sh and the core form's
+        // rho = exp(-2 pi y / h) in ds1 (both serve only the far-field series)
         m.ds1 = cr32_rho(m.y);
-        s_rec[wave][lane] = m;
+        m.sh = __uint_as_float(gmask);
+        s_rec[wave][__builtin_amdgcn_mbcnt_hi((uint32_t)(keep >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)keep, 0u))] = m;
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if constexpr (kCount) n_in += (uint64_t)__popcll(__ballot(inside));
       int nq = 0;   // queue entries not yet drained (uniform)
       // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
       // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
@@ -1244,19 +1361,24 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       // for y^2 < kF32S2Y2, else two-node), and the work queue.
       const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
       auto record = [&](const int q, const u32x4 h0, const volatile lds_u32x4* const body) {
+        // the groups that evaluate the record here (the series kernel has it
+        // for the others: weight 0 there)
+        const uint32_t gm = __builtin_amdgcn_readfirstlane(((const volatile lds_u32*)body)[7]);
+        const bool direct = (gm >> my_group) & 1u;
         const double c = __hiloint2double((int)h0.y, (int)h0.x);
         const float sv = __uint_as_float(h0.z);
         const float xm = (float)(nu_m - c) * sv;
         const u32x4 h1 = *body;
         const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
-        const float ay = __uint_as_float(h1.z);
+        const float ay = direct ? __uint_as_float(h1.z) : 0.0f;
+        if constexpr (kCount) n_pts += direct && live ? (uint64_t)pts_lane : 0u;
         bool queued = false;
         // One tier per record for the whole wavefront: the middle tier holds
         // wherever the one-node form does, so a record for which some lane
         // needs it runs it in every lane.  Lanes that need the queue instead
         // run it too, at a safe x (the run's middle at |x| = hw2) with weight
         // 0, so that no branch splits the wavefront.
-        if (__ballot(fabsf(xm) < __uint_as_float(h0.w)) == 0) {
+        if (__ballot(direct && fabsf(xm) < __uint_as_float(h0.w)) == 0) {
           // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
 #pragma unroll
           for (int j = 0; j < P; ++j) {
@@ -1267,7 +1389,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         } else {
           const float hw2 = __uint_as_float(h1.w);
           const bool mid = fabsf(xm) >= hw2;
-          queued = !mid && live;
+          queued = !mid && live && direct;
           const float xe = mid ? xm : hw2, aye = mid ? ay : 0.0f;
           // c1 is the record's (the same in every lane): compare its bits
           // (as signed integers, which order floats of either sign against a
@@ -1580,10 +1702,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     }
   }
   if constexpr (kCount) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n_pts += __shfl_xor(n_pts, off);
     if (lane == 0) {
-      const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
-      const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
-      count_add(a.work, kWorkNearEvals, n_in * w_pts);
+      count_add(a.work, kWorkNearEvals, n_pts);
       count_add(a.work, kWorkQueued, n_queued);
       count_add(a.work, kWorkCorePoints, n_core);
     }

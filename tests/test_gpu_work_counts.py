@@ -6,8 +6,10 @@
   tests/test_oracle.py).
 * ``OpticalDepthProblem.count_series_work`` (the host's prediction of how the
   line-sum kernels split that work: point evaluations of the near ranges,
-  series pairs per level) equals what the kernels count in their counting
-  variant, which also leaves the results unchanged.
+  series pairs per level, (line, group) pairs of the mixed group level)
+  equals what the kernels count in their counting variant, up to the pairs
+  the kernels themselves find the one-node form cannot take (counted
+  separately), and the counting variant leaves the results unchanged.
 
 Needs a ROCm GPU."""
 
@@ -77,11 +79,21 @@ def _check_kernel_counts(prob, precision):
     # a counting run leaves the counts of the next plain run at zero
     from pylblrtm_amd import _capi
     assert all(v == 0 for v in _capi.last_work_count().values())
-    hw = prob.count_series_work()
+    hw = prob.count_series_work(precision)
     say(f"  kernels {kw}")
     say(f"  host    {hw}")
     assert kw["rescans"] == 0
-    assert kw["near_evals"] == hw["near_evals"]
+    if precision == "fp64":
+        assert kw["near_evals"] == hw["near_evals"]
+        assert kw["group_series"] == kw["group_deferred"] == hw["group_pairs"] == 0
+    else:
+        # the near field evaluates the lines in each group's own near range,
+        # and those the group series left to it (the one-node form fails at
+        # some point of the group); the group level sees every other line of
+        # the wavefront's near range once per group
+        assert kw["near_evals"] == hw["near_evals"] + kw["group_deferred_evals"]
+        assert kw["group_series"] + kw["group_deferred"] == hw["group_pairs"]
+        assert (kw["group_deferred"] == 0) == (kw["group_deferred_evals"] == 0)
     assert kw["slice_series"] + kw["slice_exceptions"] == hw["slice_pairs"]
     # the wavefront level sees the records between the two near ranges, plus
     # every record the slice level passed on, once per live wavefront
@@ -103,6 +115,8 @@ def test_kernel_counts_equal_host_prediction_co2_window(precision):
     prob = OpticalDepthProblem(sd, prof.build_layers(), 700.0, 750.0, 0.001, 25.0, 2.0)
     kw = _check_kernel_counts(prob, precision)
     assert kw["near_evals"] > 0 and kw["slice_series"] > 0 and kw["wave_series"] > 0
+    if precision == "mixed":
+        assert kw["group_series"] > 0
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
@@ -150,3 +164,5 @@ def test_kernel_counts_with_far_field_exceptions():
         kw = _check_kernel_counts(prob, precision)
         assert kw["slice_exceptions"] > 0 and kw["wave_exceptions"] > 0
         assert kw["exception_evals"] >= kw["wave_exceptions"]
+        if precision == "mixed":   # broad lines: the group series cannot take all
+            assert kw["group_deferred"] > 0
