@@ -47,8 +47,10 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
 }
 
 // Grid point i exactly as numpy builds it: nu_min + (double(i) * step),
-// rounded after the multiply and after the add (no fused multiply-add).
+// rounded after the multiply and after the add (contraction off: the
+// __d*_rn helpers are plain operators, which hipcc would otherwise fuse).
 __device__ __forceinline__ double grid_nu(double nu_min, double step, int64_t i) {
+#pragma clang fp contract(off)
   return __dadd_rn(nu_min, __dmul_rn((double)i, step));
 }
 
@@ -85,17 +87,60 @@ constexpr int kGroupsPerWave = kWavePoints / kGroupPoints;
 
 // Middle of the group whose first grid point is i: grid(i) + (kGroupPoints-1)/2 step.
 __device__ __forceinline__ double group_middle(double nu_min, double step, int64_t i) {
+#pragma clang fp contract(off)
   return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
                    0.5 * (double)(kGroupPoints - 1) * step);
 }
 
+// The group level's test, shared bit for bit by k_far_series_mixed (which
+// sums a pair where it holds) and the near field's staging (which evaluates
+// the pair point by point where it fails), so each (group, record) pair is
+// taken exactly once: the one-node form holds at every point of group g of a
+// wavefront iff |nu_g - c| >= ds1 - (H - Hg).  nu_g - c is formed as
+// d0 + g * dg, with d0 = (float)(middle of group 0 - c) and dg the groups'
+// spacing in FP32; contraction is off in these helpers, so that no step is
+// fused into an fma in one kernel and not in the other.
+struct GroupTest {
+  double nu0;   // middle of the wavefront's group 0
+  float dg;     // kGroupPoints * step
+  float dH;     // H - Hg = (kWavePoints - kGroupPoints)/2 * step
+};
+__device__ __forceinline__ GroupTest group_test(double nu_min, double step, int64_t w_first) {
+#pragma clang fp contract(off)
+  GroupTest t;
+  t.nu0 = (nu_min + (double)w_first * step) + 0.5 * (double)(kGroupPoints - 1) * step;
+  t.dg = (float)((double)kGroupPoints * step);
+  t.dH = (float)(0.5 * (double)(kWavePoints - kGroupPoints) * step);
+  return t;
+}
+// offset of group g's middle from group 0's, in FP32
+__device__ __forceinline__ float group_offset(const GroupTest& t, int g) {
+#pragma clang fp contract(off)
+  return (float)g * t.dg;
+}
+// per record: d0 = (float)(nu0 - c) and thr = ds1 - (H - Hg)
+__device__ __forceinline__ float group_d0(const GroupTest& t, double c) {
+#pragma clang fp contract(off)
+  return (float)(t.nu0 - c);
+}
+__device__ __forceinline__ float group_thr(const GroupTest& t, float ds1) {
+#pragma clang fp contract(off)
+  return ds1 - t.dH;
+}
+__device__ __forceinline__ bool group_one_node_holds(float d0, float thr, float goff) {
+#pragma clang fp contract(off)
+  return fabsf(d0 + goff) >= thr;
+}
+
 // Middle of the wavefront whose first grid point is i: grid(i) + (kWavePoints-1)/2 step.
 __device__ __forceinline__ double wave_middle(double nu_min, double step, int64_t i) {
+#pragma clang fp contract(off)
   return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
                    0.5 * (double)(kWavePoints - 1) * step);
 }
 // Middle of the slice whose first grid point is i: grid(i) + (kSlicePoints-1)/2 step.
 __device__ __forceinline__ double slice_middle(double nu_min, double step, int64_t i) {
+#pragma clang fp contract(off)
   return __dadd_rn(__dadd_rn(nu_min, __dmul_rn((double)i, step)),
                    0.5 * (double)(kSlicePoints - 1) * step);
 }

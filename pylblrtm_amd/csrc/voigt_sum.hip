@@ -1148,7 +1148,8 @@ void k_far_series_mixed(const SumArgs a) {
     const int32_t* const gl = a.glo + (wg * kWaves + wave) * kGroupsPerWave;
     const int32_t* const gh = a.ghi + (wg * kWaves + wave) * kGroupsPerWave;
     const int32_t g_lo = min(max(gl[grp], nlo), nhi), g_hi = max(min(gh[grp], nhi), g_lo);
-    const float dHg = 0.5f * (float)(kWavePoints - kGroupPoints) * (float)a.step;
+    const GroupTest gt = group_test(a.nu_min, a.step, w_first);
+    const float goff = group_offset(gt, grp);
     float G[kN];
 #pragma unroll
     for (int n = 0; n < kN; ++n) G[n] = 0.0f;
@@ -1165,7 +1166,7 @@ void k_far_series_mixed(const SumArgs a) {
       if (candidate(r + 8)) gx = far_load(r + 8);   // the next round's record, ahead
       const bool act = candidate(r) && (m.c >= lo_b) && (m.c <= hi_b);
       const float d = (float)(nu_g - m.c);
-      const bool valid = fabsf(d) >= m.ds1 - dHg;
+      const bool valid = group_one_node_holds(group_d0(gt, m.c), group_thr(gt, m.ds1), goff);
       const bool ser = act && valid;
       if (ser) series(G, m, d, 0.125f * m.sh);
       if constexpr (kCount) {
@@ -1265,7 +1266,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const int64_t t0 = tile * a.ts;
   const int64_t t1 = min(t0 + a.ts, a.n_nu);
   const int64_t s0 = t0 + (int64_t)slice * (kMixedBlock * P);   // first point of the slice
-  const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+  // the wavefront index read into a scalar register, so that the per-wave
+  // tables (near and group ranges) are read with scalar loads
+  const int lane = (int)threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   // a wavefront wholly past the end of a short (last) tile has no points
   if (s0 + (int64_t)wave * kWavePoints >= t1) return;
   const int64_t i0 = s0 + (int64_t)threadIdx.x * P;
@@ -1290,7 +1294,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   const int32_t* const gl = a.glo + (wg * kWaves + wave) * kGroupsPerWave;
   const int32_t* const gh = a.ghi + (wg * kWaves + wave) * kGroupsPerWave;
   const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
-  const float dHg = 0.5f * (float)(kWavePoints - kGroupPoints) * (float)a.step;
+  const GroupTest gt = group_test(a.nu_min, a.step, w_first);
   const uint32_t my_group = (uint32_t)lane >> 3;
   // points of this lane (0 for a lane wholly past the tile end)
   const int pts_lane = (int)max((int64_t)0, min((int64_t)P, t1 - i0));
@@ -1319,24 +1323,29 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       // previous stage's readers are this wavefront's own, earlier in
       // program order)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      MixedRec m;
+      // the record as three 16-byte words in registers: (c, s, hw1),
+      // (hs, c1, ay, hw2), (y, a, ds1, sh)
+      u32x4 w0 = {0u, 0u, 0u, 0u}, w1 = w0, w2 = w0;
       uint32_t gmask = 0;
       if (lane < n_here) {
-        m = recs[kb + base + lane];
+        const u32x4* const src = (const u32x4*)(recs + kb + base + lane);
+        w0 = src[0];
+        w1 = src[1];
+        w2 = src[2];
+        const double c = __hiloint2double((int)w0.y, (int)w0.x);
         // The groups that evaluate the record point by point: it lies in
         // the group's range, or the one-node form fails at some point of the
         // group (the same test, in the same arithmetic, as
         // k_far_series_mixed's group level); none for a record outside the
         // tile window (or skipped).
-        if (m.c >= lo_b && m.c <= hi_b) {
+        if (c >= lo_b && c <= hi_b) {
           const int32_t r = base + lane;
+          const float d0 = group_d0(gt, c), thr = group_thr(gt, __uint_as_float(w2.z));
 #pragma unroll
           for (int grp = 0; grp < kGroupsPerWave; ++grp) {
             const int32_t g_lo = min(max(gl[grp], nlo), nhi), g_hi = max(min(gh[grp], nhi), g_lo);
-            const double nu_g =
-                group_middle(a.nu_min, a.step, w_first + (int64_t)grp * kGroupPoints);
             const bool direct = (r >= g_lo && r < g_hi) ||
-                                !(fabsf((float)(nu_g - m.c)) >= m.ds1 - dHg);
+                                !group_one_node_holds(d0, thr, group_offset(gt, grp));
             gmask |= direct ? (1u << grp) : 0u;
           }
         }
@@ -1344,12 +1353,15 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       const uint64_t keep = __ballot(gmask != 0);
       const int cnt = (int)__popcll(keep);   // records in the stage (uniform)
       if (gmask != 0) {
-        // the LDS copy carries the mask in sh and the core form's
-        // rho = exp(-2 pi y / h) in ds1 (both serve only the far-field series)
-        m.ds1 = cr32_rho(m.y);
-        m.sh = __uint_as_float(gmask);
-        s_rec[wave][__builtin_amdgcn_mbcnt_hi((uint32_t)(keep >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)keep, 0u))] = m;
+        // the LDS copy carries the core form's rho = exp(-2 pi y / h) in ds1
+        // and the mask in sh (both serve only the far-field series)
+        w2.z = __float_as_uint(cr32_rho(__uint_as_float(w2.x)));
+        w2.w = gmask;
+        lds_u32x4* const dst = (lds_u32x4*)&s_rec_lds[__builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(keep >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)keep, 0u))];
+        dst[0] = w0;
+        dst[1] = w1;
+        dst[2] = w2;
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       __builtin_amdgcn_wave_barrier();
