@@ -49,7 +49,12 @@ FLOP_PER_EVAL = 16            # per point-by-point evaluation, every mode charge
 #     = 246;
 #   FP64 wavefront level (four-node form as two pole pairs, one 28-term
 #     recurrence each): 10 shared + 2 x (22 + 219) = 492.
-FLOP_PER_PAIR = {"mixed": {"slice": 76, "wave": 76}, "fp64": {"slice": 246, "wave": 492}}
+#   mixed group level, middle-tier form (where the one-node form fails at the
+#     group; quartic denominator, 18 FP32 terms): x, the form's constants,
+#     u's and D's coefficients, rcp and four ratios (26), the first four terms
+#     (18), then 14 terms of three fma + mul (98) and 18 adds = 160.
+FLOP_PER_PAIR = {"mixed": {"slice": 76, "wave": 76, "group_middle": 160},
+                 "fp64": {"slice": 246, "wave": 492}}
 # The mixed near field's near-centre forms, counted from the source (an fma
 # counts 2, a transcendental 1): a queued point in pass A (six-node form: x,
 # r, x^2 - y^2, 4 x^2 y^2; three pole pairs combined into one quotient; one
@@ -131,15 +136,18 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
     near_name = {"fp64": "k_voigt_sum_fp64", "mixed": "k_voigt_sum_mixed",
                  "surrogate": "k_voigt_sum_surrogate"}[precision]
     pairs = kw["slice_series"] + kw["wave_series"] + kw.get("group_series", 0)
-    # a group pair runs the wavefront level's recurrence (same terms)
-    far_flop = (kw["slice_series"] * fp["slice"] + (kw["wave_series"] + kw.get("group_series", 0))
-                * fp["wave"] + kw["exception_evals"] * FLOP_PER_EVAL)
+    # a group pair in the one-node form runs the wavefront level's recurrence
+    # (same terms); those in the middle-tier form run its quartic one
+    g_mid = kw.get("group_series_middle", 0) or 0
+    far_flop = (kw["slice_series"] * fp["slice"]
+                + (kw["wave_series"] + kw.get("group_series", 0) - g_mid) * fp["wave"]
+                + g_mid * fp.get("group_middle", 0) + kw["exception_evals"] * FLOP_PER_EVAL)
     kern = {
         f"k_far_series_{base}": {
             "ms": far_ms, "units": pairs,
             "unit": "(record, slice, wavefront or group) series pairs",
             "slice_pairs": kw["slice_series"], "wave_pairs": kw["wave_series"],
-            "group_pairs": kw.get("group_series", 0),
+            "group_pairs": kw.get("group_series", 0), "group_pairs_middle_form": g_mid,
             "exception_point_evals": kw["exception_evals"],
             "flop_per_unit": far_flop / max(pairs, 1),
             "flop_per_pair": dict(fp)},

@@ -295,8 +295,9 @@ int lbl_last_timing_epilogue(double *ms_epilogue);
  *       (one per record and live wavefront of its slice)
  *   LBL_MIXED, group level (records of a wavefront's near range outside a
  *   group's own): [10] (record, group) series pairs, [11] pairs left to the
- *   near field (the one-node form fails at some point of the group), [12]
- *   the near field's point evaluations of [11] (part of [0])
+ *   near field (neither the one-node nor the middle-tier form holds at every
+ *   point of the group), [12] the near field's point evaluations of [11]
+ *   (part of [0]), [13] the pairs of [10] summed in the middle-tier form
  * All counts are 0 after a call made without counting. */
 void lbl_enable_work_count(int on);
 int lbl_last_work_count(uint64_t *counts, int32_t n);

@@ -372,6 +372,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       if (gg == gas) g_v[gg] += (double)fv;
   };
   // the one-node form holds at every point of the slice / of this wavefront
+  const float Hw = wave_half_span32(a.step);   // the series' unit sh = s Hw
   auto slice_ok = [&](const MixedRec& m) { return fabsf((float)(nu_s - m.c)) >= m.ds1 + dS; };
   auto wave_ok = [&](const MixedRec& m) { return fabsf((float)(nu_w - m.c)) >= m.ds1; };
   const MixedRec* __restrict__ lrec = recs + kb;
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       const MixedRec m = lrec[li];
       if (!(m.c >= lo_b && m.c <= hi_b)) continue;   // outside the tile window (or skipped)
       if (slice_ok(m)) {
-        far_record(M, m, li, (float)(nu_s - m.c), 4.0f * m.sh);
+        far_record(M, m, li, (float)(nu_s - m.c), 4.0f * (m.s * Hw));
       } else {
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kAdjSliceExCap) s_sx[pos] = (int32_t)li;
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       const MixedRec m = lrec[li];
       if (!(m.c >= lo_b && m.c <= hi_b)) continue;
       if (rescan && v >= nA + nB && slice_ok(m)) continue;   // the slice level summed it
-      if (wave_ok(m)) far_record(M, m, li, (float)(nu_w - m.c), m.sh);
+      if (wave_ok(m)) far_record(M, m, li, (float)(nu_w - m.c), m.s * Hw);
     }
   }
 

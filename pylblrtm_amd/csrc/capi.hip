@@ -108,7 +108,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct Layout {
-  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, glo, ghi, err, work, tau_part, far, total;
+  size_t c, s, y, a, lo, hi, wlo, whi, slo, shi, glo, ghi, err, work, deferred, tau_part, far, total;
   size_t g[5];   // adjoint coefficient arrays (only with_grad)
   int64_t n_tiles, n_wg;
   int32_t slices_per_tile, layer_groups;
@@ -150,6 +150,10 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->ghi = off; off = align_up(off + n_waves * lbl::kGroupsPerWave * 4);
   L->err = off; off = align_up(off + 4);
   L->work = off; off = align_up(off + lbl::kWorkSlots * sizeof(unsigned long long));
+  // mixed modes: the group level's left-over pair count per (layer, wavefront)
+  L->deferred = off;
+  if (!fp64 && !with_grad)
+    off = align_up(off + (size_t)std::max<int32_t>(layers->n_layers, 0) * n_waves * 4);
   // partial tau per layer group (float sum + compensation, or double): 8 B
   L->tau_part = off;
   // (the line sums keep their column totals here even with one group)
@@ -394,6 +398,7 @@ static int optical_depth_impl(const lbl_lines_t* lines, const lbl_groups_t* grou
   sa.layer_groups = L.layer_groups;
   sa.tau_part = ws + L.tau_part;
   sa.far_sum = ws + L.far;
+  sa.deferred = reinterpret_cast<uint32_t*>(ws + L.deferred);
   sa.n_mu = out->n_mu;
   for (int m = 0; m < out->n_mu; ++m) sa.inv_mu[m] = 1.0 / out->mu[m];
   sa.trans = out->transmittance;

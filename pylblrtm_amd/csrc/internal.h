@@ -104,6 +104,7 @@ struct GroupTest {
   double nu0;   // middle of the wavefront's group 0
   float dg;     // kGroupPoints * step
   float dH;     // H - Hg = (kWavePoints - kGroupPoints)/2 * step
+  float hg;     // Hg = kGroupPoints/2 * step
 };
 __device__ __forceinline__ GroupTest group_test(double nu_min, double step, int64_t w_first) {
 #pragma clang fp contract(off)
@@ -111,6 +112,7 @@ __device__ __forceinline__ GroupTest group_test(double nu_min, double step, int6
   t.nu0 = (nu_min + (double)w_first * step) + 0.5 * (double)(kGroupPoints - 1) * step;
   t.dg = (float)((double)kGroupPoints * step);
   t.dH = (float)(0.5 * (double)(kWavePoints - kGroupPoints) * step);
+  t.hg = (float)(0.5 * (double)kGroupPoints * step);
   return t;
 }
 // offset of group g's middle from group 0's, in FP32
@@ -130,6 +132,22 @@ __device__ __forceinline__ float group_thr(const GroupTest& t, float ds1) {
 __device__ __forceinline__ bool group_one_node_holds(float d0, float thr, float goff) {
 #pragma clang fp contract(off)
   return fabsf(d0 + goff) >= thr;
+}
+// Where the one-node form fails at a group, the group level sums the record's
+// middle-tier form instead (the second-order shifted form for y^2 < kF32S2Y2,
+// else the two-node form) if that holds at every point of the group: at
+// distance |nu - c| >= dm (MixedRec), i.e. iff |nu_g - c| - (Hg - step/2) >=
+// dm, which |nu_g - c| >= dm + Hg implies.  The test is group_one_node_holds's
+// with this threshold.
+__device__ __forceinline__ float group_mid_thr(const GroupTest& t, float dm) {
+#pragma clang fp contract(off)
+  return dm + t.hg;
+}
+
+// H = kWavePoints/2 * step in FP32, as the mixed records' ds1 and the series'
+// unit sh = s H are formed.
+__device__ __forceinline__ float wave_half_span32(double step) {
+  return 0.5f * (float)kWavePoints * (float)step;
 }
 
 // Middle of the wavefront whose first grid point is i: grid(i) + (kWavePoints-1)/2 step.
@@ -161,18 +179,20 @@ __device__ __forceinline__ double slice_middle(double nu_min, double step, int64
 //   y = sqrt(ln2) gamma_L / alpha_D, a = A = S sqrt(ln2/pi) / alpha_D,
 //   ds1 = H + sqrt(max(s1_r2_min(y^2) - y^2, 0)) / s: the one-node form holds
 //        at all kWavePoints points of a wavefront iff |nu_w - c| >= ds1,
-//   sh = s H (H = kWavePoints/2 * step, the far-field series' unit); at the
-//        group level the form holds at all points of a group iff
-//        |nu_g - c| >= ds1 - (H - Hg), and the unit is sh/8.
-// (The near field's LDS copy carries other values in ds1 and sh: the core
-// form's rho and the mask of the wavefront's groups that evaluate the
-// record point by point.)
+//        at the group level the form holds at all points of a group iff
+//        |nu_g - c| >= ds1 - (H - Hg),
+//   dm = sqrt(max(r2_mid - y^2, 0)) / s, r2_mid the middle tier's (hw2):
+//        the middle-tier form holds where |nu - c| >= dm (group_mid_thr).
+// The far-field series' unit is sh = s H (H = kWavePoints/2 * step,
+// wave_half_span32), sh/8 at the group level.  (The near field's LDS copy
+// carries other values in ds1 and dm: the core form's rho and the mask of
+// the wavefront's groups that evaluate the record point by point.)
 // A skipped (zero-column) record has c = NaN, ay = a = 0.
 struct alignas(16) MixedRec {
   double c;
   float s, hw1;
   float hs, c1, ay, hw2;
-  float y, a, ds1, sh;
+  float y, a, ds1, dm;
 };
 static_assert(sizeof(MixedRec) == 48, "record layout");
 
@@ -267,6 +287,10 @@ struct SumArgs {
   // when set, the counting variants of the line-sum kernels add their work
   // units to these kWorkSlots counters (same results otherwise)
   unsigned long long* work;
+  // mixed modes: per (layer, wavefront) [n_layers][n_wg * 4], the number of
+  // (record, group) pairs the group level left to the near field (written by
+  // k_far_series_mixed for every live wavefront, read by the near field)
+  uint32_t* deferred;
 };
 
 // Work units the line-sum kernels count in their counting variant:
@@ -292,6 +316,7 @@ enum WorkCount {
   kWorkGroupSeries,         // (record, group) pairs summed as group series
   kWorkGroupDeferred,       // ... left to the near field (the form fails at the group)
   kWorkGroupDeferredEvals,  // the near field's point evaluations of those
+  kWorkGroupMiddle,         // group series pairs in the middle-tier form (part of kWorkGroupSeries)
   kWorkSlots
 };
 

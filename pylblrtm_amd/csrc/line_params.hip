@@ -70,12 +70,13 @@ __device__ __forceinline__ void store_mixed(const Records& r, int64_t i, double 
   m.hw1 = half_run + x_s1;
   // the line sum picks the middle tier by the same test on c1
   const float r2_mid = (m.c1 < kF32S2Y2 - 1.5f) ? s2_r2_min32(y2) : kF32Gh2R2;
-  m.hw2 = half_run + sqrtf(fmaxf(r2_mid - y2, 0.0f));
+  const float x_s2 = sqrtf(fmaxf(r2_mid - y2, 0.0f));
+  m.hw2 = half_run + x_s2;
   m.y = y;
   m.a = a;
-  const float H = 0.5f * (float)kWavePoints * step;
+  const float H = wave_half_span32(step);
   m.ds1 = H + x_s1 / s;
-  m.sh = s * H;
+  m.dm = x_s2 / s;
   r.mix[i] = m;
 }
 

@@ -853,6 +853,9 @@ void k_far_series_mixed(const SumArgs a) {
   __shared__ int s_sxn;
   // slice-level coefficients per wavefront (Q_0 and Q_1 with compensation)
   __shared__ float s_coef[kWaves][kN + 2];
+  // the lane's compensated layer sums, parked in LDS through the group level
+  // (whose series coefficients need the registers)
+  __shared__ float s_lay[2 * P][kMixedBlock];
   // (layer, slice), layer-major so that an XCD holds neighbouring tiles of a layer
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
   const int k = (int)(lw / a.n_wg);
@@ -881,13 +884,14 @@ void k_far_series_mixed(const SumArgs a) {
   const int64_t kb = (int64_t)k * a.n_lines;
   const MixedRec* __restrict__ lrec = a.rec.mix + kb;
   // the fields the series needs, prefetched one iteration ahead
-  struct FarRec { double c; float s, c1, ay, ds1, sh; };
+  struct FarRec { double c; float s, c1, ay, ds1, dm; };
   auto far_load = [&](int64_t i) {
     const MixedRec* const p = lrec + i;
     FarRec f;
-    f.c = p->c; f.s = p->s; f.c1 = p->c1; f.ay = p->ay; f.ds1 = p->ds1; f.sh = p->sh;
+    f.c = p->c; f.s = p->s; f.c1 = p->c1; f.ay = p->ay; f.ds1 = p->ds1; f.dm = p->dm;
     return f;
   };
+  const float Hw = wave_half_span32(a.step);   // the series' unit at the wavefront level is s Hw
   // adds record m's series about a middle point at offset d = nu_m - c (units
   // of the level: sh scaled by the caller) to C (Q_0, Q_1 into C[0], C[1])
   auto series = [&](float (&C)[kN], const FarRec& m, float d, float sh) {
@@ -904,6 +908,45 @@ void k_far_series_mixed(const SumArgs a) {
       C[n] += qn;
       qa = qb;
       qb = qn;
+    }
+  };
+  // adds record m's middle-tier form (faddeeva.h: the second-order shifted
+  // form for y^2 < kF32S2Y2, else the two-node form) about a middle point at
+  // offset d (unit sh) to C.  Both forms are u = x^2 + A, f = (ay u + Cn) /
+  // (u^2 + B) (A = y^2 - 3/4, B = 2 y^2 - 33/16, Cn = 3/4 ay, or A = y^2 -
+  // 1/2, B = 2 y^2, Cn = ay; the near field's middle tier); with x = xm +
+  // sh tau, u = u0 + u1 tau + u2 tau^2 and D = u^2 + B = sum_(i<=4) d_i tau^i,
+  // and D Q = N gives Q_n = (N_n - d1 Q_(n-1) - d2 Q_(n-2) - d3 Q_(n-3) -
+  // d4 Q_(n-4)) / d0 (N_n = 0 for n > 2).  Used at the group level only,
+  // where the record is at least kSeriesRatio Hu away and the form holds at
+  // every point: the poles then lie at |tau| >= 2.69 and kN terms keep the
+  // truncation <= 1e-7 (FP32 arithmetic adds <= 1.1e-6 of the value).
+  auto series_mid = [&](float (&C)[kN], const FarRec& m, float d, float sh) {
+    const float xm = d * m.s;
+    const bool s2 = m.c1 < kF32S2Y2 - 1.5f;
+    const float A = s2 ? m.c1 + 0.75f : m.c1 + 1.0f;
+    const float B = s2 ? fmaf(2.0f, m.c1, 0.9375f) : fmaf(2.0f, m.c1, 3.0f);
+    const float Cn = s2 ? 0.75f * m.ay : m.ay;
+    const float u0 = fmaf(xm, xm, A), u1 = 2.0f * xm * sh, u2 = sh * sh;
+    const float r = __builtin_amdgcn_rcpf(fmaf(u0, u0, B));
+    const float e1 = 2.0f * u0 * u1 * r, e2 = fmaf(u1, u1, 2.0f * u0 * u2) * r;
+    const float e3 = 2.0f * u1 * u2 * r, e4 = u2 * u2 * r;
+    float q0 = fmaf(m.ay, u0, Cn) * r;
+    float q1 = fmaf(-e1, q0, m.ay * u1 * r);
+    float q2 = fmaf(-e2, q0, fmaf(-e1, q1, m.ay * u2 * r));
+    float q3 = -fmaf(e1, q2, fmaf(e2, q1, e3 * q0));
+    C[0] += q0;
+    C[1] += q1;
+    C[2] += q2;
+    C[3] += q3;
+#pragma unroll
+    for (int n = 4; n < kN; ++n) {
+      const float qn = -fmaf(e1, q3, fmaf(e2, q2, fmaf(e3, q1, e4 * q0)));
+      C[n] += qn;
+      q0 = q1;
+      q1 = q2;
+      q2 = q3;
+      q3 = qn;
     }
   };
   if (threadIdx.x == 0) s_sxn = 0;
@@ -932,7 +975,7 @@ void k_far_series_mixed(const SumArgs a) {
       const float d = (float)(nu_s - m.c);
       const bool inw = (v < n_sl) & (m.c >= lo_b) & (m.c <= hi_b);
       const bool ser = inw & (fabsf(d) >= m.ds1 + dS);
-      if (ser) series(C, m, d, 4.0f * m.sh);
+      if (ser) series(C, m, d, 4.0f * (m.s * Hw));
       if (inw & !ser) {   // rare: on to the wavefront level
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
@@ -1081,7 +1124,7 @@ void k_far_series_mixed(const SumArgs a) {
     // the rescan skips what the slice level summed
     if (v >= nA + nB + nC) inw &= !(fabsf((float)(nu_s - m.c)) >= m.ds1 + dS);
     const bool ser = inw & (fabsf(d) >= m.ds1);
-    if (ser) series(C, m, d, m.sh);
+    if (ser) series(C, m, d, m.s * Hw);
     const bool exc = inw & !ser;
     const uint64_t ex = __ballot(exc);
     if constexpr (kCount) {
@@ -1135,10 +1178,17 @@ void k_far_series_mixed(const SumArgs a) {
   // ---- group level: the records of the wavefront's near range [nlo, nhi)
   // outside a group's own near range [g_lo, g_hi), about the group's middle
   // (Hu = Hg = H/8, unit sh/8).  Lane = (group lane/8, record slot lane%8):
-  // eight records per round, each tested for all eight groups.  Where the
-  // one-node form fails at some point of the group the pair is left to the
-  // near field, which applies this same test.
-  uint64_t n_gser = 0, n_gdef = 0, n_gdef_pts = 0;   // kCount
+  // eight records per round, each tested for all eight groups.  The one-node
+  // form's series where that form holds at every point of the group, else
+  // the middle-tier form's where that holds; a pair for which neither does
+  // is left to the near field, which applies these same tests.
+  uint32_t n_gdef = 0;                             // pairs left to the near field
+  uint64_t n_gser = 0, n_gmid = 0, n_gdef_pts = 0;   // kCount
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    s_lay[j][threadIdx.x] = lay[j];
+    s_lay[P + j][threadIdx.x] = layc[j];
+  }
   {
     const int grp = lane >> 3, r8 = lane & 7;
     const int64_t g_first = w_first + (int64_t)grp * kGroupPoints;
@@ -1157,7 +1207,7 @@ void k_far_series_mixed(const SumArgs a) {
     auto candidate = [&](int32_t r) { return g_live && r < nhi && (r < g_lo || r >= g_hi); };
     FarRec gx;
     gx.c = __builtin_nan("");
-    gx.s = gx.c1 = gx.ay = gx.ds1 = gx.sh = 0.0f;
+    gx.s = gx.c1 = gx.ay = gx.ds1 = gx.dm = 0.0f;
     if (candidate(nlo + r8)) gx = far_load(nlo + r8);
     const int32_t n_rounds = (nhi - nlo + 7) >> 3;
     for (int32_t it = 0; it < n_rounds; ++it) {
@@ -1166,13 +1216,18 @@ void k_far_series_mixed(const SumArgs a) {
       if (candidate(r + 8)) gx = far_load(r + 8);   // the next round's record, ahead
       const bool act = candidate(r) && (m.c >= lo_b) && (m.c <= hi_b);
       const float d = (float)(nu_g - m.c);
-      const bool valid = group_one_node_holds(group_d0(gt, m.c), group_thr(gt, m.ds1), goff);
+      const float d0 = group_d0(gt, m.c);
+      const bool valid = group_one_node_holds(d0, group_thr(gt, m.ds1), goff);
       const bool ser = act && valid;
-      if (ser) series(G, m, d, 0.125f * m.sh);
+      if (ser) series(G, m, d, 0.125f * (m.s * Hw));
+      const bool mid = act && !valid && group_one_node_holds(d0, group_mid_thr(gt, m.dm), goff);
+      if (mid) series_mid(G, m, d, 0.125f * (m.s * Hw));
+      const bool left = act && !valid && !mid;
+      n_gdef += left ? 1u : 0u;
       if constexpr (kCount) {
-        n_gser += ser ? 1u : 0u;
-        n_gdef += act && !valid ? 1u : 0u;
-        n_gdef_pts += act && !valid ? g_pts : 0u;
+        n_gser += ser || mid ? 1u : 0u;
+        n_gmid += mid ? 1u : 0u;
+        n_gdef_pts += left ? g_pts : 0u;
       }
       if ((it & 15) == 15) {
         kahan_add(G0, G0c, G[0]);
@@ -1202,6 +1257,8 @@ void k_far_series_mixed(const SumArgs a) {
 #pragma unroll
       for (int n = kN - 2; n >= 2; --n) f = fmaf(f, t, G[n]);
       f = fmaf(f, t, g1t) * t;
+      lay[j] = s_lay[j][threadIdx.x];   // (the lane's own words: no barrier)
+      layc[j] = s_lay[P + j][threadIdx.x];
       kahan_add(lay[j], layc[j], G0);
       kahan_add(lay[j], layc[j], f - G0c);
     }
@@ -1211,6 +1268,11 @@ void k_far_series_mixed(const SumArgs a) {
     const int64_t i = i0 + j;
     if (i < t1) static_cast<float*>(a.far_sum)[(int64_t)k * a.n_nu + i] = lay[j] - layc[j];
   }
+  // the pairs this wavefront left to the near field (it skips its scan for
+  // them when there are none)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) n_gdef += __shfl_xor(n_gdef, off);
+  if (lane == 0) a.deferred[(int64_t)k * (a.n_wg * kWaves) + wg * kWaves + wave] = n_gdef;
   if constexpr (kCount) {
     if (lane == 0) {
       const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
@@ -1223,11 +1285,12 @@ void k_far_series_mixed(const SumArgs a) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
       n_gser += __shfl_xor(n_gser, off);
-      n_gdef += __shfl_xor(n_gdef, off);
+      n_gmid += __shfl_xor(n_gmid, off);
       n_gdef_pts += __shfl_xor(n_gdef_pts, off);
     }
     if (lane == 0) {
       count_add(a.work, kWorkGroupSeries, n_gser);
+      count_add(a.work, kWorkGroupMiddle, n_gmid);
       count_add(a.work, kWorkGroupDeferred, n_gdef);
       count_add(a.work, kWorkGroupDeferredEvals, n_gdef_pts);
     }
@@ -1244,6 +1307,9 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // atomics) into the owning lane's partial sums.
   constexpr int kWaves = kMixedBlock / 64;
   constexpr int kStage = 64;                        // records per wavefront stage
+  // phase 1 stages kStageG records of each group's own range (one per lane of the group)
+  constexpr int kStageG = kStage / kGroupsPerWave;
+  static_assert(kStageG * kGroupsPerWave == 64, "one record per lane per stage");
   __shared__ MixedRec s_rec[kWaves][kStage];
   // queue: x at the lane's first point, (record << 6) | lane
   __shared__ float s_qx[kWaves][kQueueW];
@@ -1311,20 +1377,366 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   // kCount: point evaluations of this lane, queued (record, lane) pairs, core points
   uint64_t n_pts = 0, n_queued = 0, n_core = 0;
 
+  // this wavefront's record stage; a record's header (c, s, hw1) and first
+  // body word (hs, c1, ay, hw2) are read as one ds_read_b128 each (volatile
+  // keeps them single reads)
+  const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
+  // this lane's group's near range (clamped to the wavefront's, as the
+  // staging of the deferred pairs tests it) and the longest of the eight
+  const int32_t my_lo = min(max(gl[my_group], nlo), nhi);
+  const int32_t my_hi = max(min(gh[my_group], nhi), my_lo);
+  int32_t n_own = 0;   // uniform (scalar loads of the per-wave tables)
+#pragma unroll
+  for (int g = 0; g < kGroupsPerWave; ++g) {
+    const int32_t g_lo = min(max(gl[g], nlo), nhi), g_hi = max(min(gh[g], nhi), g_lo);
+    n_own = max(n_own, g_hi - g_lo);
+  }
+  // the record that fills a stage slot with nothing to evaluate: weight 0 at
+  // finite x in every tier, never the one-node tier's obstacle, never queued
+  const u32x4 null0 = {0u, 0u, 0u, __float_as_uint(-1.0f)};   // c = 0, s = 0, hw1 = -1
+  const u32x4 null1 = {0u, __float_as_uint(1.0f), 0u, __float_as_uint(-1.0f)};   // hs, c1, ay, hw2
+  const u32x4 null2 = {__float_as_uint(1.0f), 0u, 0u, 0u};   // y, a, rho, mask
+
   for (int k = k_begin; k < k_end; ++k) {
     float lay[P], layc[P], part[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) lay[j] = layc[j] = part[j] = 0.0f;
     const int64_t kb = (int64_t)k * a.n_lines;
-    for (int32_t base = nlo; base < nhi; base += kStage) {
+    int nq = 0;   // queue entries not yet drained (uniform)
+    // One record (stage slot q) at this lane's P points; a lane that does not
+    // evaluate it (direct false) adds weight 0.  Three tiers per (record,
+    // lane): the shifted one-node form, the middle tier (second-order shifted
+    // form for y^2 < kF32S2Y2, else two-node) and the work queue.  One tier
+    // for the whole wavefront: the middle tier holds wherever the one-node
+    // form does, so if some lane needs it every lane runs it, and lanes that
+    // need the queue instead run it too, at a safe x (the run's middle at
+    // |x| = hw2) with weight 0, so that no branch splits the wavefront.
+    auto record = [&](const int q, const u32x4 h0, const u32x4 h1, const bool direct) {
+      const double c = __hiloint2double((int)h0.y, (int)h0.x);
+      const float sv = __uint_as_float(h0.z);
+      const float xm = (float)(nu_m - c) * sv;
+      const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
+      const float ay = direct ? __uint_as_float(h1.z) : 0.0f;
+      if constexpr (kCount) n_pts += direct && live ? (uint64_t)pts_lane : 0u;
+      bool queued = false;
+      if (__ballot(direct && fabsf(xm) < __uint_as_float(h0.w)) == 0) {
+        // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
+          const float D = fmaf(x, x, c1);
+          part[j] = fmaf(ay, __builtin_amdgcn_rcpf(D), part[j]);
+        }
+      } else {
+        const float hw2 = __uint_as_float(h1.w);
+        const bool mid = fabsf(xm) >= hw2;
+        queued = !mid && live && direct;
+        const float xe = mid ? xm : hw2, aye = mid ? ay : 0.0f;
+        // Both middle forms are u = x^2 + A, K A = (aye u + C) / (u^2 + B):
+        // the second-order shifted form, K A = (A y/sqrt(pi)) r / (r^2 - 3/2 r
+        // + 2 y^2 - 3/2) written about u = r - 3/4 (A = y^2 - 3/4, B = 2 y^2
+        // - 33/16, C = 3/4 aye), and the two-node form, K A = (A y/sqrt(pi))
+        // (w + 1) / (w^2 + 2 y^2) with w = r - 1/2 (A = y^2 - 1/2, B = 2 y^2,
+        // C = aye); the record's c1 picks one per lane (6 VALU per point).
+        const bool s2 = c1 < kF32S2Y2 - 1.5f;
+        const float A = s2 ? c1 + 0.75f : c1 + 1.0f;
+        const float B = s2 ? fmaf(2.0f, c1, 0.9375f) : fmaf(2.0f, c1, 3.0f);
+        const float C = s2 ? 0.75f * aye : aye;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xe);
+          const float u = fmaf(x, x, A);
+          const float D = fmaf(u, u, B);
+          part[j] = fmaf(fmaf(aye, u, C), __builtin_amdgcn_rcpf(D), part[j]);
+        }
+      }
+      // append the queued lanes' entries (x at the lane's first point, from
+      // the FP64 offset as the far test forms it; stage slot and lane) to
+      // this wavefront's queue; the count nq advances in every lane
+      const uint64_t msk = __ballot(queued);
+      if (msk) {   // uniform
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+        if (queued) {
+          const int pos = nq + (int)rank;   // < kQueueW: the loops drain before it fills
+          s_qx[wave][pos] = (float)(nu_f - c) * sv;
+          s_qi[wave][pos] = (uint16_t)((q << 6) | lane);
+        }
+        nq += (int)__popcll(msk);
+      }
+    };
+    // The queue, drained by this wavefront.  kSurrogate: every queued point
+    // goes to the network (inside its domain and gate) or to the full FP32
+    // dispatch.  Otherwise two passes, each dense and free of divergent
+    // regions.  Pass A: one entry per lane, all P points in the six-node
+    // form (<= 3.2e-6 outside gh6_core_r2, faddeeva.h); points inside (the
+    // core) get weight 0 and go to the wavefront's core list (x and the
+    // target kept in the entry).  Pass B: the core list, two points per lane
+    // per round, trapezoid + residue form.  Pass A's results reach the
+    // owning lane's sums through claimed plain LDS updates, pass B's
+    // through LDS float atomics.
+    auto drain = [&](const int n) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
+      __builtin_amdgcn_wave_barrier();
+      if constexpr (!kSurrogate) {
+        int ncore = 0;   // uniform
+        auto core_pass = [&](const int m) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
+          __builtin_amdgcn_wave_barrier();
+          // two points per lane per round, so the list, queue and record
+          // reads of both are in flight together; x as pass A formed it
+          for (int b = lane; b < m; b += 128) {
+            const bool two = b + 64 < m;
+            const uint32_t ca = s_ci[wave][b], cb = two ? s_ci[wave][b + 64] : ca;
+            const float xa0 = s_qx[wave][ca >> 3], xb0 = s_qx[wave][cb >> 3];
+            const uint32_t qa = s_qi[wave][ca >> 3], qb = s_qi[wave][cb >> 3];
+            const u32x4 ha1 = rb[3 * (qa >> 6) + 1], hb1 = rb[3 * (qb >> 6) + 1];   // hs, c1, ay, hw2
+            const u32x4 ha2 = rb[3 * (qa >> 6) + 2], hb2 = rb[3 * (qb >> 6) + 2];   // y, a, rho, mask
+            const float xa = fmaf((float)(ca & (P - 1)), __uint_as_float(ha1.x), xa0);
+            const float xb = fmaf((float)(cb & (P - 1)), __uint_as_float(hb1.x), xb0);
+            const float va = __uint_as_float(ha2.y) *
+                             K_cr32(xa, __uint_as_float(ha2.x), __uint_as_float(ha2.z));
+            const float vb = __uint_as_float(hb2.y) *
+                             K_cr32(xb, __uint_as_float(hb2.x), __uint_as_float(hb2.z));
+            atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + (qa & 63)], va);
+            if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + (qb & 63)], vb);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
+          __builtin_amdgcn_wave_barrier();
+        };
+        for (int e0 = 0; e0 < n; e0 += 64) {
+          const int e = e0 + lane;
+          const bool has = e < n;
+          float x0 = 0.0f;
+          uint32_t qi = 0;
+          if (has) {
+            x0 = s_qx[wave][e];
+            qi = s_qi[wave][e];
+          }
+          const int t = (int)(qi & 63);
+          const u32x4 h1 = rb[3 * (qi >> 6) + 1];   // hs, c1, ay, hw2
+          const float hs = __uint_as_float(h1.x), y2 = __uint_as_float(h1.y) + 1.5f;
+          // six-node form: A K = (2 A y / pi) sum_k W_k (r + t_k^2) / D_k, and
+          // A y = sqrt(pi) ay
+          const float ay = has ? __uint_as_float(h1.z) : 0.0f;
+          const float c2 = 2.0f * (float)kInvSqrtPi * ay;
+          const float fy2 = 4.0f * y2;
+          // core radius of the six-node form; y^2 = c1 + 3/2 carries y^2 to
+          // 1.2e-7, which only ever picks the larger radius
+          const float rcore = gh6_core_r2(y2);
+          // a product of three D ~ |z|^12 overflows FP32 beyond |z| = 1e3
+          // (broad lines, y > 1e3, or grids much coarser than a Doppler
+          // width): such points take the shifted one-node form, within 2e-6
+          // there
+          const float xf = fmaxf(fabsf(x0), fabsf(fmaf((float)(P - 1), hs, x0)));
+          const bool far_pts = __ballot(has && fmaf(xf, xf, y2) > 1e6f) != 0;
+          uint32_t cmask = 0;
+          float v[P];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, hs, x0);
+            const float x2 = x * x;
+            const float r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+            const bool core = has && r < rcore;
+            // (a select: at a core point the node sum may be infinite)
+            float g = c2 * gh6_sum32(r, xmy, fxy);
+            if (far_pts)   // uniform
+              g = r > 1e6f ? ay * __builtin_amdgcn_rcpf(r - 1.5f) : g;
+            v[j] = core ? 0.0f : g;
+            cmask |= core ? (1u << j) : 0u;
+          }
+          // Deliver the P values to the owner's accumulators with plain LDS
+          // reads and writes (LDS float atomics cost far more): in each
+          // round every pending lane writes its id to its owner's claim
+          // byte, the lanes that read their own id back hold distinct
+          // owners and update, the rest try again (entries of several
+          // records for one owner take one round each).
+          bool pending = has;
+          while (__ballot(pending)) {   // uniform
+            if (pending) s_claim[wave][t] = (uint8_t)lane;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (pending && s_claim[wave][t] == (uint8_t)lane) {
+              // all P reads first, then all P writes: one LDS round trip
+              float* const dst = s_acc + wave * 64 + t;
+              float cur[P];
+#pragma unroll
+              for (int j = 0; j < P; ++j) cur[j] = dst[j * kAccStride];
+#pragma unroll
+              for (int j = 0; j < P; ++j) dst[j * kAccStride] = cur[j] + v[j];
+              pending = false;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          }
+          // this lane's core points go to the list at an exclusive prefix sum
+          // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
+          const uint32_t cnt = __builtin_popcount(cmask);
+          uint32_t pre = 0, total = 0;
+#pragma unroll
+          for (int bit = 0; bit < 4; ++bit) {
+            const uint64_t m = __ballot((cnt >> bit) & 1u);
+            pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+            total += (uint32_t)__popcll(m) << bit;
+          }
+          if (total) {   // uniform
+            if (ncore + (int)total > kCoreCap) {
+              core_pass(ncore);
+              ncore = 0;
+            }
+            int off = ncore + (int)pre;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              if (cmask & (1u << j)) {
+                s_ci[wave][off] = (uint16_t)((uint32_t)e << 3 | (uint32_t)j);
+                ++off;
+              }
+            }
+            ncore += (int)total;
+            if constexpr (kCount) n_core += total;
+          }
+        }
+        if (ncore > 0) core_pass(ncore);
+      } else {
+        for (int e = lane; e < n; e += 64) {
+          const float x0 = s_qx[wave][e];
+          const uint32_t qi = s_qi[wave][e];
+          const MixedRec& R = s_rec[wave][qi >> 6];
+          const int tt = wave * 64 + (int)(qi & 63);   // owning thread
+          const float ac = R.a * (float)R.c;
+          int exact = 0;
+          for (int j = 0; j < P; ++j) {
+            const float x = fmaf((float)j, R.hs, x0);
+            if (surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
+            atomicAdd(&s_acc[j * kAccStride + tt], R.a * K_mixed(x, R.y));
+            ++exact;
+          }
+          if (a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
+        }
+        // network pass: 64 / P queue entries (64 points) per round (lane =
+        // entry (l / P), point (l % P)); all lanes stay active through the
+        // MFMA evaluation.
+        constexpr int kPer = 64 / P;
+        for (int g = 0; g * kPer < n; ++g) {
+          const int e = g * kPer + lane / P, j = lane % P;
+          bool use = false;
+          float x = 0.0f, y = 1.0f, am = 0.0f;
+          int tt = 0;
+          if (e < n) {
+            const uint32_t qi = s_qi[wave][e];
+            tt = wave * 64 + (int)(qi & 63);
+            const MixedRec& R = s_rec[wave][qi >> 6];
+            x = fmaf((float)j, R.hs, s_qx[wave][e]);
+            y = R.y;
+            am = R.a;
+            use = surrogate_point(x, y, am * (float)R.c, a.mlp_gate);
+          }
+          const uint64_t users = __ballot(use);
+          if (users == 0) continue;   // wavefront-uniform: nothing for the network here
+          if (a.mlp_count && lane == 0) atomicAdd(a.mlp_count, (unsigned long long)__popcll(users));
+          const float u = use ? mlp::input_u(x) : 0.0f;
+          const float v = use ? mlp::input_v(__log10f(y)) : 0.0f;
+          const float tv = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
+          if (use) atomicAdd(&s_acc[j * kAccStride + tt], am * mlp::kref(x, y) * __expf(tv));
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        part[j] += acc[j * kAccStride];
+        acc[j * kAccStride] = 0.0f;
+      }
+    };
+    auto fold = [&]() {
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        kahan_add(lay[j], layc[j], part[j]);
+        part[j] = 0.0f;
+      }
+    };
+
+    // ---- Phase 1: every group's own near range, eight records per group
+    // per stage.  The eight lanes of group g load the next eight records of
+    // its range (slots 8g .. 8g + 7), apply the exact FP64 tile-window test
+    // and stage them; in iteration i every lane evaluates its group's record
+    // 8g + i, so each iteration holds eight records and no lane idles while
+    // its group has records left.
+    for (int32_t base = 0; base < n_own; base += kStageG) {
+      const int n_it = __builtin_amdgcn_readfirstlane(min(kStageG, n_own - base));
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the previous stage's readers
+      const int slot = (int)(my_group * kStageG) + (lane & (kStageG - 1));
+      {
+        const int32_t r = my_lo + base + (lane & (kStageG - 1));
+        u32x4 w0 = null0, w1 = null1, w2 = null2;
+        if (r < my_hi) {
+          const u32x4* const src = (const u32x4*)(recs + kb + r);
+          const u32x4 v0 = src[0], v1 = src[1], v2 = src[2];
+          const double c = __hiloint2double((int)v0.y, (int)v0.x);
+          if (c >= lo_b && c <= hi_b) {   // (a skipped record's NaN centre fails)
+            w0 = v0;
+            w1 = v1;
+            // the LDS copy carries the core form's rho in ds1
+            w2 = v2;
+            w2.z = __float_as_uint(cr32_rho(__uint_as_float(v2.x)));
+          }
+        }
+        lds_u32x4* const dst = (lds_u32x4*)&s_rec_lds[slot];
+        dst[0] = w0;
+        dst[1] = w1;
+        dst[2] = w2;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // header and body of the next record read one iteration ahead
+      const volatile lds_u32x4* const pg = rb + 3 * (my_group * kStageG);
+      u32x4 h0 = pg[0], h1 = pg[1];
+      for (int it = 0; it < n_it; ++it) {
+        u32x4 n0 = h0, n1 = h1;
+        if (it + 1 < n_it) {
+          n0 = pg[3 * (it + 1)];
+          n1 = pg[3 * (it + 1) + 1];
+        }
+        // a null slot (past the group's range or outside the tile window)
+        // has hw2 = -1; such a lane is not counted and never queued
+        record((int)(my_group * kStageG) + it, h0, h1, __uint_as_float(h1.w) >= 0.0f);
+        h0 = n0;
+        h1 = n1;
+        if (nq > kQueueW - 64) {   // uniform: one more record could not be taken
+          if constexpr (kCount) n_queued += (uint64_t)nq;
+          drain(nq);
+          nq = 0;
+          if (it + 1 < n_it) {   // not held in registers through the drain
+            h0 = pg[3 * (it + 1)];
+            h1 = pg[3 * (it + 1) + 1];
+          }
+        }
+      }
+      if (nq > 0) {   // uniform: the stage is overwritten next
+        if constexpr (kCount) n_queued += (uint64_t)nq;
+        drain(nq);
+        nq = 0;
+      }
+      if (((base + kStageG) & (kFoldEvery - 1)) == 0) fold();
+    }
+    fold();
+
+    // ---- Phase 2: pairs the group series left to this kernel.  Outside a
+    // group's own range a record is evaluated point by point where neither
+    // the one-node nor the middle-tier form holds at every point of the group
+    // (group_one_node_holds with group_thr and group_mid_thr, the same tests
+    // in the same arithmetic as k_far_series_mixed's group level, so each
+    // (group, record) pair is taken exactly once); the series kernel counts
+    // them per (layer, wavefront), and the scan is skipped where there are
+    // none.  Lane l tests record base + l of the wavefront's near range for
+    // all eight groups; records some group needs are packed into the stage
+    // with that mask, and every lane takes each staged record, with weight 0
+    // where its group is not in the mask.
+    const uint32_t n_left = a.deferred[(int64_t)k * (a.n_wg * kWaves) + wg * kWaves + wave];
+    for (int32_t base = nlo; n_left != 0 && base < nhi; base += kStage) {
       const int n_here = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
-      // stage: lane l reads record base + l; the records some group of this
-      // wavefront evaluates point by point go to the stage, packed (the
-      // previous stage's readers are this wavefront's own, earlier in
-      // program order)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       // the record as three 16-byte words in registers: (c, s, hw1),
-      // (hs, c1, ay, hw2), (y, a, ds1, sh)
+      // (hs, c1, ay, hw2), (y, a, ds1, dm)
       u32x4 w0 = {0u, 0u, 0u, 0u}, w1 = w0, w2 = w0;
       uint32_t gmask = 0;
       if (lane < n_here) {
@@ -1333,28 +1745,25 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
         w1 = src[1];
         w2 = src[2];
         const double c = __hiloint2double((int)w0.y, (int)w0.x);
-        // The groups that evaluate the record point by point: it lies in
-        // the group's range, or the one-node form fails at some point of the
-        // group (the same test, in the same arithmetic, as
-        // k_far_series_mixed's group level); none for a record outside the
-        // tile window (or skipped).
         if (c >= lo_b && c <= hi_b) {
           const int32_t r = base + lane;
           const float d0 = group_d0(gt, c), thr = group_thr(gt, __uint_as_float(w2.z));
+          const float thr2 = group_mid_thr(gt, __uint_as_float(w2.w));
 #pragma unroll
           for (int grp = 0; grp < kGroupsPerWave; ++grp) {
             const int32_t g_lo = min(max(gl[grp], nlo), nhi), g_hi = max(min(gh[grp], nhi), g_lo);
-            const bool direct = (r >= g_lo && r < g_hi) ||
-                                !group_one_node_holds(d0, thr, group_offset(gt, grp));
-            gmask |= direct ? (1u << grp) : 0u;
+            const float goff = group_offset(gt, grp);
+            const bool deferred = (r < g_lo || r >= g_hi) && !group_one_node_holds(d0, thr, goff) &&
+                                  !group_one_node_holds(d0, thr2, goff);
+            gmask |= deferred ? (1u << grp) : 0u;
           }
         }
       }
       const uint64_t keep = __ballot(gmask != 0);
       const int cnt = (int)__popcll(keep);   // records in the stage (uniform)
+      if (cnt == 0) continue;                // (the stage was not written)
       if (gmask != 0) {
-        // the LDS copy carries the core form's rho = exp(-2 pi y / h) in ds1
-        // and the mask in sh (both serve only the far-field series)
+        // the LDS copy carries the core form's rho in ds1 and the mask in dm
         w2.z = __float_as_uint(cr32_rho(__uint_as_float(w2.x)));
         w2.w = gmask;
         lds_u32x4* const dst = (lds_u32x4*)&s_rec_lds[__builtin_amdgcn_mbcnt_hi(
@@ -1365,318 +1774,23 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      int nq = 0;   // queue entries not yet drained (uniform)
-      // Per record: c, s, hw1 in one 16-byte LDS read (prefetched two
-      // records ahead), and hs, c1, ay, hw2 in the next 16 bytes; volatile
-      // keeps each a single ds_read_b128.  Three tiers per (record, thread):
-      // the shifted one-node form, the middle tier (second-order shifted form
-      // for y^2 < kF32S2Y2, else two-node), and the work queue.
-      const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)s_rec_lds;
-      auto record = [&](const int q, const u32x4 h0, const volatile lds_u32x4* const body) {
-        // the groups that evaluate the record here (the series kernel has it
-        // for the others: weight 0 there)
+      for (int q = 0; q < cnt; ++q) {
+        const volatile lds_u32x4* const body = rb + 3 * q + 1;
         const uint32_t gm = __builtin_amdgcn_readfirstlane(((const volatile lds_u32*)body)[7]);
-        const bool direct = (gm >> my_group) & 1u;
-        const double c = __hiloint2double((int)h0.y, (int)h0.x);
-        const float sv = __uint_as_float(h0.z);
-        const float xm = (float)(nu_m - c) * sv;
-        const u32x4 h1 = *body;
-        const float hs = __uint_as_float(h1.x), c1 = __uint_as_float(h1.y);
-        const float ay = direct ? __uint_as_float(h1.z) : 0.0f;
-        if constexpr (kCount) n_pts += direct && live ? (uint64_t)pts_lane : 0u;
-        bool queued = false;
-        // One tier per record for the whole wavefront: the middle tier holds
-        // wherever the one-node form does, so a record for which some lane
-        // needs it runs it in every lane.  Lanes that need the queue instead
-        // run it too, at a safe x (the run's middle at |x| = hw2) with weight
-        // 0, so that no branch splits the wavefront.
-        if (__ballot(direct && fabsf(xm) < __uint_as_float(h0.w)) == 0) {
-          // shifted one-node form (faddeeva.h): K A = (A y/sqrt(pi)) / (x^2 + y^2 - 3/2)
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xm);
-            const float D = fmaf(x, x, c1);
-            part[j] = fmaf(ay, __builtin_amdgcn_rcpf(D), part[j]);
-          }
-        } else {
-          const float hw2 = __uint_as_float(h1.w);
-          const bool mid = fabsf(xm) >= hw2;
-          queued = !mid && live && direct;
-          const float xe = mid ? xm : hw2, aye = mid ? ay : 0.0f;
-          // c1 is the record's (the same in every lane): compare its bits
-          // (as signed integers, which order floats of either sign against a
-          // positive bound) in a scalar register
-          if (__builtin_amdgcn_readfirstlane(__float_as_int(c1)) <
-              __float_as_int(kF32S2Y2 - 1.5f)) {
-            // second-order shifted form: K A = (A y/sqrt(pi)) r / (r^2 - 3/2 r + 2 y^2 - 3/2),
-            // written about u = r - 3/4 = x^2 + y^2 - 3/4: the denominator is
-            // u^2 + 2 y^2 - 33/16 and the numerator ay u + 3/4 ay (6 VALU per point)
-            const float um = c1 + 0.75f, k2 = fmaf(2.0f, c1, 0.9375f), ay34 = 0.75f * aye;
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xe);
-              const float u = fmaf(x, x, um);
-              const float D = fmaf(u, u, k2);
-              part[j] = fmaf(fmaf(aye, u, ay34), __builtin_amdgcn_rcpf(D), part[j]);
-            }
-          } else {
-            // two-node form: K A = (A y/sqrt(pi)) (w + 1) / (w^2 + 2 y^2), w = x^2 + y^2 - 1/2
-            const float cw = c1 + 1.0f, d2 = fmaf(2.0f, c1, 3.0f);
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-              const float x = fmaf((float)j - 0.5f * (float)(P - 1), hs, xe);
-              const float w = fmaf(x, x, cw);
-              const float D = fmaf(w, w, d2);
-              const float N = fmaf(aye, w, aye);
-              part[j] = fmaf(N, __builtin_amdgcn_rcpf(D), part[j]);
-            }
-          }
-        }
-        // append the queued lanes' entries (x at the lane's first point, from
-        // the FP64 offset as the far test forms it; record and lane) to this
-        // wavefront's queue; the count nq advances in every lane
-        const uint64_t msk = __ballot(queued);
-        if (msk) {   // uniform
-          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-          if (queued) {
-            const int pos = nq + (int)rank;   // < kQueueW: the loop drains before it fills
-            s_qx[wave][pos] = (float)(nu_f - c) * sv;
-            s_qi[wave][pos] = (uint16_t)((q << 6) | lane);
-          }
-          nq += (int)__popcll(msk);
-        }
-      };
-      // The queue, drained by this wavefront.  kSurrogate: every queued point
-      // goes to the network (inside its domain and gate) or to the full FP32
-      // dispatch.  Otherwise two passes, each dense and free of divergent
-      // regions.  Pass A: one entry per lane, all P points in the six-node
-      // form (<= 3.2e-6 outside gh6_core_r2, faddeeva.h); points inside (the
-      // core) get weight 0 and go to the wavefront's core list (x and the
-      // target kept in the entry).  Pass B: the core list, two points per lane
-      // per round, trapezoid + residue form.  Pass A's results reach the
-      // owning lane's sums through claimed plain LDS updates, pass B's
-      // through LDS float atomics.
-      auto drain = [&](const int n) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
-        __builtin_amdgcn_wave_barrier();
-        if constexpr (!kSurrogate) {
-          int ncore = 0;   // uniform
-          auto core_pass = [&](const int m) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
-            __builtin_amdgcn_wave_barrier();
-            // two points per lane per round, so the list, queue and record
-            // reads of both are in flight together; x as pass A formed it
-            for (int b = lane; b < m; b += 128) {
-              const bool two = b + 64 < m;
-              const uint32_t ca = s_ci[wave][b], cb = two ? s_ci[wave][b + 64] : ca;
-              const float xa0 = s_qx[wave][ca >> 3], xb0 = s_qx[wave][cb >> 3];
-              const uint32_t qa = s_qi[wave][ca >> 3], qb = s_qi[wave][cb >> 3];
-              const u32x4 ha1 = rb[3 * (qa >> 6) + 1], hb1 = rb[3 * (qb >> 6) + 1];   // hs, c1, ay, hw2
-              const u32x4 ha2 = rb[3 * (qa >> 6) + 2], hb2 = rb[3 * (qb >> 6) + 2];   // y, a, rho, sh
-              const float xa = fmaf((float)(ca & (P - 1)), __uint_as_float(ha1.x), xa0);
-              const float xb = fmaf((float)(cb & (P - 1)), __uint_as_float(hb1.x), xb0);
-              const float va = __uint_as_float(ha2.y) *
-                               K_cr32(xa, __uint_as_float(ha2.x), __uint_as_float(ha2.z));
-              const float vb = __uint_as_float(hb2.y) *
-                               K_cr32(xb, __uint_as_float(hb2.x), __uint_as_float(hb2.z));
-              atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + (qa & 63)], va);
-              if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + (qb & 63)], vb);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
-            __builtin_amdgcn_wave_barrier();
-          };
-          for (int e0 = 0; e0 < n; e0 += 64) {
-            const int e = e0 + lane;
-            const bool has = e < n;
-            float x0 = 0.0f;
-            uint32_t qi = 0;
-            if (has) {
-              x0 = s_qx[wave][e];
-              qi = s_qi[wave][e];
-            }
-            const int t = (int)(qi & 63);
-            const u32x4 h1 = rb[3 * (qi >> 6) + 1];   // hs, c1, ay, hw2
-            const float hs = __uint_as_float(h1.x), y2 = __uint_as_float(h1.y) + 1.5f;
-            // six-node form: A K = (2 A y / pi) sum_k W_k (r + t_k^2) / D_k, and
-            // A y = sqrt(pi) ay
-            const float ay = has ? __uint_as_float(h1.z) : 0.0f;
-            const float c2 = 2.0f * (float)kInvSqrtPi * ay;
-            const float fy2 = 4.0f * y2;
-            // core radius of the six-node form; y^2 = c1 + 3/2 carries y^2 to
-            // 1.2e-7, which only ever picks the larger radius
-            const float rcore = gh6_core_r2(y2);
-            // a product of three D ~ |z|^12 overflows FP32 beyond |z| = 1e3
-            // (broad lines, y > 1e3, or grids much coarser than a Doppler
-            // width): such points take the shifted one-node form, within 2e-6
-            // there
-            const float xf = fmaxf(fabsf(x0), fabsf(fmaf((float)(P - 1), hs, x0)));
-            const bool far_pts = __ballot(has && fmaf(xf, xf, y2) > 1e6f) != 0;
-            uint32_t cmask = 0;
-            float v[P];
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-              const float x = fmaf((float)j, hs, x0);
-              const float x2 = x * x;
-              const float r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
-              const bool core = has && r < rcore;
-              // (a select: at a core point the node sum may be infinite)
-              float g = c2 * gh6_sum32(r, xmy, fxy);
-              if (far_pts)   // uniform
-                g = r > 1e6f ? ay * __builtin_amdgcn_rcpf(r - 1.5f) : g;
-              v[j] = core ? 0.0f : g;
-              cmask |= core ? (1u << j) : 0u;
-            }
-            // Deliver the P values to the owner's accumulators with plain LDS
-            // reads and writes (LDS float atomics cost far more): in each
-            // round every pending lane writes its id to its owner's claim
-            // byte, the lanes that read their own id back hold distinct
-            // owners and update, the rest try again (entries of several
-            // records for one owner take one round each).
-            bool pending = has;
-            while (__ballot(pending)) {   // uniform
-              if (pending) s_claim[wave][t] = (uint8_t)lane;
-              __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-              if (pending && s_claim[wave][t] == (uint8_t)lane) {
-                // all P reads first, then all P writes: one LDS round trip
-                float* const dst = s_acc + wave * 64 + t;
-                float cur[P];
-#pragma unroll
-                for (int j = 0; j < P; ++j) cur[j] = dst[j * kAccStride];
-#pragma unroll
-                for (int j = 0; j < P; ++j) dst[j * kAccStride] = cur[j] + v[j];
-                pending = false;
-              }
-              __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            }
-            // this lane's core points go to the list at an exclusive prefix sum
-            // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
-            const uint32_t cnt = __builtin_popcount(cmask);
-            uint32_t pre = 0, total = 0;
-#pragma unroll
-            for (int bit = 0; bit < 4; ++bit) {
-              const uint64_t m = __ballot((cnt >> bit) & 1u);
-              pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
-              total += (uint32_t)__popcll(m) << bit;
-            }
-            if (total) {   // uniform
-              if (ncore + (int)total > kCoreCap) {
-                core_pass(ncore);
-                ncore = 0;
-              }
-              int off = ncore + (int)pre;
-#pragma unroll
-              for (int j = 0; j < P; ++j) {
-                if (cmask & (1u << j)) {
-                  s_ci[wave][off] = (uint16_t)((uint32_t)e << 3 | (uint32_t)j);
-                  ++off;
-                }
-              }
-              ncore += (int)total;
-              if constexpr (kCount) n_core += total;
-            }
-          }
-          if (ncore > 0) core_pass(ncore);
-        } else {
-          for (int e = lane; e < n; e += 64) {
-            const float x0 = s_qx[wave][e];
-            const uint32_t qi = s_qi[wave][e];
-            const MixedRec& R = s_rec[wave][qi >> 6];
-            const int tt = wave * 64 + (int)(qi & 63);   // owning thread
-            const float ac = R.a * (float)R.c;
-            int exact = 0;
-            for (int j = 0; j < P; ++j) {
-              const float x = fmaf((float)j, R.hs, x0);
-              if (surrogate_point(x, R.y, ac, a.mlp_gate)) continue;
-              atomicAdd(&s_acc[j * kAccStride + tt], R.a * K_mixed(x, R.y));
-              ++exact;
-            }
-            if (a.exact_count) atomicAdd(a.exact_count, (unsigned long long)exact);
-          }
-          // network pass: 64 / P queue entries (64 points) per round (lane =
-          // entry (l / P), point (l % P)); all lanes stay active through the
-          // MFMA evaluation.
-          constexpr int kPer = 64 / P;
-          for (int g = 0; g * kPer < n; ++g) {
-            const int e = g * kPer + lane / P, j = lane % P;
-            bool use = false;
-            float x = 0.0f, y = 1.0f, am = 0.0f;
-            int tt = 0;
-            if (e < n) {
-              const uint32_t qi = s_qi[wave][e];
-              tt = wave * 64 + (int)(qi & 63);
-              const MixedRec& R = s_rec[wave][qi >> 6];
-              x = fmaf((float)j, R.hs, s_qx[wave][e]);
-              y = R.y;
-              am = R.a;
-              use = surrogate_point(x, y, am * (float)R.c, a.mlp_gate);
-            }
-            const uint64_t users = __ballot(use);
-            if (users == 0) continue;   // wavefront-uniform: nothing for the network here
-            if (a.mlp_count && lane == 0) atomicAdd(a.mlp_count, (unsigned long long)__popcll(users));
-            const float u = use ? mlp::input_u(x) : 0.0f;
-            const float v = use ? mlp::input_v(__log10f(y)) : 0.0f;
-            const float tv = mlp::wave_eval(u, v, a.mlp_params, a.mlp_wbf);
-            if (use) atomicAdd(&s_acc[j * kAccStride + tt], am * mlp::kref(x, y) * __expf(tv));
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int j = 0; j < P; ++j) {
-          part[j] += acc[j * kAccStride];
-          acc[j * kAccStride] = 0.0f;
-        }
-      };
-      // Two records per iteration, each header read two records ahead into
-      // its own registers (no copies between iterations); kStage entries
-      // exist, so record 1 is readable even when cnt == 1.  The record
-      // address lives in a VGPR (the empty asm makes it one) so every LDS
-      // read uses an immediate offset instead of a copy from an SGPR.  The
-      // queue is drained when it could not take two more records, and at the
-      // end of the stage.
-      u32x4 ha = rb[0], hb = rb[3];
-      const volatile lds_u32x4* pr = rb;
-      asm volatile("" : "+v"(pr));
-      int q = 0;
-      for (;;) {
-        for (; q < cnt; q += 2, pr += 6) {
-          record(q, ha, pr + 1);
-          if (q + 2 < cnt) ha = pr[6];
-          if (q + 1 < cnt) {
-            record(q + 1, hb, pr + 4);
-            if (q + 3 < cnt) hb = pr[9];
-          }
-          if (((q + 2) & (kFoldEvery - 1)) == 0) {   // after records 31, 63
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-              kahan_add(lay[j], layc[j], part[j]);
-              part[j] = 0.0f;
-            }
-          }
-          if (nq > kQueueW - 128) {   // uniform
-            q += 2;
-            pr += 6;
-            break;
-          }
-        }
-        if (nq > 0) {   // uniform
+        record(q, rb[3 * q], *body, (gm >> my_group) & 1u);
+        if (((q + 1) & (kFoldEvery - 1)) == 0) fold();
+        if (nq > kQueueW - 64) {   // uniform
           if constexpr (kCount) n_queued += (uint64_t)nq;
           drain(nq);
           nq = 0;
-          // the next two records' headers again from LDS (not held in
-          // registers through the drain)
-          if (q < cnt) ha = pr[0];
-          if (q + 1 < cnt) hb = pr[3];
         }
-        if (q >= cnt) break;
       }
-#pragma unroll
-      for (int j = 0; j < P; ++j) {
-        kahan_add(lay[j], layc[j], part[j]);
-        part[j] = 0.0f;
+      if (nq > 0) {   // uniform
+        if constexpr (kCount) n_queued += (uint64_t)nq;
+        drain(nq);
+        nq = 0;
       }
+      fold();
     }
     const double T = a.t_k[k];
     bool bad = false;

@@ -236,3 +236,111 @@ def test_adjoint_series_22_terms(y):
     assert w1 <= 1.1e-9, w1
     assert w2 <= 1.8e-7, w2
     assert w3 <= 1.8e-7, w3
+
+
+# ---- the mixed group level's middle-tier form (k_far_series_mixed
+# series_mid): where the one-node form fails at a 64-point group but the
+# middle tier holds at all its points, the form (ay u + C) / (u^2 + B),
+# u = x^2 + A, is summed as its Taylor series, 18 terms, from the quartic
+# recurrence of D Q = N; claims: truncation <= 1e-7 of the form, and the
+# kernel's FP32 arithmetic within 1.1e-6, wherever the group level uses it
+# (|xm| >= 3 sh outside the group's near range, |xm| - sh >= the form's
+# |x| threshold).
+S2_COEF32 = 171875.0     # kF32S2Coef
+S2_Y2 = 155.0            # kF32S2Y2
+
+
+def middle_constants(c1, ay):
+    """(A, B, C) of the middle tier: second-order shifted form about
+    u = r - 3/4 for y^2 < kF32S2Y2, else the two-node form about w = r - 1/2."""
+    if c1 < S2_Y2 - 1.5:
+        return c1 + 0.75, 2.0 * c1 + 0.9375, 0.75 * ay
+    return c1 + 1.0, 2.0 * c1 + 3.0, ay
+
+
+def middle_threshold(y2):
+    r2 = (S2_COEF32 * (9.0 * y2 + 5.25)) ** (1.0 / 3.0) if y2 - 1.5 < S2_Y2 - 1.5 else 625.0
+    return math.sqrt(max(r2 - y2, 0.0))
+
+
+def series_middle_kernel(xm, sh, c1, ay, f32):
+    """The kernel's recurrence, in float32 (each fmaf rounded once) or float64."""
+    T = np.float32 if f32 else float
+    fma = (lambda a, b, c: np.float32(float(a) * float(b) + float(c))) if f32 else (lambda a, b, c: a * b + c)
+    xm, sh, c1, ay = T(xm), T(sh), T(c1), T(ay)
+    s2 = c1 < T(S2_Y2 - 1.5)
+    A = c1 + T(0.75) if s2 else c1 + T(1.0)
+    B = fma(T(2.0), c1, T(0.9375)) if s2 else fma(T(2.0), c1, T(3.0))
+    Cn = T(0.75) * ay if s2 else ay
+    u0, u1, u2 = fma(xm, xm, A), T(2.0) * xm * sh, sh * sh
+    r = T(1.0) / fma(u0, u0, B)
+    e1, e2 = T(2.0) * u0 * u1 * r, fma(u1, u1, T(2.0) * u0 * u2) * r
+    e3, e4 = T(2.0) * u1 * u2 * r, u2 * u2 * r
+    q = [fma(ay, u0, Cn) * r]
+    q.append(fma(-e1, q[0], ay * u1 * r))
+    q.append(fma(-e2, q[0], fma(-e1, q[1], ay * u2 * r)))
+    q.append(-fma(e1, q[2], fma(e2, q[1], e3 * q[0])))
+    for _ in range(4, TERMS32):
+        q.append(-fma(e1, q[-1], fma(e2, q[-2], fma(e3, q[-3], e4 * q[-4]))))
+    return np.array([float(v) for v in q])
+
+
+def middle_direct(x, y, ay):
+    """The two closed forms (faddeeva.h), float64: second-order shifted
+    ay r / (r^2 - 3/2 r + 2 y^2 - 3/2) or two-node ay (w + 1) / (w^2 + 2 y^2)."""
+    y2 = y * y
+    r = x * x + y2
+    if y2 < S2_Y2:
+        return ay * r / (r * r - 1.5 * r + 2.0 * y2 - 1.5)
+    w = r - 0.5
+    return ay * (w + 1.0) / (w * w + 2.0 * y2)
+
+
+def test_middle_constants_are_the_two_forms():
+    x = np.linspace(10.0, 400.0, 64)
+    for y in (1e-3, 0.7, 3.0, 12.4, 12.5, 40.0):
+        c1 = y * y - 1.5
+        A, B, C = middle_constants(c1, 2.0)
+        u = x * x + A
+        assert np.allclose((2.0 * u + C) / (u * u + B), middle_direct(x, y, 2.0), rtol=1e-12)
+
+
+def test_middle_recurrence_is_the_rational_series():
+    for xm, sh, y in [(40.0, 12.5, 0.01), (90.0, 3.0, 8.0), (-700.0, 200.0, 30.0)]:
+        c1 = y * y - 1.5
+        A, B, C = middle_constants(c1, 1.5)
+        u0, u1, u2 = xm * xm + A, 2.0 * xm * sh, sh * sh
+        den = [u0 * u0 + B, 2.0 * u0 * u1, u1 * u1 + 2.0 * u0 * u2, 2.0 * u1 * u2, u2 * u2]
+        num = [1.5 * u0 + C, 1.5 * u1, 1.5 * u2]
+        ref = series_rational(num, den, TERMS32)
+        assert np.allclose(series_middle_kernel(xm, sh, c1, 1.5, False), ref, rtol=1e-11, atol=0.0)
+
+
+@pytest.mark.parametrize("y", [1e-4, 0.05, 0.5, 1.0, 1.0155, 1.3, 3.0, 10.0, 12.4, 12.5, 20.0, 40.0, 150.0])
+def test_middle_series_18_terms(y):
+    y2 = y * y
+    c1 = y2 - 1.5
+    x_m = middle_threshold(y2)
+    w64 = w32 = 0.0
+    for sh in np.geomspace(0.01, 3000.0, 41):
+        for xm0 in far_points(sh, x_m):
+            for xm in (xm0, -xm0):
+                f = middle_direct(xm + sh * TAU, y, 1.0)
+                p64 = poly(series_middle_kernel(xm, sh, c1, 1.0, False), TAU)
+                p32 = poly(series_middle_kernel(xm, sh, c1, 1.0, True), TAU)
+                w64 = max(w64, float(np.max(np.abs(p64 / f - 1.0))))
+                w32 = max(w32, float(np.max(np.abs(p32 / f - 1.0))))
+    assert w64 <= 1e-7, w64
+    assert w32 <= 1.1e-6, w32
+
+
+def test_middle_series_fails_nearer_than_its_threshold():
+    # the threshold matters: a record at |xm| - sh well inside the form's
+    # |x| threshold (small y, real poles of the second-order form near |x| = 1.5)
+    # (outside the near range, |xm| >= 3 sh, but at |x| < 3 where the poles
+    # lie within |tau| < 1.3)
+    sh, y = 0.6, 1e-4
+    xm = 2.2
+    f = middle_direct(xm + sh * TAU, y, 1.0)
+    p = poly(series_middle_kernel(xm, sh, y * y - 1.5, 1.0, False), TAU)
+    assert np.max(np.abs(p / f - 1.0)) > 1e-3
