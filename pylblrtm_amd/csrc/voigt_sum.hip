@@ -46,7 +46,7 @@ constexpr int kPointsFp64 = 8;  // grid points per thread of the FP64 line sums
 // radius (gh6_core_r2, 3.6 to 7 by y) take the trapezoid + residue form; a
 // wavefront's list of such points holds kCoreCap entries.
 constexpr int kCoreCap = 512;   // >= the 64 x 8 points one pass-A round can add
-constexpr int kQueueW = 512;     // queue entries per wavefront (the loop drains at kQueueW - 128)
+constexpr int kQueueW = 448;     // queue entries per wavefront (the loops drain past kQueueW - 64)
 constexpr int kAccStride = 256;  // s_acc row length of the mixed line sums
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -1316,7 +1316,8 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
   __shared__ uint16_t s_qi[kWaves][kQueueW];
   __shared__ float s_acc[kAccStride * P];
   // queued points in the core (gh6_core_r2): (queue entry << 3) | point
-  __shared__ uint16_t s_ci[kWaves][kSurrogate ? 1 : kCoreCap];
+  // (queue entry << 3) | point, overwritten in place by the point's value
+  __shared__ uint32_t s_ci[kWaves][kSurrogate ? 1 : kCoreCap];
   // claim byte per owner lane (the drain's plain read-modify-write rounds)
   __shared__ uint8_t s_claim[kWaves][64];
 
@@ -1467,19 +1468,19 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
     };
     // The queue, drained by this wavefront.  kSurrogate: every queued point
     // goes to the network (inside its domain and gate) or to the full FP32
-    // dispatch.  Otherwise two passes, each dense and free of divergent
-    // regions.  Pass A: one entry per lane, all P points in the six-node
-    // form (<= 3.2e-6 outside gh6_core_r2, faddeeva.h); points inside (the
-    // core) get weight 0 and go to the wavefront's core list (x and the
-    // target kept in the entry).  Pass B: the core list, two points per lane
-    // per round, trapezoid + residue form.  Pass A's results reach the
-    // owning lane's sums through claimed plain LDS updates, pass B's
-    // through LDS float atomics.
+    // dispatch.  Otherwise two passes per round of 64 entries, each dense and
+    // free of divergent regions.  Pass A: one entry per lane, all P points
+    // in the six-node form (<= 3.2e-6 outside gh6_core_r2, faddeeva.h);
+    // points inside (the core) go to the wavefront's core list (x and the
+    // target kept in the entry).  Pass B: the round's core list, two points
+    // per lane per round, trapezoid + residue form, each value written over
+    // its list slot, from where the entry's lane takes it back.  The
+    // entry's P values then reach the owning lane's sums through claimed
+    // plain LDS updates.
     auto drain = [&](const int n) {
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
       __builtin_amdgcn_wave_barrier();
       if constexpr (!kSurrogate) {
-        int ncore = 0;   // uniform
         auto core_pass = [&](const int m) {
           __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
           __builtin_amdgcn_wave_barrier();
@@ -1498,10 +1499,10 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
                              K_cr32(xa, __uint_as_float(ha2.x), __uint_as_float(ha2.z));
             const float vb = __uint_as_float(hb2.y) *
                              K_cr32(xb, __uint_as_float(hb2.x), __uint_as_float(hb2.z));
-            atomicAdd(&s_acc[(ca & (P - 1)) * kAccStride + wave * 64 + (qa & 63)], va);
-            if (two) atomicAdd(&s_acc[(cb & (P - 1)) * kAccStride + wave * 64 + (qb & 63)], vb);
+            s_ci[wave][b] = __float_as_uint(va);   // (slots b, b + 64: this lane's own)
+            if (two) s_ci[wave][b + 64] = __float_as_uint(vb);
           }
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the values
           __builtin_amdgcn_wave_barrier();
         };
         for (int e0 = 0; e0 < n; e0 += 64) {
@@ -1545,6 +1546,38 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             v[j] = core ? 0.0f : g;
             cmask |= core ? (1u << j) : 0u;
           }
+          // this lane's core points go to the list at an exclusive prefix sum
+          // of the lanes' counts (bit planes of the 0..8 count, one ballot
+          // each); pass B evaluates them and the lane reads its values back
+          const uint32_t cnt = __builtin_popcount(cmask);
+          uint32_t pre = 0, total = 0;
+#pragma unroll
+          for (int bit = 0; bit < 4; ++bit) {
+            const uint64_t m = __ballot((cnt >> bit) & 1u);
+            pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+            total += (uint32_t)__popcll(m) << bit;
+          }
+          if (total) {   // uniform (<= 64 P <= kCoreCap)
+            int off = (int)pre;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              if (cmask & (1u << j)) {
+                s_ci[wave][off] = (uint32_t)e << 3 | (uint32_t)j;
+                ++off;
+              }
+            }
+            core_pass((int)total);
+            off = (int)pre;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              if (cmask & (1u << j)) {
+                v[j] = __uint_as_float(s_ci[wave][off]);
+                ++off;
+              }
+            }
+            if constexpr (kCount) n_core += total;
+          }
           // Deliver the P values to the owner's accumulators with plain LDS
           // reads and writes (LDS float atomics cost far more): in each
           // round every pending lane writes its id to its owner's claim
@@ -1567,35 +1600,7 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
           }
-          // this lane's core points go to the list at an exclusive prefix sum
-          // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
-          const uint32_t cnt = __builtin_popcount(cmask);
-          uint32_t pre = 0, total = 0;
-#pragma unroll
-          for (int bit = 0; bit < 4; ++bit) {
-            const uint64_t m = __ballot((cnt >> bit) & 1u);
-            pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
-            total += (uint32_t)__popcll(m) << bit;
-          }
-          if (total) {   // uniform
-            if (ncore + (int)total > kCoreCap) {
-              core_pass(ncore);
-              ncore = 0;
-            }
-            int off = ncore + (int)pre;
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-              if (cmask & (1u << j)) {
-                s_ci[wave][off] = (uint16_t)((uint32_t)e << 3 | (uint32_t)j);
-                ++off;
-              }
-            }
-            ncore += (int)total;
-            if constexpr (kCount) n_core += total;
-          }
         }
-        if (ncore > 0) core_pass(ncore);
       } else {
         for (int e = lane; e < n; e += 64) {
           const float x0 = s_qx[wave][e];
