@@ -142,6 +142,13 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
     far_flop = (kw["slice_series"] * fp["slice"]
                 + (kw["wave_series"] + kw.get("group_series", 0) - g_mid) * fp["wave"]
                 + g_mid * fp.get("group_middle", 0) + kw["exception_evals"] * FLOP_PER_EVAL)
+    if base == "mixed" and kw.get("series_terms"):
+        # the mixed kernel sums only the terms each round's nearest record
+        # needs: a one-node pair costs its 12-FLOP set-up plus 4 per term
+        # after the first two (76 at 18 terms)
+        one_node = kw["slice_series"] + kw["wave_series"] + kw.get("group_series", 0) - g_mid
+        far_flop = (4 * one_node + 4 * kw["series_terms"] + g_mid * fp["group_middle"]
+                    + kw["exception_evals"] * FLOP_PER_EVAL)
     kern = {
         f"k_far_series_{base}": {
             "ms": far_ms, "units": pairs,
@@ -150,6 +157,7 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
             "group_pairs": kw.get("group_series", 0), "group_pairs_middle_form": g_mid,
             "exception_point_evals": kw["exception_evals"],
             "flop_per_unit": far_flop / max(pairs, 1),
+            "series_terms_summed": kw.get("series_terms"),
             "flop_per_pair": dict(fp)},
         near_name: {"ms": near_ms, "units": kw["near_evals"], "unit": "point evaluations",
                     "flop_per_unit": FLOP_PER_EVAL, "queued_near_centre_pairs": kw["queued"],

@@ -298,6 +298,8 @@ int lbl_last_timing_epilogue(double *ms_epilogue);
  *   near field (neither the one-node nor the middle-tier form holds at every
  *   point of the group), [12] the near field's point evaluations of [11]
  *   (part of [0]), [13] the pairs of [10] summed in the middle-tier form
+ *   [14] LBL_MIXED: series terms summed over the one-node pairs of [1],
+ *   [2] and [10] (each pair takes the terms its distance needs)
  * All counts are 0 after a call made without counting. */
 void lbl_enable_work_count(int on);
 int lbl_last_work_count(uint64_t *counts, int32_t n);

@@ -317,6 +317,7 @@ enum WorkCount {
   kWorkGroupDeferred,       // ... left to the near field (the form fails at the group)
   kWorkGroupDeferredEvals,  // the near field's point evaluations of those
   kWorkGroupMiddle,         // group series pairs in the middle-tier form (part of kWorkGroupSeries)
+  kWorkSeriesTerms,         // mixed: terms summed over the one-node pairs of all levels
   kWorkSlots
 };
 

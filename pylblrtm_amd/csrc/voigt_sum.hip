@@ -892,9 +892,10 @@ void k_far_series_mixed(const SumArgs a) {
     return f;
   };
   const float Hw = wave_half_span32(a.step);   // the series' unit at the wavefront level is s Hw
-  // adds record m's series about a middle point at offset d = nu_m - c (units
-  // of the level: sh scaled by the caller) to C (Q_0, Q_1 into C[0], C[1])
-  auto series = [&](float (&C)[kN], const FarRec& m, float d, float sh) {
+  // adds the first nt terms of record m's series about a middle point at
+  // offset d = nu_m - c (units of the level: sh scaled by the caller) to C
+  // (Q_0, Q_1 into C[0], C[1]); nt is wavefront-uniform (series_terms32)
+  auto series = [&](float (&C)[kN], const FarRec& m, float d, float sh, int nt) {
     const float xm = d * m.s;
     const float r = __builtin_amdgcn_rcpf(fmaf(xm, xm, m.c1));
     const float u = sh * r;
@@ -904,12 +905,28 @@ void k_far_series_mixed(const SumArgs a) {
     C[1] += qb;
 #pragma unroll
     for (int n = 2; n < kN; ++n) {
+      if ((n == 8 || n == 10 || n == 12 || n == 14) && n >= nt) break;
       const float qn = fmaf(al, qb, be * qa);
       C[n] += qn;
       qa = qb;
       qb = qn;
     }
   };
+  // the term count of a round (wavefront-uniform): the fewest terms whose
+  // truncation stays <= 2.6e-7 for every record the round sums, each at
+  // R = |nu_m - c| / Hu unit half-spans (>= kSeriesRatio) from the middle
+  // point: 18 terms from R = 3, 14 from 4, 12 from 5, 10 from 7, 8 from 10
+  // (the one-node and the middle-tier forms alike; tests/test_far_series.py)
+  auto series_terms32 = [&](bool use, float R) -> int {
+    if (__ballot(use && R < 4.0f)) return kN;
+    if (__ballot(use && R < 5.0f)) return 14;
+    if (__ballot(use && R < 7.0f)) return 12;
+    if (__ballot(use && R < 10.0f)) return 10;
+    return 8;
+  };
+  static_assert(kN == 18, "series_terms32's table ends at 18 terms");
+  // 1 / Hu of the three levels (slice 4 Hw, wavefront Hw, group Hw / 8)
+  const float inv_hw = 1.0f / Hw;
   // adds record m's middle-tier form (faddeeva.h: the second-order shifted
   // form for y^2 < kF32S2Y2, else the two-node form) about a middle point at
   // offset d (unit sh) to C.  Both forms are u = x^2 + A, f = (ay u + Cn) /
@@ -952,6 +969,16 @@ void k_far_series_mixed(const SumArgs a) {
   if (threadIdx.x == 0) s_sxn = 0;
   __syncthreads();
 
+  // kCount: terms summed by this lane's one-node series (the FLOP they take)
+  uint64_t n_terms = 0;
+  auto count_terms = [&]() {
+    if constexpr (kCount) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) n_terms += __shfl_xor(n_terms, off);
+      if (lane == 0) count_add(a.work, kWorkSeriesTerms, n_terms);
+      n_terms = 0;
+    }
+  };
   // ---- slice level: records outside [slo, shi), one per thread
   {
     float C[kN];
@@ -975,7 +1002,9 @@ void k_far_series_mixed(const SumArgs a) {
       const float d = (float)(nu_s - m.c);
       const bool inw = (v < n_sl) & (m.c >= lo_b) & (m.c <= hi_b);
       const bool ser = inw & (fabsf(d) >= m.ds1 + dS);
-      if (ser) series(C, m, d, 4.0f * (m.s * Hw));
+      const int nt = series_terms32(ser, fabsf(d) * (0.25f * inv_hw));
+      if (ser) series(C, m, d, 4.0f * (m.s * Hw), nt);
+      if constexpr (kCount) n_terms += ser ? (uint64_t)nt : 0u;
       if (inw & !ser) {   // rare: on to the wavefront level
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
@@ -1017,6 +1046,7 @@ void k_far_series_mixed(const SumArgs a) {
     }
   }
   __syncthreads();
+  count_terms();   // (the slice level's, before a wavefront past the tile end leaves)
   if (!wave_live) return;   // the last barrier is behind every wavefront
 
   float lay[P], layc[P];
@@ -1124,7 +1154,9 @@ void k_far_series_mixed(const SumArgs a) {
     // the rescan skips what the slice level summed
     if (v >= nA + nB + nC) inw &= !(fabsf((float)(nu_s - m.c)) >= m.ds1 + dS);
     const bool ser = inw & (fabsf(d) >= m.ds1);
-    if (ser) series(C, m, d, m.s * Hw);
+    const int nt = series_terms32(ser, fabsf(d) * inv_hw);
+    if (ser) series(C, m, d, m.s * Hw, nt);
+    if constexpr (kCount) n_terms += ser ? (uint64_t)nt : 0u;
     const bool exc = inw & !ser;
     const uint64_t ex = __ballot(exc);
     if constexpr (kCount) {
@@ -1219,7 +1251,9 @@ void k_far_series_mixed(const SumArgs a) {
       const float d0 = group_d0(gt, m.c);
       const bool valid = group_one_node_holds(d0, group_thr(gt, m.ds1), goff);
       const bool ser = act && valid;
-      if (ser) series(G, m, d, 0.125f * (m.s * Hw));
+      const int nt = series_terms32(ser, fabsf(d) * (8.0f * inv_hw));
+      if (ser) series(G, m, d, 0.125f * (m.s * Hw), nt);
+      if constexpr (kCount) n_terms += ser ? (uint64_t)nt : 0u;
       const bool mid = act && !valid && group_one_node_holds(d0, group_mid_thr(gt, m.dm), goff);
       if (mid) series_mid(G, m, d, 0.125f * (m.s * Hw));
       const bool left = act && !valid && !mid;
@@ -1291,6 +1325,9 @@ void k_far_series_mixed(const SumArgs a) {
     if (lane == 0) {
       count_add(a.work, kWorkGroupSeries, n_gser);
       count_add(a.work, kWorkGroupMiddle, n_gmid);
+    }
+    count_terms();
+    if (lane == 0) {
       count_add(a.work, kWorkGroupDeferred, n_gdef);
       count_add(a.work, kWorkGroupDeferredEvals, n_gdef_pts);
     }

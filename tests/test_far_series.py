@@ -344,3 +344,61 @@ def test_middle_series_fails_nearer_than_its_threshold():
     f = middle_direct(xm + sh * TAU, y, 1.0)
     p = poly(series_middle_kernel(xm, sh, y * y - 1.5, 1.0, False), TAU)
     assert np.max(np.abs(p / f - 1.0)) > 1e-3
+
+
+# ---- term counts by distance (k_far_series_mixed series_terms32): a round of
+# records sums only the terms its nearest record needs, R = |nu_m - c| / Hu
+# unit half-spans from the middle point; each (R, terms) row must keep the
+# truncation <= 2.6e-7 for both forms the mixed group and far levels use.
+TERMS_BY_RATIO32 = [(3.0, 18), (4.0, 14), (5.0, 12), (7.0, 10), (10.0, 8)]
+
+
+@pytest.mark.parametrize("ratio,terms", TERMS_BY_RATIO32)
+def test_one_node_series_terms_by_distance(ratio, terms):
+    worst = 0.0
+    for y in [1e-4, 0.05, 0.5, 1.0, 1.2247, 1.3, 3.0, 10.0, 40.0, 150.0, 1e3]:
+        y2 = y * y
+        c1 = y2 - 1.5
+        x_s1 = math.sqrt(max(s1_r2_min(y2) - y2, 0.0))
+        for sh in np.geomspace(0.01, 3000.0, 31):
+            edge = max(ratio * sh, sh + x_s1)
+            for xm0 in (edge, 1.3 * edge, 3.0 * edge):
+                for xm in (xm0, -xm0):
+                    q = series_one_node(xm, sh, c1, 1.0, terms=terms)
+                    x = xm + sh * TAU
+                    f = 1.0 / (x * x + c1)
+                    worst = max(worst, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+    assert worst <= 2.6e-7, worst
+
+
+@pytest.mark.parametrize("ratio,terms", TERMS_BY_RATIO32)
+def test_middle_series_terms_by_distance(ratio, terms, monkeypatch):
+    monkeypatch.setitem(globals(), "TERMS32", terms)
+    worst = 0.0
+    for y in [1e-4, 0.05, 0.5, 1.0, 1.3, 3.0, 10.0, 12.4, 12.5, 20.0, 40.0, 150.0]:
+        y2 = y * y
+        x_m = middle_threshold(y2)
+        for sh in np.geomspace(0.01, 3000.0, 31):
+            edge = max(ratio * sh, sh + x_m)
+            for xm0 in (edge, 1.3 * edge, 3.0 * edge):
+                for xm in (xm0, -xm0):
+                    q = series_middle_kernel(xm, sh, y2 - 1.5, 1.0, False)
+                    f = middle_direct(xm + sh * TAU, y, 1.0)
+                    worst = max(worst, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+    assert worst <= 2.6e-7, worst
+
+
+def test_fewer_terms_than_the_table_miss_the_bound():
+    # the table is not loose by a full step: two terms fewer at each row's
+    # nearest distance misses 2.6e-7 for some record
+    for ratio, terms in TERMS_BY_RATIO32:
+        worst = 0.0
+        for y in (1e-4, 1.3, 40.0):
+            y2 = y * y
+            x_s1 = math.sqrt(max(s1_r2_min(y2) - y2, 0.0))
+            for sh in np.geomspace(0.01, 3000.0, 31):
+                xm = max(ratio * sh, sh + x_s1)
+                q = series_one_node(xm, sh, y2 - 1.5, 1.0, terms=terms - 2)
+                x = xm + sh * TAU
+                worst = max(worst, float(np.max(np.abs(poly(q, TAU) * (x * x + y2 - 1.5) - 1.0))))
+        assert worst > 2.6e-7, (ratio, terms, worst)
