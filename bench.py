@@ -142,6 +142,14 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
     far_flop = (kw["slice_series"] * fp["slice"]
                 + (kw["wave_series"] + kw.get("group_series", 0) - g_mid) * fp["wave"]
                 + g_mid * fp.get("group_middle", 0) + kw["exception_evals"] * FLOP_PER_EVAL)
+    if base == "fp64" and kw.get("series_terms"):
+        # the FP64 kernel sums only the terms each round's nearest record
+        # needs: a slice pair costs 54 FLOP plus 8 per term after the fourth
+        # (246 at 28), a wavefront pair 10 + 2 x (49 + 8 per term after the
+        # fourth) (492 at 28)
+        far_flop = (22 * kw["slice_series"] + 8 * kw["series_terms"]
+                    + 44 * kw["wave_series"] + 16 * kw.get("series_terms_wave", 0)
+                    + kw["exception_evals"] * FLOP_PER_EVAL)
     if base == "mixed" and kw.get("series_terms"):
         # the mixed kernel sums only the terms each round's nearest record
         # needs: a one-node pair costs its 12-FLOP set-up plus 4 per term
@@ -158,6 +166,7 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
             "exception_point_evals": kw["exception_evals"],
             "flop_per_unit": far_flop / max(pairs, 1),
             "series_terms_summed": kw.get("series_terms"),
+            "series_terms_summed_wave_level": kw.get("series_terms_wave"),
             "flop_per_pair": dict(fp)},
         near_name: {"ms": near_ms, "units": kw["near_evals"], "unit": "point evaluations",
                     "flop_per_unit": FLOP_PER_EVAL, "queued_near_centre_pairs": kw["queued"],

@@ -298,8 +298,9 @@ int lbl_last_timing_epilogue(double *ms_epilogue);
  *   near field (neither the one-node nor the middle-tier form holds at every
  *   point of the group), [12] the near field's point evaluations of [11]
  *   (part of [0]), [13] the pairs of [10] summed in the middle-tier form
- *   [14] LBL_MIXED: series terms summed over the one-node pairs of [1],
- *   [2] and [10] (each pair takes the terms its distance needs)
+ *   [14] series terms summed (each pair takes the terms its distance
+ *   needs): LBL_MIXED over the one-node pairs of [1], [2] and [10], LBL_FP64
+ *   over the pairs of [1]   [15] LBL_FP64: terms of each pair of [2]
  * All counts are 0 after a call made without counting. */
 void lbl_enable_work_count(int on);
 int lbl_last_work_count(uint64_t *counts, int32_t n);

@@ -317,7 +317,9 @@ enum WorkCount {
   kWorkGroupDeferred,       // ... left to the near field (the form fails at the group)
   kWorkGroupDeferredEvals,  // the near field's point evaluations of those
   kWorkGroupMiddle,         // group series pairs in the middle-tier form (part of kWorkGroupSeries)
-  kWorkSeriesTerms,         // mixed: terms summed over the one-node pairs of all levels
+  kWorkSeriesTerms,         // terms summed: mixed, over the one-node pairs of all levels;
+                            // FP64, over the slice level's pairs
+  kWorkSeriesTerms2,        // FP64: terms of each wavefront-level pair (both pole pairs)
   kWorkSlots
 };
 

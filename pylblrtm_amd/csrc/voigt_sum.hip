@@ -505,7 +505,7 @@ __device__ __forceinline__ bool surrogate_point(float x, float y, float ac, floa
 // level, and one that fails there is evaluated point by point with the full
 // FP64 dispatch.
 template <int P, bool kCount>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void k_far_series_fp64(const SumArgs a) {
   static_assert(P == kPointsFp64 && 64 * P == kWavePoints && kBlock * P == kSlicePoints,
                 "slice and wavefront geometry");
@@ -556,7 +556,7 @@ void k_far_series_fp64(const SumArgs a) {
   // adds the Taylor coefficients of (n0 + n1 tau + n2 tau^2) / (d0 + ... + d4 tau^4)
   // to C: D Q = N term by term
   auto rational = [&](double (&C)[kN], double n0, double n1, double n2, double d0, double d1,
-                      double d2, double d3, double d4) {
+                      double d2, double d3, double d4, int nt) {
     const double inv = rcp64_1nr(d0);
     const double e1 = -d1 * inv, e2 = -d2 * inv, e3 = -d3 * inv, e4 = -d4 * inv;
     double q0 = n0 * inv;
@@ -569,6 +569,7 @@ void k_far_series_fp64(const SumArgs a) {
     C[3] += q3;
 #pragma unroll
     for (int n = 4; n < kN; ++n) {
+      if ((n == 12 || n == 14 || n == 16 || n == 18 || n == 20 || n == 24) && n >= nt) break;
       const double qn = fma(e1, q3, fma(e2, q2, fma(e3, q1, e4 * q0)));
       C[n] += qn;
       q0 = q1; q1 = q2; q2 = q3; q3 = qn;
@@ -576,7 +577,7 @@ void k_far_series_fp64(const SumArgs a) {
   };
   // second-order shifted form about a middle point d = nu_m - c away:
   // ay r / (r^2 - 3/2 r + 2 y^2 - 3/2), r = X + y^2, X = x^2 = X0 + a1 tau + a2 tau^2
-  auto series = [&](double (&C)[kN], double d, double sv, double y, double am, double Hu) {
+  auto series = [&](double (&C)[kN], double d, double sv, double y, double am, double Hu, int nt) {
     const double y2 = y * y;
     const double xm = d * sv, sh = sv * Hu;
     const double a1 = 2.0 * xm * sh, a2 = sh * sh;
@@ -584,11 +585,12 @@ void k_far_series_fp64(const SumArgs a) {
     const double t = fma(2.0, r0, -1.5);
     const double g = am * y * kInvSqrtPi;
     rational(C, g * r0, g * a1, g * a2, fma(r0, r0 - 1.5, fma(2.0, y2, -1.5)), t * a1,
-             fma(a1, a1, t * a2), 2.0 * a1 * a2, a2 * a2);
+             fma(a1, a1, t * a2), 2.0 * a1 * a2, a2 * a2, nt);
   };
   // four-node form (|z| >= 35, faddeeva.h) as two pairs of poles, each
   // w_k (X + b) / (X^2 + (2 y^2 - 2 t_k^2) X + b^2), b = y^2 + t_k^2
-  auto series_gh4 = [&](double (&C)[kN], double d, double sv, double y, double am, double Hu) {
+  auto series_gh4 = [&](double (&C)[kN], double d, double sv, double y, double am, double Hu,
+                        int nt) {
     const double y2 = y * y;
     const double xm = d * sv, sh = sv * Hu;
     const double a1 = 2.0 * xm * sh, a2 = sh * sh;
@@ -600,9 +602,26 @@ void k_far_series_fp64(const SumArgs a) {
       const double b = y2 + t2, c = 2.0 * (y2 - t2);
       const double t = fma(2.0, X0, c);
       rational(C, w * (X0 + b), w * a1, w * a2, fma(X0, X0 + c, b * b), t * a1,
-               fma(a1, a1, t * a2), 2.0 * a1 * a2, a2 * a2);
+               fma(a1, a1, t * a2), 2.0 * a1 * a2, a2 * a2, nt);
     }
   };
+  // the term count of a round (wavefront-uniform): the fewest terms whose
+  // truncation stays <= 1e-11 for every record the round sums, each at R =
+  // |nu_m - c| / Hu unit half-spans (>= kSeriesRatio) from the middle point:
+  // 28 terms from R = 3, 24 from 3.5, 20 from 4.5, 18 from 5, 16 from 6, 14
+  // from 8, 12 from 12 (the second-order form and the four-node pole pairs
+  // alike; tests/test_far_series.py)
+  auto series_terms64 = [&](bool use, double R) -> int {
+    if (__ballot(use && R < 3.5)) return kN;
+    if (__ballot(use && R < 4.5)) return 24;
+    if (__ballot(use && R < 5.0)) return 20;
+    if (__ballot(use && R < 6.0)) return 18;
+    if (__ballot(use && R < 8.0)) return 16;
+    if (__ballot(use && R < 12.0)) return 14;
+    return 12;
+  };
+  static_assert(kN == 28, "series_terms64's table ends at 28 terms");
+  uint64_t n_terms = 0, n_terms2 = 0;   // kCount: terms summed (slice, wavefront level)
   // the four-node form holds at every point within Hu of the middle point
   auto holds_gh4 = [&](double d, double sv, double y2, double Hu) {
     const double xn = sv * (fabs(d) - Hu);
@@ -639,7 +658,9 @@ void k_far_series_fp64(const SumArgs a) {
       const double d = nu_s - c;
       const bool inw = (v < n_sl) & (c >= lo_b) & (c <= hi_b);
       const bool ser = inw && holds(d, sv, y * y, Hs);
-      if (ser) series(C, d, sv, y, am, Hs);
+      const int nt = series_terms64(ser, fabs(d) / Hs);
+      if (ser) series(C, d, sv, y, am, Hs, nt);
+      if constexpr (kCount) n_terms += ser ? (uint64_t)nt : 0u;
       if (inw & !ser) {   // on to the wavefront level
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kSliceExCap) s_sx[pos] = (int32_t)sl_index(v);
@@ -650,9 +671,12 @@ void k_far_series_fp64(const SumArgs a) {
       }
     }
     if constexpr (kCount) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) n_terms += __shfl_xor(n_terms, off);
       if (lane == 0) {
         count_add(a.work, kWorkSliceSeries, n_sser);
         count_add(a.work, kWorkSliceExceptions, n_sexc);
+        count_add(a.work, kWorkSeriesTerms, n_terms);
       }
     }
 #pragma unroll
@@ -764,7 +788,9 @@ void k_far_series_fp64(const SumArgs a) {
     // the four-node form here: it holds much nearer than the second-order
     // form at FP64 accuracy, so broad lines need no point-by-point pass
     const bool ser = inw && holds_gh4(d, sv, y2, H);
-    if (ser) series_gh4(C, d, sv, y, am, H);
+    const int nt = series_terms64(ser, fabs(d) / H);
+    if (ser) series_gh4(C, d, sv, y, am, H, nt);
+    if constexpr (kCount) n_terms2 += ser ? (uint64_t)nt : 0u;
     const bool exc = inw & !ser;
     const uint64_t ex = __ballot(exc);
     if constexpr (kCount) {
@@ -800,9 +826,12 @@ void k_far_series_fp64(const SumArgs a) {
     if (i < t1) out[i] = f + exv[j];
   }
   if constexpr (kCount) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n_terms2 += __shfl_xor(n_terms2, off);
     if (lane == 0) {
       const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
       count_add(a.work, kWorkWaveSeries, n_wser);
+      count_add(a.work, kWorkSeriesTerms2, n_terms2);
       count_add(a.work, kWorkWaveExceptions, n_wexc);
       count_add(a.work, kWorkExceptionEvals, n_wexc * w_pts);
       count_add(a.work, kWorkWaveFromSliceList, n_wlist);

@@ -402,3 +402,35 @@ def test_fewer_terms_than_the_table_miss_the_bound():
                 x = xm + sh * TAU
                 worst = max(worst, float(np.max(np.abs(poly(q, TAU) * (x * x + y2 - 1.5) - 1.0))))
         assert worst > 2.6e-7, (ratio, terms, worst)
+
+
+# ---- FP64 term counts by distance (k_far_series_fp64 series_terms64): each
+# (R, terms) row must keep the truncation <= 1e-11 for the second-order form
+# (slice level) and the four-node pole pairs (wavefront level).
+TERMS_BY_RATIO64 = [(3.0, 28), (3.5, 24), (4.5, 20), (5.0, 18), (6.0, 16), (8.0, 14), (12.0, 12)]
+
+
+@pytest.mark.parametrize("ratio,terms", TERMS_BY_RATIO64)
+def test_fp64_series_terms_by_distance(ratio, terms):
+    w_so = w_gh4 = 0.0
+    for y in [1e-4, 0.1, 1.0, 3.0, 10.0, 34.0, 35.0, 50.0, 200.0, 1e3, 1e4]:
+        y2 = y * y
+        x_s2 = math.sqrt(max((S2_COEF64 * (9.0 * y2 + 5.25)) ** (1.0 / 3.0) - y2, 0.0))
+        x_4 = math.sqrt(max(GH4_R2 - y2, 0.0))
+        for sh in np.geomspace(0.01, 3000.0, 25):
+            for x_form, which in ((x_s2, "so"), (x_4, "gh4")):
+                edge = max(ratio * sh, sh + x_form)
+                for xm0 in (edge, 1.3 * edge, 3.0 * edge):
+                    for xm in (xm0, -xm0):
+                        x = xm + sh * TAU
+                        if which == "so":
+                            q = series_rational(*second_order_coeffs(xm, sh, y, 1.0), terms)
+                            r = x * x + y2
+                            f = r / (r * r - 1.5 * r + 2.0 * y2 - 1.5)
+                            w_so = max(w_so, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+                        else:
+                            q = sum(series_rational(*gh4_pair_coeffs(xm, sh, y, p), terms) for p in (0, 1))
+                            f = gh4_direct(x, y)
+                            w_gh4 = max(w_gh4, float(np.max(np.abs(poly(q, TAU) / f - 1.0))))
+    assert w_so <= 1e-11, w_so
+    assert w_gh4 <= 1e-11, w_gh4
