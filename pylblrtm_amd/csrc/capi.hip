@@ -152,7 +152,7 @@ bool make_layout(const lbl_lines_t* lines, const lbl_layers_t* layers, const lbl
   L->work = off; off = align_up(off + lbl::kWorkSlots * sizeof(unsigned long long));
   // mixed modes: the group level's left-over pair count per (layer, wavefront)
   L->deferred = off;
-  if (!fp64 && !with_grad)
+  if (!with_grad)
     off = align_up(off + (size_t)std::max<int32_t>(layers->n_layers, 0) * n_waves * 4);
   // partial tau per layer group (float sum + compensation, or double): 8 B
   L->tau_part = off;

@@ -74,18 +74,29 @@ __device__ __forceinline__ void flag_od(uint32_t* err, int k) {
 
 // ------------------------------------------------------------------ FP64 --
 
+// The four-node form (|z| >= 35, faddeeva.h) holds at every point within Hu
+// of a middle point d = nu_m - c away.  Shared by the series kernel's levels
+// and the near field's test for the pairs the group level leaves to it, so
+// both reach the same verdict bit for bit.
+__device__ __forceinline__ bool gh4_holds_within(double d, double sv, double y2, double Hu) {
+#pragma clang fp contract(off)
+  const double xn = sv * (fabs(d) - Hu);
+  return fma(xn, xn, y2) >= kFp64Gh4R2;
+}
+
 // The full FP64 dispatch, called (not inlined) from the rare near-centre
 // paths so that its registers do not crowd the far-field loop.
 __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x, y); }
 __device__ __noinline__ float K_mixed_call(float x, float y) { return K_mixed(x, y); }
 
 // Record of the FP64 line sum as staged in LDS (one per (layer, line)); the
-// queue drain reads y and A from the global records.  hw1, hw2, hw4: |x| at
-// the middle of a thread's run above which all P points are in the shifted
-// one-node, second-order shifted and four-node regions.
+// queue drain reads y and A from the global records.  hw2, hw4: |x| at the
+// middle of a thread's run above which all P points are in the second-order
+// shifted and four-node regions.
 struct alignas(16) Rec64 {
   double c, s;        // shifted centre, sqrt(ln2)/alpha_D
-  double hw1, hs;     // sqrt(max(s1_r2_min64(y^2) - y^2, 0)) + (P-1)/2 hs; step*s
+  int64_t rec;        // the line's index (the drain reads y and A from the layer's arrays)
+  double hs;          // step*s
   double c1, ay;      // y^2 - 3/2; A y/sqrt(pi)
   double hw2, hw4;    // sqrt(max(s2_r2_min64(y^2) - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
 };
@@ -107,7 +118,7 @@ constexpr int kCoreCap64 = 512;   // >= the 64 x 8 points one pass-A round can a
 // read-modify-writes; the points it cannot hold (|z| < 7, the core) go to a
 // list that pass B evaluates one point per lane with trapezoid + residue.
 template <int P, bool kCount>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void k_voigt_sum_fp64(const SumArgs a) {
   // every wavefront works alone on its own near range (k_far_series_fp64 has
   // summed the rest of the tile): its own LDS stage of kStage records, its
@@ -119,6 +130,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
   __shared__ uint16_t s_ci[kWaves][kCoreCap64];    // (record << 9) | (lane << 3) | point
   __shared__ double s_acc[kBlock * P];
   __shared__ uint8_t s_claim[kWaves][64];          // claim byte per owner lane
+  __shared__ uint8_t s_gmask[kWaves][kStage];      // phase 2: the groups that take a staged record
 
   // (layer group, tile slice), group-major so that an XCD holds neighbouring
   // tiles of one layer range
@@ -155,273 +167,322 @@ void k_voigt_sum_fp64(const SumArgs a) {
   // column totals live in tau_part[grp], read and rewritten once per layer,
   // so that no registers hold them through the record loops and the drains
   double* const tau_acc = static_cast<double*>(a.tau_part) + (int64_t)grp * a.n_nu;
-  uint64_t n_in = 0, n_queued = 0, n_core = 0;   // kCount: records in the window, queued pairs, core points
+  uint64_t n_evals = 0, n_queued = 0, n_core = 0;   // kCount: point evaluations, queued pairs, core points
+  // the wavefront's groups of kGroupPoints points (8 lanes each) and their
+  // own near ranges (records within kSeriesRatio Hg of the group's middle)
+  const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
+  const double Hg = 0.5 * kGroupPoints * a.step;
+  const int32_t* const gl = a.glo + (wg * kWaves + wave) * kGroupsPerWave;
+  const int32_t* const gh = a.ghi + (wg * kWaves + wave) * kGroupsPerWave;
+  const uint32_t my_group = (uint32_t)lane >> 3;
+  const int64_t g_first = w_first + (int64_t)my_group * kGroupPoints;
+  const bool live_g = g_first < t1;
+  const uint64_t g_pts = (uint64_t)max((int64_t)0, min((int64_t)kGroupPoints, t1 - g_first));
+  const int32_t my_lo = min(max(gl[my_group], nlo), nhi);
+  const int32_t my_hi = max(min(gh[my_group], nhi), my_lo);
+  int32_t n_own = 0;   // the longest of the eight (scalar loads)
+#pragma unroll
+  for (int g = 0; g < kGroupsPerWave; ++g) {
+    const int32_t g_lo = min(max(gl[g], nlo), nhi), g_hi = max(min(gh[g], nhi), g_lo);
+    n_own = max(n_own, g_hi - g_lo);
+  }
+  const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)(lds_rec64*)s_rec[wave];
+  auto f64 = [](unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); };
+  auto f64i = [](unsigned lo, unsigned hi) { return (int64_t)(((uint64_t)hi << 32) | lo); };
 
   for (int k = k_begin; k < k_end; ++k) {
     double lay[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) lay[j] = 0.0;
     const int64_t kb = (int64_t)k * a.n_lines;
-    for (int32_t base = nlo; base < nhi; base += kStage) {
-      const int cnt = __builtin_amdgcn_readfirstlane(min(kStage, nhi - base));
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    int nq = 0;   // queue entries not yet drained (uniform)
+    // the staged record of line r of this layer: the tier thresholds, and the
+    // line's index (for the drain's y and A); a record outside the tile
+    // window (or skipped) adds 0 and is never queued
+    auto stage = [&](const int slot, const int32_t r, const bool valid) {
+      double c = 0.0, sv = 0.0, y = 1.0e3, am = 0.0;
       bool inside = false;
-      if (lane < cnt) {
-        const int64_t r = kb + base + lane;
-        double c = a.rec.c[r], sv = rs[r], y = ry[r], am = ra[r];
-        inside = c >= lo_b && c <= hi_b;
-        if (!inside) {  // outside the tile window (or skipped): adds 0
-          c = 0.0; sv = 0.0; y = 1.0e3; am = 0.0;
-        }
-        Rec64 R;
-        const double y2 = y * y, hs = a.step * sv;
-        R.c = c; R.s = sv; R.hs = hs;
-        R.hw1 = fma(half, hs, sqrt(fmax(s1_r2_min64(y2) - y2, 0.0)));
-        R.hw2 = fma(half, hs, sqrt(fmax(s2_r2_min64(y2) - y2, 0.0)));
-        R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
-        R.c1 = y2 - 1.5; R.ay = am * y * kInvSqrtPi;
-        s_rec[wave][lane] = R;
+      if (valid) {
+        const double rc_ = a.rec.c[kb + r];
+        inside = rc_ >= lo_b && rc_ <= hi_b;
+        if (inside) { c = rc_; sv = rs[kb + r]; y = ry[kb + r]; am = ra[kb + r]; }
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      if constexpr (kCount) n_in += (uint64_t)__popcll(__ballot(inside));
-      const int ncnt = cnt;
-      int nq = 0;   // queue entries not yet drained (uniform)
-      // Records are read from LDS as 16-byte pieces: (c, s) and (hw1, hs) two
-      // records ahead into their own registers (two records per iteration),
-      // (c1, ay) at the record and (hw2, hw4) only off the one-node branch.
-      const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)(lds_rec64*)s_rec[wave];
-      auto f64 = [](unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); };
-      auto record = [&](const int q, const u32x4 h0, const u32x4 h1,
-                        const volatile lds_u32x4* const more) {
-        const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
-        const double hw1 = f64(h1.x, h1.y), hs = f64(h1.z, h1.w);
-        const double xm = (nu_m - rc) * rsv;
-        const double ax = fabs(xm);
-        const u32x4 h2 = more[0];
-        const double c1 = f64(h2.x, h2.y), ay = f64(h2.z, h2.w);
-        // One tier per record for the whole wavefront, the lowest any lane
-        // needs (each form holds wherever the ones above it do): no lane pays
-        // for two tiers.  Lanes that need the queue run the four-node form at
-        // a safe x with weight 0 and queue the pair.
-        if (__ballot(ax < hw1) == 0) {
-          // shifted one-node form, A K = ay / D, D = x^2 + y^2 - 3/2; points step
-          // from the first one by integer multiples of hs (inline constants of
-          // v_fma_f64); the eight quotients share one v_rcp_f64 (Montgomery's
-          // batch inversion: D >= 3.9e5 and a product of eight stays far
-          // inside the FP64 range)
-          static_assert(P == 8, "one batch of eight points");
-          const double x0 = fma(-half, hs, xm);
-          double D[P];
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const double x = fma((double)j, hs, x0);
-            D[j] = fma(x, x, c1);
-          }
-          const double p01 = D[0] * D[1], p23 = D[2] * D[3], p45 = D[4] * D[5], p67 = D[6] * D[7];
-          const double p03 = p01 * p23, p47 = p45 * p67;
-          const double r = ay * rcp64_1nr(p03 * p47);     // ay / (D0 ... D7)
-          const double r03 = r * p47, r47 = r * p03;
-          const double r01 = r03 * p23, r23 = r03 * p01, r45 = r47 * p67, r67 = r47 * p45;
-          lay[0] = fma(r01, D[1], lay[0]);
-          lay[1] = fma(r01, D[0], lay[1]);
-          lay[2] = fma(r23, D[3], lay[2]);
-          lay[3] = fma(r23, D[2], lay[3]);
-          lay[4] = fma(r45, D[5], lay[4]);
-          lay[5] = fma(r45, D[4], lay[5]);
-          lay[6] = fma(r67, D[7], lay[6]);
-          lay[7] = fma(r67, D[6], lay[7]);
-          return;
-        }
-        const u32x4 h3 = more[1];
-        const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
-        if (__ballot(ax < hw2) == 0) {
-          // second-order shifted form (faddeeva.h), A K = ay r / D with
-          // r = x^2 + y^2, D = r^2 - 3/2 r + 2 y^2 - 3/2 >= 6.9e7; the quotients
-          // by batch inversion in two batches of four (a product of four D
-          // ~ |z|^16 stays inside the FP64 range for |z| < 1e19, also in lanes
-          // far past the one-node threshold)
-          const double y2 = c1 + 1.5, cc = fma(2.0, c1, 1.5);
-          const double x0 = fma(-half, hs, xm);
-          double rr[P], D[P];
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const double x = fma((double)j, hs, x0);
-            rr[j] = fma(x, x, y2);
-            D[j] = fma(rr[j], rr[j] - 1.5, cc);
-          }
-#pragma unroll
-          for (int h = 0; h < P; h += 4) {
-            const double pa = D[h] * D[h + 1], pb = D[h + 2] * D[h + 3];
-            const double r = ay * rcp64_1nr(pa * pb);
-            const double ra_ = r * pb, rb_ = r * pa;
-            lay[h] = fma(ra_ * D[h + 1], rr[h], lay[h]);
-            lay[h + 1] = fma(ra_ * D[h], rr[h + 1], lay[h + 1]);
-            lay[h + 2] = fma(rb_ * D[h + 3], rr[h + 2], lay[h + 2]);
-            lay[h + 3] = fma(rb_ * D[h + 2], rr[h + 3], lay[h + 3]);
-          }
-          return;
-        }
-        // four-node form (|z| >= 35 at every point of the run; at the safe
-        // middle x = hw4 for lanes whose run reaches nearer, with weight 0);
-        // A (2y/pi) W = ay (2/sqrt(pi)) W; the two points of a pair share one
-        // v_rcp_f64 (a product of two D0 D1 ~ |z|^16)
-        const bool near = ax < hw4;
-        {
-          const double y2 = c1 + 1.5, fy2 = 4.0 * y2;
-          const double ayw = near ? 0.0 : ay;
-          const double w0 = ayw * (2.0 * kInvSqrtPi * kGh4W[0]);
-          const double w1 = ayw * (2.0 * kInvSqrtPi * kGh4W[1]);
-          const double x0 = fma(-half, hs, near ? hw4 : xm);
-#pragma unroll
-          for (int j = 0; j < P; j += 2) {
-            double Np[2], Dp[2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-              const double x = fma((double)(j + u), hs, x0);
-              const double x2 = x * x;
-              const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
-              const double u0 = xmy - t0sq, u1 = xmy - t1sq;
-              const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
-              Np[u] = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
-              Dp[u] = D0 * D1;
-            }
-            const double rp = rcp64_1nr(Dp[0] * Dp[1]);
-            lay[j] = fma(Np[0] * Dp[1], rp, lay[j]);
-            lay[j + 1] = fma(Np[1] * Dp[0], rp, lay[j + 1]);
-          }
-        }
-        // append the queued lanes' (record, lane) entries; nq advances in
-        // every lane (the loop drains before the queue could overflow)
-        const bool queued = near && live;
-        const uint64_t msk = __ballot(queued);
-        if (msk) {   // uniform
-          const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
-          if (queued) s_q[wave][nq + (int)rank] = (uint16_t)((q << 6) | lane);
-          nq += (int)__popcll(msk);
-        }
-      };
-      // The queue, drained by this wavefront in two dense passes.  Pass A: one
-      // entry per lane, its P points at their exact grid values in the
-      // 10-node form (<= 1e-11 for |z| >= 7, y >= 1e-8); core points get
-      // weight 0 and go to the core list.  Pass B: the core list, one point
-      // per lane, trapezoid + residue.
-      auto drain = [&](const int n) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
-        __builtin_amdgcn_wave_barrier();
-        int ncore = 0;   // uniform
-        auto core_pass = [&](const int m) {
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
-          __builtin_amdgcn_wave_barrier();
-          for (int b = lane; b < m; b += 64) {
-            const uint32_t ci = s_ci[wave][b];
-            const int qq = (int)(ci >> 9), t = (int)((ci >> 3) & 63), j = (int)(ci & (P - 1));
-            const u32x4 h0 = rb[4 * qq];   // c, s
-            const int64_t r = kb + base + qq;
-            const double y = ry[r], am = ra[r];
-            const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)(wave * 64 + t) * P + j) -
-                              f64(h0.x, h0.y)) * f64(h0.z, h0.w);
-            atomicAdd(&s_acc[j * kBlock + wave * 64 + t], am * K_cr64(x, y));
-          }
-          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
-          __builtin_amdgcn_wave_barrier();
-        };
-        for (int e0 = 0; e0 < n; e0 += 64) {
-          const int e = e0 + lane;
-          const bool has = e < n;
-          const uint32_t qi = has ? s_q[wave][e] : 0u;
-          const int qq = (int)(qi >> 6), t = (int)(qi & 63);
-          const u32x4 h0 = rb[4 * qq], h2 = rb[4 * qq + 2];   // (c, s), (c1, ay)
-          const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
-          // A (2y/pi) = (2/sqrt(pi)) ay; idle lanes add 0
-          const double cw = has ? 2.0 * kInvSqrtPi * f64(h2.z, h2.w) : 0.0;
-          const double y = has ? ry[kb + base + qq] : 1.0;
-          const double y2 = y * y;
-          const double rcore = y >= kFp64Gh10Ymin ? kFp64Gh10R2 : kFp64Gh8R2;
-          const int64_t ib = s0 + (int64_t)(wave * 64 + t) * P;
-          uint32_t cmask = 0;
-          double v[P];
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const double x = (grid_nu(a.nu_min, a.step, ib + j) - rc) * rsv;
-            const double x2 = x * x;
-            const double r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
-            const bool core = has && r < rcore;
-            // (a select: at a core point the node sum may be infinite)
-            const double g = gh_sum64<5>(r, xmy, fxy, kGh10T, kGh10W);
-            v[j] = core ? 0.0 : cw * g;
-            cmask |= core ? (1u << j) : 0u;
-          }
-          // Deliver the P values to the owner's sums with plain LDS reads and
-          // writes: in each round every pending lane writes its id to its
-          // owner's claim byte, the lanes that read their own id back hold
-          // distinct owners and update, the rest try again.
-          bool pending = has;
-          while (__ballot(pending)) {   // uniform
-            if (pending) s_claim[wave][t] = (uint8_t)lane;
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            if (pending && s_claim[wave][t] == (uint8_t)lane) {
-              double* const dst = s_acc + wave * 64 + t;
-#pragma unroll
-              for (int j = 0; j < P; ++j) dst[j * kBlock] += v[j];
-              pending = false;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-          }
-          // this lane's core points go to the list at an exclusive prefix sum
-          // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
-          const uint32_t cnt8 = __builtin_popcount(cmask);
-          uint32_t pre = 0, total = 0;
-#pragma unroll
-          for (int bit = 0; bit < 4; ++bit) {
-            const uint64_t m = __ballot((cnt8 >> bit) & 1u);
-            pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
-            total += (uint32_t)__popcll(m) << bit;
-          }
-          if (total) {   // uniform
-            if (ncore + (int)total > kCoreCap64) {
-              core_pass(ncore);
-              ncore = 0;
-            }
-            int off = ncore + (int)pre;
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-              if (cmask & (1u << j)) {
-                s_ci[wave][off] = (uint16_t)((uint32_t)qq << 9 | (uint32_t)t << 3 | (uint32_t)j);
-                ++off;
-              }
-            }
-            ncore += (int)total;
-            if constexpr (kCount) n_core += total;
-          }
-        }
-        if (ncore > 0) core_pass(ncore);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+      Rec64 R;
+      const double y2 = y * y, hs = a.step * sv;
+      R.c = c; R.s = sv; R.hs = hs;
+      R.rec = valid ? r : 0;
+      R.hw2 = fma(half, hs, sqrt(fmax(s2_r2_min64(y2) - y2, 0.0)));
+      R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
+      R.c1 = y2 - 1.5; R.ay = am * y * kInvSqrtPi;
+      s_rec[wave][slot] = R;
+      return inside;
+    };
+    // Records are read from LDS as 16-byte pieces: (c, s) and (record, hs)
+    // one iteration ahead, (c1, ay) and (hw2, hw4) at the record.  The group
+    // level summed every (group, record) pair outside the group's own near
+    // range where the four-node form holds at all its points, so the
+    // one-node form (|z| >= 630 for narrow lines) is never reached here.
+    auto record = [&](const int q, const u32x4 h0, const u32x4 h1,
+                      const volatile lds_u32x4* const more, const bool direct) {
+      const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
+      const double hs = f64(h1.z, h1.w);
+      const double xm = (nu_m - rc) * rsv;
+      const double ax = fabs(xm);
+      const u32x4 h2 = more[0], h3 = more[1];
+      const double c1 = f64(h2.x, h2.y), ay = direct ? f64(h2.z, h2.w) : 0.0;
+      const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
+      // One tier for the whole wavefront, the lowest any lane needs (each
+      // form holds wherever the ones above it do): no lane pays for two
+      // tiers.  Lanes that need the queue run the four-node form at a safe x
+      // with weight 0 and queue the pair.
+      if (__ballot(direct && ax < hw2) == 0) {
+        // second-order shifted form (faddeeva.h), A K = ay r / D with
+        // r = x^2 + y^2, D = r^2 - 3/2 r + 2 y^2 - 3/2 >= 6.9e7; the quotients
+        // by batch inversion in two batches of four (a product of four D
+        // ~ |z|^16 stays inside the FP64 range for |z| < 1e19)
+        const double y2 = c1 + 1.5, cc = fma(2.0, c1, 1.5);
+        const double x0 = fma(-half, hs, xm);
+        double rr[P], D[P];
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          lay[j] += acc[j * kBlock];
-          acc[j * kBlock] = 0.0;
+          const double x = fma((double)j, hs, x0);
+          rr[j] = fma(x, x, y2);
+          D[j] = fma(rr[j], rr[j] - 1.5, cc);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      };
-      u32x4 ha0 = rb[0], ha1 = rb[1], hb0 = rb[4], hb1 = rb[5];
-      const volatile lds_u32x4* pr = rb;
-      asm volatile("" : "+v"(pr));
-      int q = 0;
-      for (;;) {
-        for (; q < ncnt; q += 2, pr += 8) {
-          record(q, ha0, ha1, pr + 2);
-          if (q + 2 < ncnt) { ha0 = pr[8]; ha1 = pr[9]; }
-          if (q + 1 < ncnt) {
-            record(q + 1, hb0, hb1, pr + 6);
-            if (q + 3 < ncnt) { hb0 = pr[12]; hb1 = pr[13]; }
+#pragma unroll
+        for (int h = 0; h < P; h += 4) {
+          const double pa = D[h] * D[h + 1], pb = D[h + 2] * D[h + 3];
+          const double r = ay * rcp64_1nr(pa * pb);
+          const double ra_ = r * pb, rb_ = r * pa;
+          lay[h] = fma(ra_ * D[h + 1], rr[h], lay[h]);
+          lay[h + 1] = fma(ra_ * D[h], rr[h + 1], lay[h + 1]);
+          lay[h + 2] = fma(rb_ * D[h + 3], rr[h + 2], lay[h + 2]);
+          lay[h + 3] = fma(rb_ * D[h + 2], rr[h + 3], lay[h + 3]);
+        }
+        return;
+      }
+      // four-node form (|z| >= 35 at every point of the run; at the safe
+      // middle x = hw4 for lanes whose run reaches nearer, with weight 0);
+      // A (2y/pi) W = ay (2/sqrt(pi)) W; the two points of a pair share one
+      // v_rcp_f64 (a product of two D0 D1 ~ |z|^16)
+      const bool near = direct && ax < hw4;
+      {
+        const double y2 = c1 + 1.5, fy2 = 4.0 * y2;
+        const double ayw = near ? 0.0 : ay;
+        const double w0 = ayw * (2.0 * kInvSqrtPi * kGh4W[0]);
+        const double w1 = ayw * (2.0 * kInvSqrtPi * kGh4W[1]);
+        const double x0 = fma(-half, hs, near ? hw4 : xm);
+#pragma unroll
+        for (int j = 0; j < P; j += 2) {
+          double Np[2], Dp[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const double x = fma((double)(j + u), hs, x0);
+            const double x2 = x * x;
+            const double r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+            const double u0 = xmy - t0sq, u1 = xmy - t1sq;
+            const double D0 = fma(u0, u0, fxy), D1 = fma(u1, u1, fxy);
+            Np[u] = w0 * (r + t0sq) * D1 + w1 * (r + t1sq) * D0;
+            Dp[u] = D0 * D1;
           }
-          if (nq > kQueueW64 - 128) {   // uniform
-            q += 2;
-            pr += 8;
+          const double rp = rcp64_1nr(Dp[0] * Dp[1]);
+          lay[j] = fma(Np[0] * Dp[1], rp, lay[j]);
+          lay[j + 1] = fma(Np[1] * Dp[0], rp, lay[j + 1]);
+        }
+      }
+      // append the queued lanes' (stage slot, lane) entries; nq advances in
+      // every lane (the loops drain before the queue could overflow)
+      const bool queued = near && live;
+      const uint64_t msk = __ballot(queued);
+      if (msk) {   // uniform
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(msk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)msk, 0u));
+        if (queued) s_q[wave][nq + (int)rank] = (uint16_t)((q << 6) | lane);
+        nq += (int)__popcll(msk);
+      }
+    };
+    // The queue, drained by this wavefront in two dense passes.  Pass A: one
+    // entry per lane, its P points at their exact grid values in the
+    // 10-node form (<= 1e-11 for |z| >= 7, y >= 1e-8); core points get
+    // weight 0 and go to the core list.  Pass B: the core list, one point
+    // per lane, trapezoid + residue.
+    auto drain = [&](const int n) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the queue writes
+      __builtin_amdgcn_wave_barrier();
+      int ncore = 0;   // uniform
+      auto core_pass = [&](const int m) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
+        __builtin_amdgcn_wave_barrier();
+        for (int b = lane; b < m; b += 64) {
+          const uint32_t ci = s_ci[wave][b];
+          const int qq = (int)(ci >> 9), t = (int)((ci >> 3) & 63), j = (int)(ci & (P - 1));
+          const u32x4 h0 = rb[4 * qq], h1 = rb[4 * qq + 1];   // (c, s), (record, hs)
+          const int64_t r = kb + (int64_t)f64i(h1.x, h1.y);
+          const double y = ry[r], am = ra[r];
+          const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)(wave * 64 + t) * P + j) -
+                            f64(h0.x, h0.y)) * f64(h0.z, h0.w);
+          atomicAdd(&s_acc[j * kBlock + wave * 64 + t], am * K_cr64(x, y));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
+        __builtin_amdgcn_wave_barrier();
+      };
+      for (int e0 = 0; e0 < n; e0 += 64) {
+        const int e = e0 + lane;
+        const bool has = e < n;
+        const uint32_t qi = has ? s_q[wave][e] : 0u;
+        const int qq = (int)(qi >> 6), t = (int)(qi & 63);
+        const u32x4 h0 = rb[4 * qq], h1 = rb[4 * qq + 1], h2 = rb[4 * qq + 2];   // (c, s), (record, hs), (c1, ay)
+        const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
+        // A (2y/pi) = (2/sqrt(pi)) ay; idle lanes add 0
+        const double cw = has ? 2.0 * kInvSqrtPi * f64(h2.z, h2.w) : 0.0;
+        const double y = has ? ry[kb + (int64_t)f64i(h1.x, h1.y)] : 1.0;
+        const double y2 = y * y;
+        const double rcore = y >= kFp64Gh10Ymin ? kFp64Gh10R2 : kFp64Gh8R2;
+        const int64_t ib = s0 + (int64_t)(wave * 64 + t) * P;
+        uint32_t cmask = 0;
+        double v[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const double x = (grid_nu(a.nu_min, a.step, ib + j) - rc) * rsv;
+          const double x2 = x * x;
+          const double r = x2 + y2, xmy = x2 - y2, fxy = 4.0 * x2 * y2;
+          const bool core = has && r < rcore;
+          // (a select: at a core point the node sum may be infinite)
+          const double g = gh_sum64<5>(r, xmy, fxy, kGh10T, kGh10W);
+          v[j] = core ? 0.0 : cw * g;
+          cmask |= core ? (1u << j) : 0u;
+        }
+        // Deliver the P values to the owner's sums with plain LDS reads and
+        // writes: in each round every pending lane writes its id to its
+        // owner's claim byte, the lanes that read their own id back hold
+        // distinct owners and update, the rest try again.
+        bool pending = has;
+        while (__ballot(pending)) {   // uniform
+          if (pending) s_claim[wave][t] = (uint8_t)lane;
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          if (pending && s_claim[wave][t] == (uint8_t)lane) {
+            double* const dst = s_acc + wave * 64 + t;
+#pragma unroll
+            for (int j = 0; j < P; ++j) dst[j * kBlock] += v[j];
+            pending = false;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+        // this lane's core points go to the list at an exclusive prefix sum
+        // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
+        const uint32_t cnt8 = __builtin_popcount(cmask);
+        uint32_t pre = 0, total = 0;
+#pragma unroll
+        for (int bit = 0; bit < 4; ++bit) {
+          const uint64_t m = __ballot((cnt8 >> bit) & 1u);
+          pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+          total += (uint32_t)__popcll(m) << bit;
+        }
+        if (total) {   // uniform
+          if (ncore + (int)total > kCoreCap64) {
+            core_pass(ncore);
+            ncore = 0;
+          }
+          int off = ncore + (int)pre;
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            if (cmask & (1u << j)) {
+              s_ci[wave][off] = (uint16_t)((uint32_t)qq << 9 | (uint32_t)t << 3 | (uint32_t)j);
+              ++off;
+            }
+          }
+          ncore += (int)total;
+          if constexpr (kCount) n_core += total;
+        }
+      }
+      if (ncore > 0) core_pass(ncore);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int j = 0; j < P; ++j) {
+        lay[j] += acc[j * kBlock];
+        acc[j * kBlock] = 0.0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    };
+
+    // Phase 1: every group's own near range, eight records per group per
+    // stage (slots 8g .. 8g + 7, one loaded by each lane of the group); in
+    // iteration i every lane evaluates its group's record 8g + i, so each
+    // iteration holds eight records and no lane idles while its group has
+    // records left.  Phase 2: the pairs the group level left to this kernel
+    // (outside the group's own range the four-node form fails at some point
+    // of the group: gh4_holds_within, the test k_far_series_fp64 makes, in
+    // the same arithmetic), only where the series kernel counted some for
+    // this (layer, wavefront): lane l tests record base + l of the
+    // wavefront's near range for all eight groups, the records some group
+    // needs are packed into the stage with that mask, and every lane takes
+    // each, with weight 0 where its group is not in it.  One loop runs the
+    // stages of both phases, so that staging, the record and the drain each
+    // have a single call site.
+    const uint32_t n_left = a.deferred[(int64_t)k * (a.n_wg * kWaves) + wg * kWaves + wave];
+    int32_t p1 = 0;     // phase 1: offset of the next stage in each group's range
+    int32_t p2 = nlo;   // phase 2: next record of the near range to test
+    for (;;) {
+      bool ph2;
+      int slot;
+      int32_t r;
+      bool write, valid;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the previous stage's readers
+      if (p1 < n_own) {   // uniform
+        ph2 = false;
+        slot = lane;      // 8 g + (lane & 7)
+        r = my_lo + p1 + (lane & 7);
+        valid = r < my_hi;
+        write = true;
+      } else if (n_left != 0 && p2 < nhi) {   // uniform
+        ph2 = true;
+        r = p2 + lane;
+        uint32_t gmask = 0;
+        if (r < nhi) {
+          const double c = a.rec.c[kb + r];
+          if (c >= lo_b && c <= hi_b) {
+            const double sv = rs[kb + r], y = ry[kb + r];
+#pragma unroll
+            for (int g = 0; g < kGroupsPerWave; ++g) {
+              const int32_t g_lo = min(max(gl[g], nlo), nhi), g_hi = max(min(gh[g], nhi), g_lo);
+              const double nu_g = group_middle(a.nu_min, a.step, w_first + (int64_t)g * kGroupPoints);
+              const bool left = (r < g_lo || r >= g_hi) && !gh4_holds_within(nu_g - c, sv, y * y, Hg);
+              gmask |= left ? (1u << g) : 0u;
+            }
+          }
+        }
+        const uint64_t keep = __ballot(gmask != 0);
+        slot = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(keep >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)keep, 0u));
+        write = valid = gmask != 0;
+        if (write) s_gmask[wave][slot] = (uint8_t)gmask;
+      } else {
+        break;
+      }
+      const bool inside = write && stage(slot, r, valid);
+      if constexpr (kCount) n_evals += !ph2 && inside && live_g ? g_pts : 0u;
+      const int n_it = ph2 ? (int)__popcll(__ballot(write))
+                           : __builtin_amdgcn_readfirstlane(min(8, n_own - p1));
+      if (ph2) p2 += kStage; else p1 += 8;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // the queue is drained when it could not take one more record, and at
+      // the stage's end (it refers to the stage's slots)
+      int it = 0;
+      for (;;) {
+        for (; it < n_it; ++it) {
+          const int q = ph2 ? it : (int)(my_group * 8) + it;
+          bool direct = true;
+          if (ph2) {   // uniform
+            const uint32_t gm = __builtin_amdgcn_readfirstlane((uint32_t)s_gmask[wave][it]);
+            direct = (gm >> my_group) & 1u;
+            if constexpr (kCount) n_evals += direct && live_g && (lane & 7) == 0 ? g_pts : 0u;
+          }
+          record(q, rb[4 * q], rb[4 * q + 1], rb + 4 * q + 2, direct);
+          if (nq > kQueueW64 - 64) {   // uniform
+            ++it;
             break;
           }
         }
@@ -429,12 +490,8 @@ void k_voigt_sum_fp64(const SumArgs a) {
           if constexpr (kCount) n_queued += (uint64_t)nq;
           drain(nq);
           nq = 0;
-          // the next two records' headers again from LDS (not held in
-          // registers through the drain)
-          if (q < ncnt) { ha0 = pr[0]; ha1 = pr[1]; }
-          if (q + 1 < ncnt) { hb0 = pr[4]; hb1 = pr[5]; }
         }
-        if (q >= ncnt) break;
+        if (it >= n_it) break;
       }
     }
     const double T = a.t_k[k];
@@ -466,10 +523,10 @@ void k_voigt_sum_fp64(const SumArgs a) {
     }
   }
   if constexpr (kCount) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n_evals += __shfl_xor(n_evals, off);
     if (lane == 0) {
-      const int64_t w_first = s0 + (int64_t)wave * kWavePoints;
-      const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
-      count_add(a.work, kWorkNearEvals, n_in * w_pts);
+      count_add(a.work, kWorkNearEvals, n_evals);
       count_add(a.work, kWorkQueued, n_queued);
       count_add(a.work, kWorkCorePoints, n_core);
     }
@@ -622,10 +679,8 @@ void k_far_series_fp64(const SumArgs a) {
   };
   static_assert(kN == 28, "series_terms64's table ends at 28 terms");
   uint64_t n_terms = 0, n_terms2 = 0;   // kCount: terms summed (slice, wavefront level)
-  // the four-node form holds at every point within Hu of the middle point
   auto holds_gh4 = [&](double d, double sv, double y2, double Hu) {
-    const double xn = sv * (fabs(d) - Hu);
-    return fma(xn, xn, y2) >= kFp64Gh4R2;
+    return gh4_holds_within(d, sv, y2, Hu);
   };
   if (threadIdx.x == 0) s_sxn = 0;
   double* const exv = s_exv + threadIdx.x * P;
@@ -822,13 +877,92 @@ void k_far_series_fp64(const SumArgs a) {
     double f = C[kN - 1];
 #pragma unroll
     for (int n = kN - 2; n >= 0; --n) f = fma(f, t, C[n]);
-    const int64_t i = i0 + j;
-    if (i < t1) out[i] = f + exv[j];
+    exv[j] += f;   // (this lane's own slots) far_sum is written after the group level
   }
+
+  // ---- group level: the records of the wavefront's near range [nlo, nhi)
+  // outside a group's own near range [g_lo, g_hi) (records within
+  // kSeriesRatio Hg of its middle), about the group's middle (Hu = Hg = H/8).
+  // Lane = (group lane/8, record slot lane%8): eight records per round, each
+  // tested for all eight groups.  The four-node form's series (two pole
+  // pairs, as at the wavefront level) where that form holds at every point of
+  // the group (holds_gh4); a pair where it does not is left to the near
+  // field, which applies this same test in the same arithmetic, so each
+  // (group, record) pair is taken exactly once.
+  uint32_t n_gdef = 0;                              // pairs left to the near field
+  uint64_t n_gser = 0, n_gdef_pts = 0;              // kCount
+  {
+    const int grp = lane >> 3, r8 = lane & 7;
+    const int64_t g_first = w_first + (int64_t)grp * kGroupPoints;
+    const bool g_live = g_first < t1;
+    const uint64_t g_pts = (uint64_t)max((int64_t)0, min((int64_t)kGroupPoints, t1 - g_first));
+    const double nu_g = group_middle(a.nu_min, a.step, g_first);
+    const double Hg = 0.5 * kGroupPoints * a.step;
+    const int32_t* const gl = a.glo + (wg * kWaves + wave) * kGroupsPerWave;
+    const int32_t* const gh = a.ghi + (wg * kWaves + wave) * kGroupsPerWave;
+    const int32_t g_lo = min(max(gl[grp], nlo), nhi), g_hi = max(min(gh[grp], nhi), g_lo);
+    double G[kN];
+#pragma unroll
+    for (int n = 0; n < kN; ++n) G[n] = 0.0;
+    auto candidate = [&](int32_t r) { return g_live && r < nhi && (r < g_lo || r >= g_hi); };
+    double gc = __builtin_nan(""), gs = 0.0, gy = 1.0, ga = 0.0;
+    if (candidate(nlo + r8)) {
+      gc = rc[nlo + r8]; gs = rs[nlo + r8]; gy = ry[nlo + r8]; ga = ra[nlo + r8];
+    }
+    const int32_t n_rounds = (nhi - nlo + 7) >> 3;
+    for (int32_t it = 0; it < n_rounds; ++it) {
+      const int32_t r = nlo + it * 8 + r8;
+      const double c = gc, sv = gs, y = gy, am = ga;
+      if (candidate(r + 8)) {   // the next round's record, ahead
+        gc = rc[r + 8]; gs = rs[r + 8]; gy = ry[r + 8]; ga = ra[r + 8];
+      }
+      const bool act = candidate(r) && (c >= lo_b) && (c <= hi_b);
+      const double d = nu_g - c;
+      const bool valid = holds_gh4(d, sv, y * y, Hg);
+      const bool ser = act && valid;
+      const int nt = series_terms64(ser, fabs(d) / Hg);
+      if (ser) series_gh4(G, d, sv, y, am, Hg, nt);
+      n_gdef += act && !valid ? 1u : 0u;
+      if constexpr (kCount) {
+        n_terms2 += ser ? (uint64_t)nt : 0u;
+        n_gser += ser ? 1u : 0u;
+        n_gdef_pts += act && !valid ? g_pts : 0u;
+      }
+    }
+    // totals over the group's eight lanes, in each of them
+#pragma unroll
+    for (int off = 4; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int n = 0; n < kN; ++n) G[n] += __shfl_xor(G[n], off);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      // tau of point (lane % 8) * P + j of the group (exact in FP64)
+      const double tg = ((double)(r8 * P + j) - 0.5 * (double)(kGroupPoints - 1)) *
+                        (2.0 / (double)kGroupPoints);
+      double f = G[kN - 1];
+#pragma unroll
+      for (int n = kN - 2; n >= 0; --n) f = fma(f, tg, G[n]);
+      const int64_t i = i0 + j;
+      if (i < t1) out[i] = exv[j] + f;
+    }
+  }
+  // the pairs this wavefront left to the near field (it skips its scan for
+  // them when there are none)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) n_gdef += __shfl_xor(n_gdef, off);
+  if (lane == 0) a.deferred[(int64_t)k * (a.n_wg * kWaves) + wg * kWaves + wave] = n_gdef;
   if constexpr (kCount) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) n_terms2 += __shfl_xor(n_terms2, off);
+    for (int off = 32; off >= 1; off >>= 1) {
+      n_terms2 += __shfl_xor(n_terms2, off);
+      n_gser += __shfl_xor(n_gser, off);
+      n_gdef_pts += __shfl_xor(n_gdef_pts, off);
+    }
     if (lane == 0) {
+      count_add(a.work, kWorkGroupSeries, n_gser);
+      count_add(a.work, kWorkGroupDeferred, n_gdef);
+      count_add(a.work, kWorkGroupDeferredEvals, n_gdef_pts);
       const uint64_t w_pts = (uint64_t)min((int64_t)kWavePoints, t1 - w_first);
       count_add(a.work, kWorkWaveSeries, n_wser);
       count_add(a.work, kWorkSeriesTerms2, n_terms2);

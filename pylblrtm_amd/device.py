@@ -375,22 +375,20 @@ class OpticalDepthProblem:
     def count_series_work(self, precision: str = "mixed") -> dict:
         """Work units of the line sums, split as the kernels do.  A slice of
         SLICE_POINTS points is four wavefronts of WAVE_POINTS, a wavefront
-        eight groups of GROUP_POINTS (mixed modes); the near range of a
-        slice, wavefront or group is the lines whose nu0 lies within
-        SERIES_RATIO times its half-span (+ the shift bound) of its middle
-        point.  slice_pairs: series summed once per (layer, line, slice)
-        outside the slice's near range; wave_pairs: once per (layer, line,
-        wavefront) between the two near ranges; far_pairs: their total.
-        FP64: near_evals = (layer, line in the tile window and the
-        wavefront's near range, point) triples, evaluated point by point.
-        Mixed: group_pairs = (layer, line, group) with the line in the
+        eight groups of GROUP_POINTS; the near range of a slice, wavefront or
+        group is the lines whose nu0 lies within SERIES_RATIO times its
+        half-span (+ the shift bound) of its middle point.  slice_pairs:
+        series summed once per (layer, line, slice) outside the slice's near
+        range; wave_pairs: once per (layer, line, wavefront) between the two
+        near ranges; group_pairs = (layer, line, group) with the line in the
         wavefront's near range but not the group's, which the group series
-        takes unless the one-node form fails at the group (those it leaves to
-        the near field, which the kernels count); near_evals = (layer, line
-        in the group's near range, point of the group) triples.  Groups of
-        lines with a zero column are left out (they add 0)."""
+        takes unless its form fails at the group (the one-node and middle-tier
+        forms mixed, the four-node form FP64; the pairs it leaves to the near
+        field only the kernels count); far_pairs: their total.  near_evals =
+        (layer, line in the group's near range, point of the group) triples,
+        evaluated point by point.  Groups of lines with a zero column are
+        left out (they add 0)."""
         wave_pts, slice_pts, group_pts, ratio = 512, 2048, 64, 3.0
-        mixed = precision != "fp64"
         t0, t1, lo_b, hi_b = self.tile_edges()
         h_nu0, h_shift, h_group = self._host_lines()
         margin = abs(self.shift_bound) * (1 + 1e-9) + 1e-9
@@ -409,7 +407,6 @@ class OpticalDepthProblem:
         w_first, w_tile, w_mid, w_half = units(wave_pts)
         s_first, s_tile, s_mid, s_half = units(slice_pts)
         g_first, g_tile, g_mid, g_half = units(group_pts)
-        w_pts = np.minimum(t1[w_tile] - w_first, wave_pts)
         g_pts = np.minimum(t1[g_tile] - g_first, group_pts)
         # slice of each wavefront, wavefront of each group
         w_slice = np.searchsorted(s_first, w_first, side="right") - 1
@@ -438,14 +435,11 @@ class OpticalDepthProblem:
                                 - np.searchsorted(nu0, w_mid - w_half, side="left"), s_near[w_slice])
             slice_pairs += int(np.sum(win[s_tile] - s_near))
             wave_pairs += int(np.sum(s_near[w_slice] - w_near))
-            if mixed:
-                g_near = np.minimum(np.searchsorted(nu0, g_mid + g_half, side="right")
-                                    - np.searchsorted(nu0, g_mid - g_half, side="left"),
-                                    w_near[g_wave])
-                near_evals += int(np.sum(g_near * g_pts))
-                group_pairs += int(np.sum(w_near[g_wave] - g_near))
-            else:
-                near_evals += int(np.sum(w_near * w_pts))
+            g_near = np.minimum(np.searchsorted(nu0, g_mid + g_half, side="right")
+                                - np.searchsorted(nu0, g_mid - g_half, side="left"),
+                                w_near[g_wave])
+            near_evals += int(np.sum(g_near * g_pts))
+            group_pairs += int(np.sum(w_near[g_wave] - g_near))
         return {"near_evals": near_evals, "slice_pairs": slice_pairs, "wave_pairs": wave_pairs,
                 "group_pairs": group_pairs, "far_pairs": slice_pairs + wave_pairs + group_pairs}
 

@@ -83,17 +83,15 @@ def _check_kernel_counts(prob, precision):
     say(f"  kernels {kw}")
     say(f"  host    {hw}")
     assert kw["rescans"] == 0
+    # the near field evaluates the lines in each group's own near range, and
+    # those the group series left to it (its form fails at some point of the
+    # group: one-node and middle tier mixed, four-node FP64); the group level
+    # sees every other line of the wavefront's near range once per group
+    assert kw["near_evals"] == hw["near_evals"] + kw["group_deferred_evals"]
+    assert kw["group_series"] + kw["group_deferred"] == hw["group_pairs"]
+    assert (kw["group_deferred"] == 0) == (kw["group_deferred_evals"] == 0)
     if precision == "fp64":
-        assert kw["near_evals"] == hw["near_evals"]
-        assert kw["group_series"] == kw["group_deferred"] == hw["group_pairs"] == 0
-    else:
-        # the near field evaluates the lines in each group's own near range,
-        # and those the group series left to it (the one-node form fails at
-        # some point of the group); the group level sees every other line of
-        # the wavefront's near range once per group
-        assert kw["near_evals"] == hw["near_evals"] + kw["group_deferred_evals"]
-        assert kw["group_series"] + kw["group_deferred"] == hw["group_pairs"]
-        assert (kw["group_deferred"] == 0) == (kw["group_deferred_evals"] == 0)
+        assert kw["group_series_middle"] == 0
     assert kw["slice_series"] + kw["slice_exceptions"] == hw["slice_pairs"]
     # the wavefront level sees the records between the two near ranges, plus
     # every record the slice level passed on, once per live wavefront
