@@ -47,14 +47,14 @@ FLOP_PER_EVAL = 16            # per point-by-point evaluation, every mode charge
 #     denominator polynomials (27), rcp + Newton step, four ratios and the
 #     first four terms (27), then 24 terms of a four-term recurrence + add (192)
 #     = 246;
-#   FP64 wavefront level (four-node form as two pole pairs, one 28-term
-#     recurrence each): 10 shared + 2 x (22 + 219) = 492.
+#   FP64 wavefront and group levels (four-node form as two pole pairs, one
+#     28-term recurrence each): 10 shared + 2 x (22 + 219) = 492.
 #   mixed group level, middle-tier form (where the one-node form fails at the
 #     group; quartic denominator, 18 FP32 terms): x, the form's constants,
 #     u's and D's coefficients, rcp and four ratios (26), the first four terms
 #     (18), then 14 terms of three fma + mul (98) and 18 adds = 160.
 FLOP_PER_PAIR = {"mixed": {"slice": 76, "wave": 76, "group_middle": 160},
-                 "fp64": {"slice": 246, "wave": 492}}
+                 "fp64": {"slice": 246, "wave": 492, "group": 492}}
 # The mixed near field's near-centre forms, counted from the source (an fma
 # counts 2, a transcendental 1): a queued point in pass A (six-node form: x,
 # r, x^2 - y^2, 4 x^2 y^2; three pole pairs combined into one quotient; one
@@ -63,6 +63,16 @@ FLOP_PER_PAIR = {"mixed": {"slice": 76, "wave": 76, "group_middle": 160},
 # 95.  Each is on top of the point's FLOP_PER_EVAL in the near-field tiers.
 FLOP_PER_QUEUED_POINT = 35
 FLOP_PER_CORE_POINT = 95
+# The FP64 near field's, counted the same way: a queued point in pass A
+# (ten-node form: x, r, x^2 - y^2, 4 x^2 y^2 (9); five pole pairs into one
+# quotient (41); one reciprocal with its Newton step and the product (6); the
+# weight and the delivery (2)) 58; a core point in pass B (x (4); lattice
+# choice, r, x^2 - y^2, 4 x^2 y^2 (8); thirteen pole pairs into one quotient
+# (156); reciprocal and scale (10); the residue's bound and its test (6); the
+# product and the delivery (2)) 186, the residue itself, summed only near the
+# real axis, not counted.
+FLOP_PER_QUEUED_POINT64 = 58
+FLOP_PER_CORE_POINT64 = 186
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -145,10 +155,12 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
     if base == "fp64" and kw.get("series_terms"):
         # the FP64 kernel sums only the terms each round's nearest record
         # needs: a slice pair costs 54 FLOP plus 8 per term after the fourth
-        # (246 at 28), a wavefront pair 10 + 2 x (49 + 8 per term after the
-        # fourth) (492 at 28)
+        # (246 at 28), a wavefront or group pair 10 + 2 x (49 + 8 per term
+        # after the fourth) (492 at 28); the kernel counts the terms of both
+        # four-node levels together
         far_flop = (22 * kw["slice_series"] + 8 * kw["series_terms"]
-                    + 44 * kw["wave_series"] + 16 * kw.get("series_terms_wave", 0)
+                    + 44 * (kw["wave_series"] + kw.get("group_series", 0))
+                    + 16 * kw.get("series_terms_wave", 0)
                     + kw["exception_evals"] * FLOP_PER_EVAL)
     if base == "mixed" and kw.get("series_terms"):
         # the mixed kernel sums only the terms each round's nearest record
@@ -172,15 +184,15 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
                     "flop_per_unit": FLOP_PER_EVAL, "queued_near_centre_pairs": kw["queued"],
                     "queued_core_points": kw["core_points"]},
     }
-    if base == "mixed" and kw.get("queued") is not None:
+    if base in ("mixed", "fp64") and kw.get("queued") is not None:
         # the near-centre forms' work on top of the tiers' (per evaluation,
         # averaged over the launch's point evaluations)
-        near_flop = (kw["near_evals"] * FLOP_PER_EVAL + kw["queued"] * 8 * FLOP_PER_QUEUED_POINT
-                     + kw["core_points"] * FLOP_PER_CORE_POINT)
+        fq, fc = ((FLOP_PER_QUEUED_POINT64, FLOP_PER_CORE_POINT64) if base == "fp64"
+                  else (FLOP_PER_QUEUED_POINT, FLOP_PER_CORE_POINT))
+        near_flop = kw["near_evals"] * FLOP_PER_EVAL + kw["queued"] * 8 * fq + kw["core_points"] * fc
         kern[near_name]["flop_per_unit"] = near_flop / max(kw["near_evals"], 1)
         kern[near_name]["flop_breakdown"] = {
-            "tiers": FLOP_PER_EVAL, "per_queued_point": FLOP_PER_QUEUED_POINT,
-            "per_core_point": FLOP_PER_CORE_POINT}
+            "tiers": FLOP_PER_EVAL, "per_queued_point": fq, "per_core_point": fc}
     for name, d in kern.items():
         d["achieved_tflops"] = d["units"] * d["flop_per_unit"] / (max(d["ms"], 1e-9) * 1e-3) / 1e12
         d["peak"] = peak

@@ -105,7 +105,7 @@ typedef __attribute__((address_space(3))) Rec64 lds_rec64;
 // 4 workgroups per CU need under 40 KB of LDS each: 16 KB of records, 16 KB
 // of FP64 partial sums, and per wavefront a queue of (record, lane) entries
 // and a list of core points.
-constexpr int kQueueW64 = 256;    // queue entries per wavefront (the loop drains at kQueueW64 - 128)
+constexpr int kQueueW64 = 256;    // queue entries per wavefront (drained when more than kQueueW64 - 64 are pending)
 constexpr int kCoreCap64 = 512;   // >= the 64 x 8 points one pass-A round can add
 
 // FP64 line sum.  Same tiling as the mixed kernel: per record and thread
