@@ -88,13 +88,23 @@ __device__ __forceinline__ float radiation32(float nu, float T) {
   return nu * tanhf(0.5f * (float)kC2 * nu / T);
 }
 
-// One thread per (layer, line).  Grid: x over lines, y over layers.
+// One thread per (layer, line).  Grid: one workgroup per (256-line chunk,
+// layer), chunk-major and XCD-aware, so the layers of a chunk run on one XCD
+// at about the same time and read the chunk's line parameters from its L2
+// instead of streaming the whole line list once per layer; a 2-D grid (x over
+// chunks, y over layers) where the 1-D one would exceed its size limit.
 // R = double: everything in FP64 (LBL_FP64).
 // R = float : the centre and the Boltzmann exponent in FP64, the rest FP32.
 template <typename R>
 __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
   __shared__ double s_q[kMaxGroups];
-  const int k = blockIdx.y;
+  int64_t chunk = blockIdx.x;
+  int k = blockIdx.y;
+  if (gridDim.y == 1 && a.layers.n_layers > 1) {
+    const int64_t w = xcd_remap(blockIdx.x, gridDim.x);
+    chunk = w / a.layers.n_layers;
+    k = (int)(w - chunk * a.layers.n_layers);
+  }
   const double T = a.layers.t_k[k];
   const double P = a.layers.p_atm[k];
   // Q(T_ref) / Q(T_k) per group (forward.py:236-240), once per workgroup.
@@ -116,7 +126,7 @@ __global__ __launch_bounds__(256) void k_line_params(const LineParamsArgs a) {
   }
   __syncthreads();
 
-  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t l = chunk * blockDim.x + threadIdx.x;
   if (l >= a.lines.n_lines) return;
   const int64_t out = (int64_t)k * a.lines.n_lines + l;
   const int g = a.lines.group[l];
@@ -274,7 +284,11 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_line_params(const LineParamsArgs& a, int precision, hipStream_t s) {
   if (a.lines.n_lines <= 0 || a.layers.n_layers <= 0) return hipSuccess;
-  dim3 grid((unsigned)((a.lines.n_lines + 255) / 256), (unsigned)a.layers.n_layers);
+  const int64_t chunks = (a.lines.n_lines + 255) / 256;
+  const int64_t blocks = chunks * a.layers.n_layers;
+  // 1-D while the grid's thread count fits in 32 bits
+  const dim3 grid = blocks < (int64_t(1) << 24) ? dim3((unsigned)blocks, 1)
+                                                 : dim3((unsigned)chunks, (unsigned)a.layers.n_layers);
   if (precision == LBL_FP64)
     k_line_params<double><<<grid, 256, 0, s>>>(a);
   else
