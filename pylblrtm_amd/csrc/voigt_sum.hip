@@ -1737,14 +1737,23 @@ __device__ __forceinline__ void voigt_sum_mixed_body(const SumArgs& a) {
           for (int j = 0; j < P; ++j) {
             const float x = fmaf((float)j, hs, x0);
             const float x2 = x * x;
-            const float r = x2 + y2, xmy = x2 - y2, fxy = x2 * fy2;
+            const float r = fmaf(x, x, y2), xmy = fmaf(x, x, -y2), fxy = x2 * fy2;
             const bool core = has && r < rcore;
             // (a select: at a core point the node sum may be infinite)
-            float g = c2 * gh6_sum32(r, xmy, fxy);
-            if (far_pts)   // uniform
-              g = r > 1e6f ? ay * __builtin_amdgcn_rcpf(r - 1.5f) : g;
+            const float g = c2 * gh6_sum32(r, xmy, fxy);
             v[j] = core ? 0.0f : g;
             cmask |= core ? (1u << j) : 0u;
+          }
+          // (a branch of its own, so that the common case does not also
+          // compute the far form; the same r as above, so core points never
+          // take it: gh6_core_r2 < 1e6)
+          if (far_pts) {   // uniform
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              const float x = fmaf((float)j, hs, x0);
+              const float r = fmaf(x, x, y2);
+              if (r > 1e6f) v[j] = ay * __builtin_amdgcn_rcpf(r - 1.5f);
+            }
           }
           // this lane's core points go to the list at an exclusive prefix sum
           // of the lanes' counts (bit planes of the 0..8 count, one ballot
