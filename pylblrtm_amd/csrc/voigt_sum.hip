@@ -89,15 +89,14 @@ __device__ __forceinline__ bool gh4_holds_within(double d, double sv, double y2,
 __device__ __noinline__ double K_fp64_call(double x, double y) { return K_fp64(x, y); }
 __device__ __noinline__ float K_mixed_call(float x, float y) { return K_mixed(x, y); }
 
-// Record of the FP64 line sum as staged in LDS (one per (layer, line)); the
-// queue drain reads y and A from the global records.  hw2, hw4: |x| at the
+// Record of the FP64 line sum as staged in LDS (one per (layer, line)), with
+// everything the tiers and the queue drain read.  hw2, hw4: |x| at the
 // middle of a thread's run above which all P points are in the second-order
 // shifted and four-node regions.
 struct alignas(16) Rec64 {
   double c, s;        // shifted centre, sqrt(ln2)/alpha_D
-  int64_t rec;        // the line's index (the drain reads y and A from the layer's arrays)
-  double hs;          // step*s
-  double c1, ay;      // y^2 - 3/2; A y/sqrt(pi)
+  double y, hs;       // Lorentz/Doppler ratio; step*s
+  double am, ay;      // the line's weight A; A y/sqrt(pi)
   double hw2, hw4;    // sqrt(max(s2_r2_min64(y^2) - y^2, 0)), sqrt(max(35^2 - y^2, 0)), each + (P-1)/2 hs
 };
 static_assert(sizeof(Rec64) == 64, "four 16-byte LDS reads per record");
@@ -110,13 +109,13 @@ constexpr int kCoreCap64 = 512;   // >= the 64 x 8 points one pass-A round can a
 
 // FP64 line sum.  Same tiling as the mixed kernel: per record and thread
 // the middle x of the thread's P points decides, for all of them at once,
-// between the shifted one-node form and its second-order refinement
-// (faddeeva.h, error <= 1e-11), the 4-node form (|z| >= 35) and the
-// near-centre queue.  The wavefront drains its queue when it could not take
-// two more records, in two dense passes: pass A evaluates every queued point
-// with the 10-node form and delivers it to the owner's sums with plain LDS
-// read-modify-writes; the points it cannot hold (|z| < 7, the core) go to a
-// list that pass B evaluates one point per lane with trapezoid + residue.
+// between the second-order shifted form (faddeeva.h, error <= 1e-11), the
+// 4-node form (|z| >= 35) and the near-centre queue.  The wavefront drains
+// its queue when it could not take one more record, in two dense passes:
+// pass A evaluates every queued point with the 10-node form and delivers it
+// to the owner's sums with plain LDS read-modify-writes; the points it
+// cannot hold (|z| < 7, the core) go to a list that pass B evaluates one
+// point per lane with trapezoid + residue.
 template <int P, bool kCount>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void k_voigt_sum_fp64(const SumArgs a) {
@@ -188,7 +187,6 @@ void k_voigt_sum_fp64(const SumArgs a) {
   }
   const volatile lds_u32x4* const rb = (const volatile lds_u32x4*)(lds_rec64*)s_rec[wave];
   auto f64 = [](unsigned lo, unsigned hi) { return __hiloint2double((int)hi, (int)lo); };
-  auto f64i = [](unsigned lo, unsigned hi) { return (int64_t)(((uint64_t)hi << 32) | lo); };
 
   for (int k = k_begin; k < k_end; ++k) {
     double lay[P];
@@ -196,9 +194,9 @@ void k_voigt_sum_fp64(const SumArgs a) {
     for (int j = 0; j < P; ++j) lay[j] = 0.0;
     const int64_t kb = (int64_t)k * a.n_lines;
     int nq = 0;   // queue entries not yet drained (uniform)
-    // the staged record of line r of this layer: the tier thresholds, and the
-    // line's index (for the drain's y and A); a record outside the tile
-    // window (or skipped) adds 0 and is never queued
+    // the staged record of line r of this layer: the tier thresholds, and y
+    // and A for the drain; a record outside the tile window (or skipped)
+    // adds 0 and is never queued
     auto stage = [&](const int slot, const int32_t r, const bool valid) {
       double c = 0.0, sv = 0.0, y = 1.0e3, am = 0.0;
       bool inside = false;
@@ -210,26 +208,25 @@ void k_voigt_sum_fp64(const SumArgs a) {
       Rec64 R;
       const double y2 = y * y, hs = a.step * sv;
       R.c = c; R.s = sv; R.hs = hs;
-      R.rec = valid ? r : 0;
       R.hw2 = fma(half, hs, sqrt(fmax(s2_r2_min64(y2) - y2, 0.0)));
       R.hw4 = fma(half, hs, sqrt(fmax(kFp64Gh4R2 - y2, 0.0)));
-      R.c1 = y2 - 1.5; R.ay = am * y * kInvSqrtPi;
+      R.y = y; R.am = am; R.ay = am * y * kInvSqrtPi;
       s_rec[wave][slot] = R;
       return inside;
     };
-    // Records are read from LDS as 16-byte pieces: (c, s) and (record, hs)
-    // one iteration ahead, (c1, ay) and (hw2, hw4) at the record.  The group
+    // Records are read from LDS as 16-byte pieces: (c, s) and (y, hs) one
+    // iteration ahead, (A, ay) and (hw2, hw4) at the record.  The group
     // level summed every (group, record) pair outside the group's own near
     // range where the four-node form holds at all its points, so the
     // one-node form (|z| >= 630 for narrow lines) is never reached here.
     auto record = [&](const int q, const u32x4 h0, const u32x4 h1,
                       const volatile lds_u32x4* const more, const bool direct) {
       const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
-      const double hs = f64(h1.z, h1.w);
+      const double y = f64(h1.x, h1.y), hs = f64(h1.z, h1.w);
       const double xm = (nu_m - rc) * rsv;
       const double ax = fabs(xm);
       const u32x4 h2 = more[0], h3 = more[1];
-      const double c1 = f64(h2.x, h2.y), ay = direct ? f64(h2.z, h2.w) : 0.0;
+      const double y2 = y * y, ay = direct ? f64(h2.z, h2.w) : 0.0;
       const double hw2 = f64(h3.x, h3.y), hw4 = f64(h3.z, h3.w);
       // One tier for the whole wavefront, the lowest any lane needs (each
       // form holds wherever the ones above it do): no lane pays for two
@@ -240,7 +237,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
         // r = x^2 + y^2, D = r^2 - 3/2 r + 2 y^2 - 3/2 >= 6.9e7; the quotients
         // by batch inversion in two batches of four (a product of four D
         // ~ |z|^16 stays inside the FP64 range for |z| < 1e19)
-        const double y2 = c1 + 1.5, cc = fma(2.0, c1, 1.5);
+        const double cc = fma(2.0, y2, -1.5);
         const double x0 = fma(-half, hs, xm);
         double rr[P], D[P];
 #pragma unroll
@@ -267,7 +264,7 @@ void k_voigt_sum_fp64(const SumArgs a) {
       // v_rcp_f64 (a product of two D0 D1 ~ |z|^16)
       const bool near = direct && ax < hw4;
       {
-        const double y2 = c1 + 1.5, fy2 = 4.0 * y2;
+        const double fy2 = 4.0 * y2;
         const double ayw = near ? 0.0 : ay;
         const double w0 = ayw * (2.0 * kInvSqrtPi * kGh4W[0]);
         const double w1 = ayw * (2.0 * kInvSqrtPi * kGh4W[1]);
@@ -316,9 +313,8 @@ void k_voigt_sum_fp64(const SumArgs a) {
         for (int b = lane; b < m; b += 64) {
           const uint32_t ci = s_ci[wave][b];
           const int qq = (int)(ci >> 9), t = (int)((ci >> 3) & 63), j = (int)(ci & (P - 1));
-          const u32x4 h0 = rb[4 * qq], h1 = rb[4 * qq + 1];   // (c, s), (record, hs)
-          const int64_t r = kb + (int64_t)f64i(h1.x, h1.y);
-          const double y = ry[r], am = ra[r];
+          const u32x4 h0 = rb[4 * qq], h1 = rb[4 * qq + 1], h2 = rb[4 * qq + 2];   // (c, s), (y, hs), (A, ay)
+          const double y = f64(h1.x, h1.y), am = f64(h2.x, h2.y);
           const double x = (grid_nu(a.nu_min, a.step, s0 + (int64_t)(wave * 64 + t) * P + j) -
                             f64(h0.x, h0.y)) * f64(h0.z, h0.w);
           atomicAdd(&s_acc[j * kBlock + wave * 64 + t], am * K_cr64(x, y));
@@ -331,11 +327,11 @@ void k_voigt_sum_fp64(const SumArgs a) {
         const bool has = e < n;
         const uint32_t qi = has ? s_q[wave][e] : 0u;
         const int qq = (int)(qi >> 6), t = (int)(qi & 63);
-        const u32x4 h0 = rb[4 * qq], h1 = rb[4 * qq + 1], h2 = rb[4 * qq + 2];   // (c, s), (record, hs), (c1, ay)
+        const u32x4 h0 = rb[4 * qq], h1 = rb[4 * qq + 1], h2 = rb[4 * qq + 2];   // (c, s), (y, hs), (A, ay)
         const double rc = f64(h0.x, h0.y), rsv = f64(h0.z, h0.w);
         // A (2y/pi) = (2/sqrt(pi)) ay; idle lanes add 0
         const double cw = has ? 2.0 * kInvSqrtPi * f64(h2.z, h2.w) : 0.0;
-        const double y = has ? ry[kb + (int64_t)f64i(h1.x, h1.y)] : 1.0;
+        const double y = has ? f64(h1.x, h1.y) : 1.0;
         const double y2 = y * y;
         const double rcore = y >= kFp64Gh10Ymin ? kFp64Gh10R2 : kFp64Gh8R2;
         const int64_t ib = s0 + (int64_t)(wave * 64 + t) * P;
