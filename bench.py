@@ -121,11 +121,11 @@ def time_mode(prob, precision, mus, steps, warmup, capi, torch):
 def layer_groups(prob, precision):
     """Layer ranges of the line sum for this grid (layer_groups_for in
     csrc/internal.h: enough (slice, layer range) workgroups to fill the GPU,
-    65,536 mixed / 16,384 FP64, at most one range per layer)."""
+    25,000 in both precisions, at most one range per layer)."""
     pts = 256 * 8                      # points per workgroup slice, both precisions
     n_tiles = -(-prob.n_nu // prob.tile_samples)
     n_wg = n_tiles * -(-prob.tile_samples // pts)
-    target = 16384 if precision == "fp64" else 65536
+    target = 25000
     if n_wg >= target or prob.n_layers <= 1:
         return 1
     return min(-(-target // n_wg), prob.n_layers)

@@ -325,11 +325,14 @@ enum WorkCount {
 
 // Workgroups a line-sum launch aims for.  Many short workgroups (a layer
 // range of a tile slice each) let the hardware scheduler balance the uneven
-// tiles: on the five-gas workload (1001 slices, 50 layers) the mixed sum
-// measured 333 / 310 / 284 / 275 / 265 ms at 1 / 2 / 5 / 17 / 50 layer
-// groups, the FP64 sum 837 / 762 / 590 / 536 / 548 ms.
-constexpr int64_t kTargetWorkgroupsMixed = 65536;
-constexpr int64_t kTargetWorkgroupsFp64 = 16384;
+// tiles; fewer layer groups cost fewer workgroup prologues and less traffic
+// of partial optical depths.  On the five-gas workload (1001 slices, 50
+// layers) the mixed near field (with the epilogue) measured 5.39 / 5.27 ms
+// at 50 / 25 layer groups (one / two layers each; uneven ranges such as 20,
+// 30 or 40 groups of one to three layers measured 5.6-6.1 ms), the FP64 one
+// 24.8 / 24.4 ms at 17 / 25 groups.
+constexpr int64_t kTargetWorkgroupsMixed = 25000;
+constexpr int64_t kTargetWorkgroupsFp64 = 25000;
 
 // Number of layer ranges for a grid of n_wg tile slices.
 inline int32_t layer_groups_for(int64_t n_wg, int32_t n_layers, int precision) {
