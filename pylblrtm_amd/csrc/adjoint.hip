@@ -43,6 +43,7 @@ constexpr int kVjpFold = 32;
 constexpr int kAdjTerms = 22;
 constexpr int kAdjSliceExCap = 1024;
 constexpr int kAdjQueueCap = 256;   // near-centre (record, lane) pairs per wavefront per gas run
+constexpr int kAdjCoreCap = 512;    // core points (|z| < 8) of queued pairs listed per wavefront (>= 64 x 8)
 
 struct GradArgs {
   lbl_lines_t lines;
@@ -218,6 +219,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
   __shared__ int32_t s_sx[kAdjSliceExCap];   // slice-level exceptions
   __shared__ int s_sxn;
   __shared__ uint16_t s_aq[kWaves][kAdjQueueCap];   // near-centre queue: (staged record, lane)
+  __shared__ uint16_t s_acl[kWaves][kAdjCoreCap];   // its core points: (queue entry, point)
   __shared__ float s_at[kBlock], s_av[kBlock];       // queue results per owning thread
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int64_t lw = xcd_remap(blockIdx.x, a.n_wg * a.n_layers);
@@ -454,6 +456,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       acc_v = fmaf(wr, N.av * AK + Kx * N.bxv + Ky * N.dyv, acc_v);
     }
   };
+  // The drain's first pass: the same sums at the points of lane o outside
+  // the core (|z| >= 8) in the four-node form, whose gradient is the
+  // node-pair form of kgrad_mixed; the core points are flagged in cmask for
+  // the second pass (kgrad_cr32, one point per lane).  (A select: at a core
+  // point the four-node sum may be infinite.)
+  auto near_pair_a = [&](int qi, int o, const float (&wr_o)[P], float& acc_t, float& acc_v,
+                         uint32_t& cmask) {
+    const VjpFar& R = s_far[qi];
+    const VjpNear& N = s_near[qi];
+    const int64_t i0o = t0 + (int64_t)slice * (kBlock * P) + (int64_t)(wv * 64 + o) * P;
+    const double nu_mo = __dadd_rn(grid_nu(a.nu_min, a.step, i0o), 0.5 * (P - 1) * a.step);
+    const float xm = (float)(nu_mo - R.c) * R.s;
+    const float y = N.y;
+    const float t2s[2] = {(float)(kGh4T[0] * kGh4T[0]), (float)(kGh4T[1] * kGh4T[1])};
+    const float cs[2] = {(float)(2.0 * kGh4W[0] * kInvPi), (float)(2.0 * kGh4W[1] * kInvPi)};
+    acc_t = 0.0f;
+    acc_v = 0.0f;
+    cmask = 0u;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+      const bool core = x * x + y * y < kF32Gh4R2;
+      float K, Kx, Ky;
+      kgrad_pairs32<2>(x, y, t2s, cs, K, Kx, Ky);
+      const float AK = N.a * K;
+      Kx *= N.a;
+      Ky *= N.a;
+      const float ct = wr_o[j] * (N.at * AK + Kx * (ax_t * x) + Ky * N.dyt);
+      const float cv = wr_o[j] * (N.av * AK + Kx * N.bxv + Ky * N.dyv);
+      acc_t += core ? 0.0f : ct;
+      acc_v += core ? 0.0f : cv;
+      cmask |= core ? (1u << j) : 0u;
+    }
+  };
   for (int32_t base = 0; base < n_stage; base += kVjpChunk) {
     const int cnt = min(kVjpChunk, n_stage - base);
     __syncthreads();   // previous chunk consumed (s_far, s_near, s_wcnt, s_idx, s_ds1)
@@ -608,6 +644,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       if (naq > 0) {   // uniform
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        // second pass: the listed core points, one per lane (every lane takes
+        // part in the owners' WR shuffles)
+        auto core_pass = [&](const int m) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // the list writes
+          __builtin_amdgcn_wave_barrier();
+          for (int b0 = 0; b0 < m; b0 += 64) {   // uniform
+            const int b = b0 + lane;
+            const bool ok = b < m;
+            const uint32_t item = ok ? (uint32_t)s_acl[wv][b] : 0u;
+            const int j = (int)(item & 7u);
+            const uint32_t ent = (uint32_t)s_aq[wv][item >> 3];
+            const int o = (int)(ent & 63u), qi = (int)(ent >> 6);
+            float wr = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < P; ++jj) {
+              const float w_ = __shfl(WR[jj], o);
+              wr = jj == j ? w_ : wr;
+            }
+            if (ok) {
+              const VjpFar& R = s_far[qi];
+              const VjpNear& N = s_near[qi];
+              const int64_t i0o = t0 + (int64_t)slice * (kBlock * P) + (int64_t)(wv * 64 + o) * P;
+              const double nu_mo = __dadd_rn(grid_nu(a.nu_min, a.step, i0o), 0.5 * (P - 1) * a.step);
+              const float xm = (float)(nu_mo - R.c) * R.s;
+              const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
+              float K, Kx, Ky;
+              kgrad_cr32(x, N.y, K, Kx, Ky);
+              const float AK = N.a * K;
+              Kx *= N.a;
+              Ky *= N.a;
+              atomicAdd(&s_at[wv * 64 + o], wr * (N.at * AK + Kx * (ax_t * x) + Ky * N.dyt));
+              atomicAdd(&s_av[wv * 64 + o], wr * (N.av * AK + Kx * N.bxv + Ky * N.dyv));
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");   // before the list is reused
+          __builtin_amdgcn_wave_barrier();
+        };
+        int ncore = 0;   // uniform
         for (int e0 = 0; e0 < naq; e0 += 64) {   // every lane takes part (the owners' WR by shuffle)
           const int e = e0 + lane;
           const uint32_t ent = e < naq ? (uint32_t)s_aq[wv][e] : 0u;
@@ -615,13 +689,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
           float wro[P];
 #pragma unroll
           for (int j = 0; j < P; ++j) wro[j] = __shfl(WR[j], o);
+          float at_ = 0.0f, av_ = 0.0f;
+          uint32_t cmask = 0u;
+          if (e < naq) near_pair_a((int)(ent >> 6), o, wro, at_, av_, cmask);
+          // this lane's core points go to the list at an exclusive prefix sum
+          // of the lanes' counts (bit planes of the 0..8 count, one ballot each)
+          const uint32_t cn = __builtin_popcount(cmask);
+          uint32_t pre = 0, total = 0;
+#pragma unroll
+          for (int bit = 0; bit < 4; ++bit) {
+            const uint64_t m = __ballot((cn >> bit) & 1u);
+            pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+            total += (uint32_t)__popcll(m) << bit;
+          }
+          if (total) {   // uniform (<= 64 P = kAdjCoreCap)
+            if (ncore + (int)total > kAdjCoreCap) {
+              core_pass(ncore);
+              ncore = 0;
+            }
+            int off = ncore + (int)pre;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+              if (cmask & (1u << j)) {
+                s_acl[wv][off] = (uint16_t)((uint32_t)e << 3 | (uint32_t)j);
+                ++off;
+              }
+            }
+            ncore += (int)total;
+          }
           if (e < naq) {
-            float at_, av_;
-            near_pair((int)(ent >> 6), o, wro, at_, av_);
             atomicAdd(&s_at[wv * 64 + o], at_);
             atomicAdd(&s_av[wv * 64 + o], av_);
           }
         }
+        if (ncore > 0) core_pass(ncore);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
         ft += s_at[tid];
