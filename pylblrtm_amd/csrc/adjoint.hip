@@ -43,7 +43,13 @@ constexpr int kVjpFold = 32;
 constexpr int kAdjTerms = 22;
 constexpr int kAdjSliceExCap = 1024;
 constexpr int kAdjQueueCap = 256;   // near-centre (record, lane) pairs per wavefront per gas run
-constexpr int kAdjCoreCap = 512;    // core points (|z| < 8) of queued pairs listed per wavefront (>= 64 x 8)
+constexpr int kAdjCoreCap = 512;    // core points of queued pairs listed per wavefront (>= 64 x 8)
+// The drain's first pass: in FP32 the six-node form's K and gradient are
+// within 1e-6 of K and of |grad K| from |z| = 5.5 (from 7 for y < 1e-4,
+// where the Gaussian exp(-x^2) it leaves out is not yet negligible against
+// y/(sqrt(pi) x^2)), better than the four-node form from |z| = 8, where
+// kgrad_mixed switches to it; inside, the core.
+__device__ __forceinline__ float adj_core_r2(float y) { return y < 1e-4f ? 49.0f : 30.25f; }
 
 struct GradArgs {
   lbl_lines_t lines;
@@ -457,10 +463,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     }
   };
   // The drain's first pass: the same sums at the points of lane o outside
-  // the core (|z| >= 8) in the four-node form, whose gradient is the
+  // the core (|z|^2 >= adj_core_r2(y)) in the six-node form, its gradient in the
   // node-pair form of kgrad_mixed; the core points are flagged in cmask for
   // the second pass (kgrad_cr32, one point per lane).  (A select: at a core
-  // point the four-node sum may be infinite.)
+  // point the node sum may be infinite.)
   auto near_pair_a = [&](int qi, int o, const float (&wr_o)[P], float& acc_t, float& acc_v,
                          uint32_t& cmask) {
     const VjpFar& R = s_far[qi];
@@ -468,18 +474,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     const int64_t i0o = t0 + (int64_t)slice * (kBlock * P) + (int64_t)(wv * 64 + o) * P;
     const double nu_mo = __dadd_rn(grid_nu(a.nu_min, a.step, i0o), 0.5 * (P - 1) * a.step);
     const float xm = (float)(nu_mo - R.c) * R.s;
-    const float y = N.y;
-    const float t2s[2] = {(float)(kGh4T[0] * kGh4T[0]), (float)(kGh4T[1] * kGh4T[1])};
-    const float cs[2] = {(float)(2.0 * kGh4W[0] * kInvPi), (float)(2.0 * kGh4W[1] * kInvPi)};
+    const float y = N.y, rc2 = adj_core_r2(y);
+    const float cs[3] = {kGh6W[0] * (float)(2.0 * kInvPi), kGh6W[1] * (float)(2.0 * kInvPi),
+                         kGh6W[2] * (float)(2.0 * kInvPi)};
     acc_t = 0.0f;
     acc_v = 0.0f;
     cmask = 0u;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const float x = fmaf((float)j - 0.5f * (float)(P - 1), R.hs, xm);
-      const bool core = x * x + y * y < kF32Gh4R2;
+      const bool core = x * x + y * y < rc2;
       float K, Kx, Ky;
-      kgrad_pairs32<2>(x, y, t2s, cs, K, Kx, Ky);
+      kgrad_pairs32<3>(x, y, kGh6T2, cs, K, Kx, Ky);
       const float AK = N.a * K;
       Kx *= N.a;
       Ky *= N.a;

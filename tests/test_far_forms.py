@@ -311,3 +311,63 @@ def test_six_node_form_outside_its_core_radius():
     xs = np.linspace(4.3, 4.7, 41)
     ys = np.full_like(xs, 1e-2)
     assert np.max(np.abs(k_gh6_f32(xs, ys) / wofz(xs + 1j * ys).real - 1.0)) > 5e-6
+
+
+def kgrad_pairs_f32(x, y, t2s, ws):
+    """faddeeva.h kgrad_pairs32 (K, dK/dx, dK/dy of a Gauss-Hermite node-pair
+    form, weights 2W/pi), evaluated in float32 like the kernel."""
+    f32 = np.float32
+    x, y = f32(x), f32(y)
+    x2, y2 = x * x, y * y
+    r = x2 + y2
+    sf = sp = sy = f32(0)
+    for t2, w in zip(t2s, ws):
+        t2, c = f32(t2), f32(w) * f32(2 / np.pi)
+        wv = r - t2
+        d = f32(4) * t2 * y2
+        rD = f32(1) / (wv * wv + d)
+        F = (wv + f32(2) * t2) * rD
+        Fp = (f32(4) * t2 * (t2 - x2) - wv * wv) * rD * rD
+        sf = sf + c * F
+        sp = sp + c * Fp
+        sy = sy + c * (F - f32(2) * d * F * rD)
+    return y * sf, f32(2) * x * y * sp, f32(2) * y2 * sp + sy
+
+
+def _kgrad_exact(x, y):
+    z = x + 1j * y
+    w = wofz(z)
+    wp = -2.0 * z * w + 2j / SQRT_PI
+    return w.real, wp.real, -wp.imag
+
+
+def test_six_node_gradient_outside_the_adjoint_core():
+    """Adjoint drain, first pass (adjoint.hip adj_core_r2): in FP32 the
+    six-node form's K is within 1e-6 of K and its dK/dx, dK/dy within 1e-6 of
+    |grad K| wherever |z| >= 5.5 (|z| >= 7 for y < 1e-4), for y from 1e-10 to
+    50 -- better than the four-node form from |z| = 8, where kgrad_mixed
+    switches to it (1.5e-6 here in FP32); and both radii are needed: below
+    y = 1e-4 the form leaves out exp(-x^2), which at |z| = 5.5 is 7% of K for
+    y = 1e-10, and at |z| = 5 it misses 1e-6 for y = 1e-3."""
+    rng = np.random.default_rng(31)
+    n = 1_000_000
+
+    def worst(t2s, ws, rad, ylo, yhi):
+        y = np.exp(rng.uniform(math.log(ylo), math.log(yhi), n))
+        r = rad * np.exp(rng.uniform(0.0, math.log(8.0), n))
+        r[: n // 4] = rad * (1 + 1e-6)
+        ok = r > y
+        yy, rr = y[ok], r[ok]
+        x = np.sqrt(rr * rr - yy * yy)
+        K, Kx, Ky = (v.astype(np.float64) for v in kgrad_pairs_f32(x, yy, t2s, ws))
+        E, Ex, Ey = _kgrad_exact(x, yy)
+        g = np.hypot(Ex, Ey)
+        return (np.max(np.abs(K / E - 1.0)),
+                np.max(np.maximum(np.abs(Kx - Ex), np.abs(Ky - Ey)) / g))
+
+    t4, w4 = np.polynomial.hermite.hermgauss(4)
+    assert max(worst(t4[t4 > 0] ** 2, w4[t4 > 0], 8.0, 1e-10, 50.0)) <= 1.5e-6
+    assert max(worst(GH6_T2, GH6_W, 7.0, 1e-10, 1e-4)) <= 1e-6
+    assert max(worst(GH6_T2, GH6_W, 5.5, 1e-4, 50.0)) <= 1e-6
+    assert worst(GH6_T2, GH6_W, 5.5, 1e-10, 1e-9)[0] > 1e-2
+    assert max(worst(GH6_T2, GH6_W, 5.0, 1e-3, 2.0)) > 1e-6
