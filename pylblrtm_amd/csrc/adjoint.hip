@@ -39,7 +39,8 @@ constexpr int kVjpFold = 32;
 // Far-field series of the adjoint: terms of the one-node form's r = 1/(x^2 + c1)
 // and r^2 (whose double pole converges more slowly): outside the near range
 // 22 terms leave <= 1.1e-9 of r and 1.8e-7 of r^2 (and of x r^2, formed as x
-// times the truncated r^2 series).
+// times the truncated r^2 series).  A record farther out needs fewer for the
+// same 1.8e-7 (adj_terms).
 constexpr int kAdjTerms = 22;
 constexpr int kAdjSliceExCap = 1024;
 constexpr int kAdjQueueCap = 256;   // near-centre (record, lane) pairs per wavefront per gas run
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
   // S1 = sum W r, S2 = sum W r^2, S3 = sum W x r^2 of one record from the
   // moments M (read from LDS as they are needed: registers are scarce here)
   auto series = [&](const float* M, float xm, float sh, float c1, float& S1, float& S2,
-                    float& S3) {
+                    float& S3, const int nt) {
     const float a0 = __builtin_amdgcn_rcpf(fmaf(xm, xm, c1));
     const float u = sh * a0;
     const float al = -2.0f * xm * u, be = -sh * u;
@@ -347,6 +348,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     float s3 = fmaf(bn, M[2], bp * M[1]);
 #pragma unroll
     for (int n = 2; n < kNT; ++n) {
+      if ((n == 6 || n == 8 || n == 11 || n == 15 || n == 19) && n >= nt) break;
       const float a_n = fmaf(al, an, be * ap);
       const float b_n = fmaf(a0, a_n, fmaf(al, bn, be * bp));
       s1 = fmaf(a_n, M[n], s1);
@@ -360,10 +362,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     S3 = fmaf(xm, s2, sh * s3);
   };
   // adds record li's series (middle point at offset d, half-span sh/s) to the gradients
-  auto far_record = [&](const float* M, const MixedRec& m, int64_t li, float d, float sh) {
+  auto far_record = [&](const float* M, const MixedRec& m, int64_t li, float d, float sh,
+                        const int nt) {
     const int64_t r = kb + li;
     float S1, S2, S3;
-    series(M, d * m.s, sh, m.c1, S1, S2, S3);
+    series(M, d * m.s, sh, m.c1, S1, S2, S3, nt);
     const float y = m.y, am = m.a;
     const float aq = am * (float)kInvSqrtPi, ay = m.ay;
     const float cty = a.dyt[r] * aq, cvy = a.dyv[r] * aq;
@@ -379,8 +382,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     for (int gg = 0; gg < 8; ++gg)
       if (gg == gas) g_v[gg] += (double)fv;
   };
+  // the term count of a round (wavefront-uniform): the fewest terms that keep
+  // r, r^2 and x r^2 within 1.8e-7 for every record the round sums, each at
+  // R = |nu_m - c| / Hu unit half-spans (>= 3): 22 terms from R = 3, 19 from
+  // 3.5, 15 from 5, 11 from 8, 8 from 14, 6 from 30
+  // (tests/test_far_series.py)
+  auto adj_terms = [&](bool use, float R) -> int {
+    if (__ballot(use && R < 3.5f)) return kNT;
+    if (__ballot(use && R < 5.0f)) return 19;
+    if (__ballot(use && R < 8.0f)) return 15;
+    if (__ballot(use && R < 14.0f)) return 11;
+    if (__ballot(use && R < 30.0f)) return 8;
+    return 6;
+  };
+  static_assert(kNT == 22, "adj_terms's table ends at 22 terms");
   // the one-node form holds at every point of the slice / of this wavefront
   const float Hw = wave_half_span32(a.step);   // the series' unit sh = s Hw
+  const float inv_hs = 0.25f / Hw;             // 1 / Hs
   auto slice_ok = [&](const MixedRec& m) { return fabsf((float)(nu_s - m.c)) >= m.ds1 + dS; };
   auto wave_ok = [&](const MixedRec& m) { return fabsf((float)(nu_w - m.c)) >= m.ds1; };
   const MixedRec* __restrict__ lrec = recs + kb;
@@ -388,13 +406,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     // slice level: records of the tile outside [slo, shi), one per thread
     const float* const M = s_msum;
     const int32_t n_below = slo - rlo, n_sl = n_below + (rhi - shi);
+    auto sl_index = [&](int32_t v) -> int64_t {
+      return v < n_below ? (int64_t)rlo + v : (int64_t)shi + (v - n_below);
+    };
+    // the next record is loaded one iteration ahead
+    MixedRec nx;
+    if (tid < n_sl) nx = lrec[sl_index(tid)];
     for (int32_t v = tid; v < n_sl; v += kBlock) {
-      const int64_t li = v < n_below ? (int64_t)rlo + v : (int64_t)shi + (v - n_below);
-      const MixedRec m = lrec[li];
-      if (!(m.c >= lo_b && m.c <= hi_b)) continue;   // outside the tile window (or skipped)
-      if (slice_ok(m)) {
-        far_record(M, m, li, (float)(nu_s - m.c), 4.0f * (m.s * Hw));
-      } else {
+      const int64_t li = sl_index(v);
+      const MixedRec m = nx;
+      if (v + kBlock < n_sl) nx = lrec[sl_index(v + kBlock)];
+      // (outside the tile window, or skipped: nothing)
+      const bool inw = m.c >= lo_b && m.c <= hi_b;
+      const float d = (float)(nu_s - m.c);
+      const bool ser = inw && slice_ok(m);
+      const int nt = adj_terms(ser, fabsf(d) * inv_hs);
+      if (ser) {
+        far_record(M, m, li, d, 4.0f * (m.s * Hw), nt);
+      } else if (inw) {
         const int pos = atomicAdd(&s_sxn, 1);
         if (pos < kAdjSliceExCap) s_sx[pos] = (int32_t)li;
       }
@@ -422,7 +451,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
       const MixedRec m = lrec[li];
       if (!(m.c >= lo_b && m.c <= hi_b)) continue;
       if (rescan && v >= nA + nB && slice_ok(m)) continue;   // the slice level summed it
-      if (wave_ok(m)) far_record(M, m, li, (float)(nu_w - m.c), m.s * Hw);
+      if (wave_ok(m)) far_record(M, m, li, (float)(nu_w - m.c), m.s * Hw, kNT);
     }
   }
 

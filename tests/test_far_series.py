@@ -238,6 +238,49 @@ def test_adjoint_series_22_terms(y):
     assert w3 <= 1.8e-7, w3
 
 
+# The adjoint's slice level sums only the terms each round's nearest record
+# needs (adjoint.hip adj_terms): each (R, terms) row keeps r, r^2 and x r^2
+# within 1.9e-7 for every record at R >= the row's unit half-spans where the
+# one-node form holds (the 22-term claim's 1.8e-7 for r^2; this sweep
+# measures 1.803e-7 for 22 terms at R = 3, and every other row <= 1.5e-7).
+TERMS_BY_RATIO_ADJ = [(3.0, 22), (3.5, 19), (5.0, 15), (8.0, 11), (14.0, 8), (30.0, 6)]
+
+
+def _adjoint_worst(ratio, terms, ys):
+    w = 0.0
+    for y in ys:
+        y2 = y * y
+        c1 = y2 - 1.5
+        x_s1 = math.sqrt(max(s1_r2_min(y2) - y2, 0.0))
+        for sh in np.geomspace(0.01, 3000.0, 31):
+            edge = max(ratio * sh, sh + x_s1)
+            for xm0 in (edge, 1.3 * edge, 3.0 * edge):
+                for xm in (xm0, -xm0):
+                    a, b = adjoint_coeffs(xm, sh, c1, terms=terms)
+                    x = xm + sh * TAU
+                    r = 1.0 / (x * x + c1)
+                    bx = xm * b
+                    bx[1:] += sh * b[:-1]
+                    xr2 = poly(bx, TAU) + sh * b[-1] * TAU ** terms
+                    w = max(w, float(np.max(np.abs(poly(a, TAU) / r - 1.0))),
+                            float(np.max(np.abs(poly(b, TAU) / (r * r) - 1.0))),
+                            float(np.max(np.abs(xr2 / (x * r * r) - 1.0))))
+    return w
+
+
+@pytest.mark.parametrize("ratio,terms", TERMS_BY_RATIO_ADJ)
+def test_adjoint_series_terms_by_distance(ratio, terms):
+    w = _adjoint_worst(ratio, terms, (1e-4, 0.5, 1.2247, 1.3, 3.0, 10.0, 40.0, 150.0))
+    assert w <= 1.9e-7, w
+
+
+def test_adjoint_fewer_terms_miss_the_bound():
+    # two terms fewer at each row's nearest distance misses 1.9e-7 for some record
+    for ratio, terms in TERMS_BY_RATIO_ADJ:
+        w = _adjoint_worst(ratio, terms - 2, (1e-4, 1.3, 40.0))
+        assert w > 1.9e-7, (ratio, terms, w)
+
+
 # ---- the mixed group level's middle-tier form (k_far_series_mixed
 # series_mid): where the one-node form fails at a 64-point group but the
 # middle tier holds at all its points, the form (ay u + C) / (u^2 + B),
