@@ -160,14 +160,17 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
         # four-node levels together
         far_flop = (22 * kw["slice_series"] + 8 * kw["series_terms"]
                     + 44 * (kw["wave_series"] + kw.get("group_series", 0))
-                    + 16 * kw.get("series_terms_wave", 0)
+                    + 16 * kw.get("series_terms_2", 0)
                     + kw["exception_evals"] * FLOP_PER_EVAL)
     if base == "mixed" and kw.get("series_terms"):
         # the mixed kernel sums only the terms each round's nearest record
         # needs: a one-node pair costs its 12-FLOP set-up plus 4 per term
-        # after the first two (76 at 18 terms)
+        # after the first two (76 at 18 terms), a middle-tier pair 16 plus 8
+        # per term (160 at 18; the kernel counts its terms in slot 15)
         one_node = kw["slice_series"] + kw["wave_series"] + kw.get("group_series", 0) - g_mid
-        far_flop = (4 * one_node + 4 * kw["series_terms"] + g_mid * fp["group_middle"]
+        mid_flop = (16 * g_mid + 8 * kw["series_terms_2"] if kw.get("series_terms_2")
+                    else g_mid * fp["group_middle"])
+        far_flop = (4 * one_node + 4 * kw["series_terms"] + mid_flop
                     + kw["exception_evals"] * FLOP_PER_EVAL)
     kern = {
         f"k_far_series_{base}": {
@@ -178,7 +181,8 @@ def kernel_rooflines(precision, split_ms, kw, traffic):
             "exception_point_evals": kw["exception_evals"],
             "flop_per_unit": far_flop / max(pairs, 1),
             "series_terms_summed": kw.get("series_terms"),
-            "series_terms_summed_wave_level": kw.get("series_terms_wave"),
+            ("series_terms_summed_four_node_levels" if base == "fp64"
+             else "series_terms_summed_middle_form"): kw.get("series_terms_2"),
             "flop_per_pair": dict(fp)},
         near_name: {"ms": near_ms, "units": kw["near_evals"], "unit": "point evaluations",
                     "flop_per_unit": FLOP_PER_EVAL, "queued_near_centre_pairs": kw["queued"],

@@ -293,14 +293,15 @@ int lbl_last_timing_epilogue(double *ms_epilogue);
  *   [7] queued core points (LBL_MIXED)    [8] slice-list overflows
  *   [9] wavefront-level candidates taken from the slice level's list [3]
  *       (one per record and live wavefront of its slice)
- *   LBL_MIXED, group level (records of a wavefront's near range outside a
- *   group's own): [10] (record, group) series pairs, [11] pairs left to the
- *   near field (neither the one-node nor the middle-tier form holds at every
- *   point of the group), [12] the near field's point evaluations of [11]
- *   (part of [0]), [13] the pairs of [10] summed in the middle-tier form
+ *   group level (records of a wavefront's near range outside a group's
+ *   own): [10] (record, group) series pairs, [11] pairs left to the near
+ *   field (no series form of the level holds at every point of the group),
+ *   [12] the near field's point evaluations of [11] (part of [0]),
+ *   [13] LBL_MIXED: the pairs of [10] summed in the middle-tier form
  *   [14] series terms summed (each pair takes the terms its distance
  *   needs): LBL_MIXED over the one-node pairs of [1], [2] and [10], LBL_FP64
- *   over the pairs of [1]   [15] LBL_FP64: terms of each pair of [2]
+ *   over the pairs of [1]   [15] LBL_MIXED: over the pairs of [13];
+ *   LBL_FP64: over the pairs of [2] and [10] (each pole pair's terms once)
  * All counts are 0 after a call made without counting. */
 void lbl_enable_work_count(int on);
 int lbl_last_work_count(uint64_t *counts, int32_t n);

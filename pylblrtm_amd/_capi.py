@@ -172,7 +172,7 @@ def grid_from_nu(nu, tile_width: float, cutoff: float) -> LblGrid:
 WORK_COUNT_NAMES = ("near_evals", "slice_series", "wave_series", "slice_exceptions",
                     "wave_exceptions", "exception_evals", "queued", "core_points", "rescans",
                     "wave_from_slice_list", "group_series", "group_deferred", "group_deferred_evals",
-                    "group_series_middle", "series_terms", "series_terms_wave")
+                    "group_series_middle", "series_terms", "series_terms_2")
 
 
 def enable_work_count(on: bool = True) -> None:

@@ -319,7 +319,8 @@ enum WorkCount {
   kWorkGroupMiddle,         // group series pairs in the middle-tier form (part of kWorkGroupSeries)
   kWorkSeriesTerms,         // terms summed: mixed, over the one-node pairs of all levels;
                             // FP64, over the slice level's pairs
-  kWorkSeriesTerms2,        // FP64: terms of each wavefront-level pair (both pole pairs)
+  kWorkSeriesTerms2,        // FP64: terms of each wavefront- and group-level pair (both pole
+                            // pairs); mixed: terms of the middle-tier group pairs
   kWorkSlots
 };
 

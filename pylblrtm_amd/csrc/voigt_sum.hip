@@ -1096,8 +1096,10 @@ void k_far_series_mixed(const SumArgs a) {
   // d4 Q_(n-4)) / d0 (N_n = 0 for n > 2).  Used at the group level only,
   // where the record is at least kSeriesRatio Hu away and the form holds at
   // every point: the poles then lie at |tau| >= 2.69 and kN terms keep the
-  // truncation <= 1e-7 (FP32 arithmetic adds <= 1.1e-6 of the value).
-  auto series_mid = [&](float (&C)[kN], const FarRec& m, float d, float sh) {
+  // truncation <= 1e-7 (FP32 arithmetic adds <= 1.1e-6 of the value); a
+  // round whose nearest record is farther sums fewer (nt, series_terms32's
+  // table, which holds for this form too).
+  auto series_mid = [&](float (&C)[kN], const FarRec& m, float d, float sh, int nt) {
     const float xm = d * m.s;
     const bool s2 = m.c1 < kF32S2Y2 - 1.5f;
     const float A = s2 ? m.c1 + 0.75f : m.c1 + 1.0f;
@@ -1117,6 +1119,7 @@ void k_far_series_mixed(const SumArgs a) {
     C[3] += q3;
 #pragma unroll
     for (int n = 4; n < kN; ++n) {
+      if ((n == 8 || n == 10 || n == 12 || n == 14) && n >= nt) break;
       const float qn = -fmaf(e1, q3, fmaf(e2, q2, fmaf(e3, q1, e4 * q0)));
       C[n] += qn;
       q0 = q1;
@@ -1128,8 +1131,9 @@ void k_far_series_mixed(const SumArgs a) {
   if (threadIdx.x == 0) s_sxn = 0;
   __syncthreads();
 
-  // kCount: terms summed by this lane's one-node series (the FLOP they take)
-  uint64_t n_terms = 0;
+  // kCount: terms summed by this lane's one-node series and by its
+  // middle-tier group series (the FLOP they take)
+  uint64_t n_terms = 0, n_terms2 = 0;
   auto count_terms = [&]() {
     if constexpr (kCount) {
 #pragma unroll
@@ -1414,7 +1418,11 @@ void k_far_series_mixed(const SumArgs a) {
       if (ser) series(G, m, d, 0.125f * (m.s * Hw), nt);
       if constexpr (kCount) n_terms += ser ? (uint64_t)nt : 0u;
       const bool mid = act && !valid && group_one_node_holds(d0, group_mid_thr(gt, m.dm), goff);
-      if (mid) series_mid(G, m, d, 0.125f * (m.s * Hw));
+      if (__ballot(mid)) {   // uniform
+        const int ntm = series_terms32(mid, fabsf(d) * (8.0f * inv_hw));
+        if (mid) series_mid(G, m, d, 0.125f * (m.s * Hw), ntm);
+        if constexpr (kCount) n_terms2 += mid ? (uint64_t)ntm : 0u;
+      }
       const bool left = act && !valid && !mid;
       n_gdef += left ? 1u : 0u;
       if constexpr (kCount) {
@@ -1480,10 +1488,12 @@ void k_far_series_mixed(const SumArgs a) {
       n_gser += __shfl_xor(n_gser, off);
       n_gmid += __shfl_xor(n_gmid, off);
       n_gdef_pts += __shfl_xor(n_gdef_pts, off);
+      n_terms2 += __shfl_xor(n_terms2, off);
     }
     if (lane == 0) {
       count_add(a.work, kWorkGroupSeries, n_gser);
       count_add(a.work, kWorkGroupMiddle, n_gmid);
+      count_add(a.work, kWorkSeriesTerms2, n_terms2);
     }
     count_terms();
     if (lane == 0) {
