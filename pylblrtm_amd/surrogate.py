@@ -9,7 +9,7 @@ K_ref = exp(-x^2)/(1 + sqrt(pi) y) + y/(sqrt(pi)(x^2 + y^2 + 1/2)).
 
 There is no counterpart in the reference (its Voigt is scipy.special.wofz,
 pylblrtm/forward.py:62); the exact device Faddeeva function stays the
-fallback and the judge of the surrogate's error.
+fallback and the yardstick of the surrogate's error.
 """
 
 from __future__ import annotations
