@@ -211,68 +211,51 @@ __global__ void k_tile_ranges(const TileRangeArgs a) {
 
 }  // namespace
 
-// One thread per (workgroup slice, wavefront): the records whose nu0 lies in
-// [nu_w - half_width, nu_w + half_width] (lower and upper bound in the
-// sorted nu0), nu_w the wavefront's middle point as the line sum forms it.
+// Near ranges of every wavefront, slice and 64-point group: the records
+// whose nu0 lies in [nu_m - half width, nu_m + half width] (lower and upper
+// bound in the sorted nu0), nu_m the unit's middle point as the line sums
+// form it.  One thread per (wavefront, range): ranges 0..kGroupsPerWave-1
+// are the wavefront's groups', the last its own (and, for its slice's first
+// wavefront, the slice's); each thread's binary searches are independent,
+// so their dependent loads overlap across threads.
 __global__ void k_wave_ranges(const WaveRangeArgs a) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.n_wg * a.waves_per_slice) return;
+  constexpr int kParts = kGroupsPerWave + 1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_wg * a.waves_per_slice * kParts) return;
+  const int64_t g = t / kParts;
+  const int part = (int)(t - g * kParts);
   const int64_t wg = g / a.waves_per_slice;
   const int w = (int)(g - wg * a.waves_per_slice);
   const int64_t tile = wg / a.slices_per_tile;
   const int64_t slice = wg - tile * a.slices_per_tile;
   const int64_t first =
       tile * a.ts + (slice * a.waves_per_slice + w) * (int64_t)kWavePoints;
+  // [lower_bound(lo_v), upper_bound(hi_v)) of the nu0-sorted list
+  auto range = [&](double lo_v, double hi_v, int32_t* out_lo, int32_t* out_hi) {
+    int64_t lo = 0, hi = a.n_lines;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (a.nu0[mid] < lo_v) lo = mid + 1; else hi = mid;
+    }
+    *out_lo = (int32_t)lo;
+    hi = a.n_lines;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (a.nu0[mid] <= hi_v) lo = mid + 1; else hi = mid;
+    }
+    *out_hi = (int32_t)lo;
+  };
+  if (part < kGroupsPerWave) {   // a group of the wavefront
+    const double nu_g = group_middle(a.nu_min, a.step, first + (int64_t)part * kGroupPoints);
+    range(nu_g - a.group_half_width, nu_g + a.group_half_width,
+          a.glo + g * kGroupsPerWave + part, a.ghi + g * kGroupsPerWave + part);
+    return;
+  }
   const double nu_w = wave_middle(a.nu_min, a.step, first);
-  const double lo_v = nu_w - a.half_width, hi_v = nu_w + a.half_width;
-  int64_t lo = 0, hi = a.n_lines;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (a.nu0[mid] < lo_v) lo = mid + 1; else hi = mid;
-  }
-  const int64_t first_rec = lo;
-  hi = a.n_lines;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (a.nu0[mid] <= hi_v) lo = mid + 1; else hi = mid;
-  }
-  a.wlo[g] = (int32_t)first_rec;
-  a.whi[g] = (int32_t)lo;
+  range(nu_w - a.half_width, nu_w + a.half_width, a.wlo + g, a.whi + g);
   if (w == 0) {   // the slice's near range
     const double nu_s = slice_middle(a.nu_min, a.step, tile * a.ts + slice * (int64_t)kSlicePoints);
-    const double lo_s = nu_s - a.slice_half_width, hi_s = nu_s + a.slice_half_width;
-    int64_t l = 0, h = a.n_lines;
-    while (l < h) {
-      const int64_t mid = (l + h) >> 1;
-      if (a.nu0[mid] < lo_s) l = mid + 1; else h = mid;
-    }
-    const int64_t f = l;
-    h = a.n_lines;
-    while (l < h) {
-      const int64_t mid = (l + h) >> 1;
-      if (a.nu0[mid] <= hi_s) l = mid + 1; else h = mid;
-    }
-    a.slo[wg] = (int32_t)f;
-    a.shi[wg] = (int32_t)l;
-  }
-  // the wavefront's groups (records are sorted by nu0, and the groups' ranges
-  // move up with their middles)
-  for (int grp = 0; grp < kGroupsPerWave; ++grp) {
-    const double nu_g = group_middle(a.nu_min, a.step, first + (int64_t)grp * kGroupPoints);
-    const double lo_g = nu_g - a.group_half_width, hi_g = nu_g + a.group_half_width;
-    int64_t l = 0, h = a.n_lines;
-    while (l < h) {
-      const int64_t mid = (l + h) >> 1;
-      if (a.nu0[mid] < lo_g) l = mid + 1; else h = mid;
-    }
-    const int64_t f = l;
-    h = a.n_lines;
-    while (l < h) {
-      const int64_t mid = (l + h) >> 1;
-      if (a.nu0[mid] <= hi_g) l = mid + 1; else h = mid;
-    }
-    a.glo[g * kGroupsPerWave + grp] = (int32_t)f;
-    a.ghi[g * kGroupsPerWave + grp] = (int32_t)l;
+    range(nu_s - a.slice_half_width, nu_s + a.slice_half_width, a.slo + wg, a.shi + wg);
   }
 }
 
@@ -303,7 +286,7 @@ hipError_t launch_tile_ranges(const TileRangeArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_wave_ranges(const WaveRangeArgs& a, hipStream_t s) {
-  const int64_t n = a.n_wg * a.waves_per_slice;
+  const int64_t n = a.n_wg * a.waves_per_slice * (kGroupsPerWave + 1);
   if (n <= 0) return hipSuccess;
   k_wave_ranges<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(a);
   return hipGetLastError();
